@@ -1,0 +1,235 @@
+// Internal host-side definitions shared by every translation unit of
+// libgalois_amd.so: the device graph object, error plumbing across the C ABI,
+// the per-graph scratch cache and the HIP-event profiling helper.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/galois_amd.h"
+
+namespace gx {
+
+// ---------------------------------------------------------------- errors
+// Internal code throws gx::Error; every extern "C" entry point catches it at
+// the boundary (GX_API_BEGIN/END), records the message and returns the code.
+struct Error : std::runtime_error {
+  gx_status code;
+  Error(gx_status c, const std::string& msg) : std::runtime_error(msg), code(c) {}
+};
+
+void set_last_error(const std::string& msg);
+
+[[noreturn]] inline void fail(gx_status c, const std::string& msg) { throw Error(c, msg); }
+
+inline void hip_check(hipError_t e, const char* what, const char* file, int line) {
+  if (e != hipSuccess) {
+    fail(e == hipErrorOutOfMemory ? GX_ERR_ALLOC : GX_ERR_HIP,
+         std::string(what) + ": " + hipGetErrorString(e) + " (" + file + ":" + std::to_string(line) + ")");
+  }
+}
+#define GX_HIP(expr) ::gx::hip_check((expr), #expr, __FILE__, __LINE__)
+// Launch-error check after every kernel launch.
+#define GX_LAUNCHED(name) ::gx::hip_check(hipGetLastError(), name, __FILE__, __LINE__)
+
+#define GX_REQUIRE(cond, msg)                        \
+  do {                                               \
+    if (!(cond)) ::gx::fail(GX_ERR_INVALID, (msg)); \
+  } while (0)
+
+#define GX_API_BEGIN try {
+#define GX_API_END                                     \
+  }                                                    \
+  catch (const ::gx::Error& e) {                       \
+    ::gx::set_last_error(e.what());                    \
+    return e.code;                                     \
+  }                                                    \
+  catch (const std::bad_alloc&) {                      \
+    ::gx::set_last_error("host allocation failed");    \
+    return GX_ERR_ALLOC;                               \
+  }                                                    \
+  catch (const std::exception& e) {                    \
+    ::gx::set_last_error(e.what());                    \
+    return GX_ERR_INVALID;                             \
+  }                                                    \
+  return GX_OK;
+
+// ------------------------------------------------------------ device memory
+// Owning device buffer (hipMalloc/hipFree), move-only.
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  DevBuf(DevBuf&& o) noexcept : p(o.p), bytes(o.bytes) { o.p = nullptr; o.bytes = 0; }
+  DevBuf& operator=(DevBuf&& o) noexcept {
+    if (this != &o) { release(); p = o.p; bytes = o.bytes; o.p = nullptr; o.bytes = 0; }
+    return *this;
+  }
+  ~DevBuf() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  // Grow to at least `need` bytes (contents are not preserved).
+  void ensure(size_t need) {
+    if (need <= bytes) return;
+    release();
+    size_t b = need < 256 ? 256 : need;
+    GX_HIP(hipMalloc(&p, b));
+    bytes = b;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Scratch slots a graph keeps between calls so that repeated runs allocate
+// nothing.  Each algorithm uses its own slots.
+enum Slot : int {
+  WS_Q0 = 0, WS_Q1, WS_Q2, WS_Q3, WS_COUNTERS, WS_BITMAP0, WS_BITMAP1,
+  WS_V0, WS_V1, WS_V2, WS_V3, WS_V4, WS_V5, WS_REDUCE, WS_BC_LEVEL, WS_BC_SIGMA,
+  WS_BC_DELTA, WS_BC_MASK0, WS_BC_MASK1, WS_BC_MASK2, WS_HOST_PINNED,
+  WS_NUM_SLOTS
+};
+
+// Tables of the pull PageRank sweep over the transpose, built once per graph.
+struct PullPlan {
+  bool ready = false;
+  int64_t nwin = 0, nlong = 0, npieces = 0;
+  DevBuf buf;
+};
+
+}  // namespace gx
+
+// The opaque graph handle of the C ABI.
+struct gx_graph {
+  int64_t n = 0, m = 0;
+  uint32_t flags = 0;
+  // CSR (out-edges)
+  int32_t* row = nullptr;
+  int32_t* col = nullptr;
+  void* w = nullptr;  // uint32 or float weights (m) or null
+  bool owns_csr = true;
+  gx::DevBuf row_buf, col_buf, w_buf;
+  // CSC (in-edges), built on demand; tperm maps CSC slot -> CSR edge id.
+  gx::DevBuf trow_buf, tcol_buf, tperm_buf;
+  bool has_transpose = false;
+  // Derived per-vertex data, built on demand.
+  gx::DevBuf outdeg_buf;  // uint32 out-degree (for transpose-based PR)
+  bool has_outdeg = false;
+  gx::PullPlan pull_plan;
+  hipStream_t stream = nullptr;
+  gx::DevBuf ws[gx::WS_NUM_SLOTS];
+  void* pinned = nullptr;  // small pinned host block for counters
+  size_t pinned_bytes = 0;
+  ~gx_graph();
+
+  int32_t* trow() const { return trow_buf.as<int32_t>(); }
+  int32_t* tcol() const { return tcol_buf.as<int32_t>(); }
+  int32_t* tperm() const { return tperm_buf.as<int32_t>(); }
+  template <class T> T* scratch(int slot, size_t count) {
+    ws[slot].ensure(count * sizeof(T));
+    return ws[slot].as<T>();
+  }
+  // Pinned host block of at least `bytes` for small device->host reads.
+  void* host_pinned(size_t bytes);
+};
+
+namespace gx {
+
+bool profiling_enabled();
+
+// Accumulates the device time of the launches of one kernel with HIP events
+// (enabled by gx_set_profiling).  Events are recorded on the graph's stream,
+// the stream the kernel is launched on.
+struct KernelTimer {
+  hipStream_t s;
+  bool on;
+  std::vector<hipEvent_t> evs;
+  explicit KernelTimer(hipStream_t st) : s(st), on(profiling_enabled()) {}
+  ~KernelTimer() {
+    for (auto e : evs) (void)hipEventDestroy(e);
+  }
+  void begin() {
+    if (!on) return;
+    hipEvent_t e;
+    GX_HIP(hipEventCreate(&e));
+    GX_HIP(hipEventRecord(e, s));
+    evs.push_back(e);
+  }
+  void end() { begin(); }
+  // Sum of (end - begin) over recorded pairs; the stream must be synchronised.
+  double total_ms() {
+    double t = 0;
+    for (size_t i = 0; i + 1 < evs.size(); i += 2) {
+      float ms = 0;
+      GX_HIP(hipEventElapsedTime(&ms, evs[i], evs[i + 1]));
+      t += ms;
+    }
+    return t;
+  }
+  uint32_t count() const { return static_cast<uint32_t>(evs.size() / 2); }
+};
+
+// Wall-clock of a whole algorithm region on the graph's stream.
+struct RegionTimer {
+  hipStream_t s;
+  hipEvent_t a = nullptr, b = nullptr;
+  explicit RegionTimer(hipStream_t st) : s(st) {
+    GX_HIP(hipEventCreate(&a));
+    GX_HIP(hipEventCreate(&b));
+    GX_HIP(hipEventRecord(a, s));
+  }
+  ~RegionTimer() {
+    if (a) (void)hipEventDestroy(a);
+    if (b) (void)hipEventDestroy(b);
+  }
+  double stop() {
+    GX_HIP(hipEventRecord(b, s));
+    GX_HIP(hipEventSynchronize(b));
+    float ms = 0;
+    GX_HIP(hipEventElapsedTime(&ms, a, b));
+    return ms;
+  }
+};
+
+inline void check_graph(const gx_graph* g) {
+  GX_REQUIRE(g != nullptr, "graph handle is NULL");
+}
+inline void check_vertex(const gx_graph* g, uint32_t v, const char* what) {
+  if (static_cast<int64_t>(v) >= g->n)
+    fail(GX_ERR_INVALID, std::string(what) + " " + std::to_string(v) + " out of range (n = " +
+                             std::to_string(g->n) + ")");
+}
+inline void check_ptr(const void* p, const char* what) {
+  if (!p) fail(GX_ERR_INVALID, std::string(what) + " is NULL");
+}
+
+inline unsigned grid_for(int64_t items, int block, int64_t max_blocks = 1 << 20) {
+  int64_t b = (items + block - 1) / block;
+  if (b < 1) b = 1;
+  if (b > max_blocks) b = max_blocks;
+  return static_cast<unsigned>(b);
+}
+
+// Device-side helpers exported by graph_dev.hip for other translation units.
+void ensure_outdeg(gx_graph* g);    // g->outdeg_buf = out-degree per vertex
+void build_transpose(gx_graph* g);  // CSC + permutation (deterministic)
+// Exclusive prefix sum of `count` uint32 values into uint32 `out` (may alias).
+void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s);
+// Device->host copy of a few bytes through the graph's pinned block.
+template <class T> T read_scalar(gx_graph* g, const T* dev) {
+  T* h = static_cast<T*>(g->host_pinned(sizeof(T)));
+  GX_HIP(hipMemcpyAsync(h, dev, sizeof(T), hipMemcpyDeviceToHost, g->stream));
+  GX_HIP(hipStreamSynchronize(g->stream));
+  return *h;
+}
+void read_bytes(gx_graph* g, void* host, const void* dev, size_t bytes);
+
+}  // namespace gx

@@ -1,0 +1,281 @@
+/*
+ * galois_amd.h — C ABI of the MI355X (gfx950) lonestar analytics library.
+ *
+ * Plain C: integer status codes, plain pointers and sizes, no C++ or torch
+ * types.  Every algorithm entry point takes a device graph (gx_graph) and
+ * writes its result into a caller-owned DEVICE buffer; the library owns its
+ * scratch (frontiers, buckets, per-source state) inside the gx_graph and
+ * frees it with the graph.
+ *
+ * Each entry point names the reference interface it replaces
+ * (paths relative to the Galois source tree).  The reference's GPU path is the
+ * LonestarGPU "skelapp" link-time ABI (libgpu/src/skelapp/skel.cu:12-32:
+ * gg_main(CSRGraphTy& hg, CSRGraphTy& gg), output(), prog_opts) plus the
+ * D-IrGL C-style personality kernels (e.g.
+ * lonestar/analytics/distributed/k-core/kcore_push_cuda.h:1-80); the CPU path
+ * is the lonestar app bodies under lonestar/analytics/cpu/.
+ *
+ * Conventions
+ *  - Graph layout: CSR with int32 row offsets (n+1, row[0] = 0) and int32
+ *    column indices (m); optional uint32 edge weights (m).  Vertex ids are
+ *    0..n-1.  m must be < 2^31.
+ *  - Distances/levels are uint32 with the reference infinity
+ *    GX_DIST_INFINITY = UINT32_MAX/2 - 1 = 2147483646
+ *    (lonestar/liblonestar/include/Lonestar/BFS_SSSP.h:31-32).
+ *  - Calls on one gx_graph are not thread-safe; calls on different graphs are.
+ *  - All work is enqueued on the graph's stream (gx_graph_set_stream; default
+ *    the null stream) and every call returns after that stream has finished
+ *    the work, so outputs are ready when the call returns.
+ *  - Errors never exit(): a non-zero gx_status is returned and
+ *    gx_last_error() holds a message (thread-local).
+ */
+#ifndef GALOIS_AMD_H
+#define GALOIS_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GX_DIST_INFINITY 2147483646u /* BFS_SSSP.h:31-32 */
+#define GX_BC_INFINITY 1073741823u   /* BetweennessCentrality.cpp:32 */
+
+typedef enum gx_status {
+  GX_OK = 0,
+  GX_ERR_INVALID = 1,  /* bad argument: null pointer, size, out-of-range id */
+  GX_ERR_HIP = 2,      /* a HIP runtime call or kernel launch failed       */
+  GX_ERR_ALLOC = 3,    /* device or host allocation failed                 */
+  GX_ERR_IO = 4,       /* file open/read/write or .gr format error         */
+  GX_ERR_CAPACITY = 5, /* an internal queue bound would be exceeded        */
+  GX_ERR_NO_DEVICE = 6 /* no usable gfx950 device                          */
+} gx_status;
+
+/* Graph flags. */
+#define GX_GRAPH_SYMMETRIC 1u /* every edge (u,v) has its reverse (v,u)      */
+#define GX_GRAPH_WEIGHTED 2u  /* uint32 weights present                      */
+#define GX_GRAPH_FWEIGHTED 4u /* float32 weights present (adjoint PageRank)   */
+
+typedef struct gx_graph gx_graph;
+
+/* Per-call statistics (every field optional to read; pass NULL to skip). */
+typedef struct gx_stats {
+  uint32_t iterations;      /* levels / bucket phases / rounds / PR iterations */
+  uint32_t launches;        /* kernel launches issued by the call              */
+  uint64_t edges_visited;   /* edges scanned (the GTEPS numerator)             */
+  uint64_t items;           /* vertices processed from worklists               */
+  double residual;          /* PageRank: final L1 residual, computed in FP64   */
+  double elapsed_ms;        /* device time of the algorithm region (hipEvent)  */
+  double main_kernel_ms;    /* summed device time of the dominant kernel, when
+                               gx_set_profiling(1) is on (else 0)             */
+  uint32_t main_kernel_launches; /* launches of the dominant kernel          */
+  uint32_t reserved;
+} gx_stats;
+
+/* Host-side CSR as read from / written to a .gr file. */
+typedef struct gx_host_csr {
+  int64_t n, m;
+  int32_t* row;        /* n+1 */
+  int32_t* col;        /* m */
+  void* edge_data;     /* m * edge_size bytes, or NULL */
+  uint32_t edge_size;  /* bytes per edge datum (0 = none, 4 = uint32/float) */
+  uint32_t version;    /* .gr version read (1 or 2); written files use 1 */
+} gx_host_csr;
+
+/* ---------------------------------------------------------------- errors */
+
+/* Copy the calling thread's last error message into buf (always
+ * NUL-terminated).  Returns the full message length. */
+size_t gx_last_error(char* buf, size_t len);
+
+/* Library/device query: 0 if a gfx950 device is usable. */
+gx_status gx_init(int device);
+/* When on, entry points time every launch of their dominant kernel with HIP
+ * events and report it in gx_stats.main_kernel_ms.  Off by default. */
+void gx_set_profiling(int on);
+
+/* ---------------------------------------------------------- graph lifecycle
+ * Replaces CSRGraph::copy_to_gpu (libgpu/src/csr_graph.cu:274-293) and the
+ * skelapp ownership of hg/gg (libgpu/src/skelapp/skel.cu:45-75). */
+
+/* Copy a host CSR to the device (library-owned copy). */
+gx_status gx_graph_create(int64_t n, int64_t m, const int32_t* row,
+                          const int32_t* col, const void* weights,
+                          uint32_t flags, gx_graph** out);
+/* Wrap caller-owned device arrays (borrowed: not freed by gx_graph_free). */
+gx_status gx_graph_create_device(int64_t n, int64_t m, int32_t* row_dev,
+                                 int32_t* col_dev, void* weights_dev,
+                                 uint32_t flags, gx_graph** out);
+gx_status gx_graph_free(gx_graph* g);
+gx_status gx_graph_info(const gx_graph* g, int64_t* n, int64_t* m,
+                        uint32_t* flags);
+/* hipStream_t passed as void* to keep HIP types out of the ABI. */
+gx_status gx_graph_set_stream(gx_graph* g, void* stream);
+/* Device pointers of the CSR / CSC arrays (any out-pointer may be NULL). */
+gx_status gx_graph_device_arrays(const gx_graph* g, const int32_t** row,
+                                 const int32_t** col, const void** weights,
+                                 const int32_t** trow, const int32_t** tcol,
+                                 const int32_t** tperm);
+/* Copy the CSR back to host arrays sized n+1 / m / m*4 (any may be NULL). */
+gx_status gx_graph_download(const gx_graph* g, int32_t* row, int32_t* col,
+                            void* weights);
+/* Copy the CSC (transpose) back: trow n+1, tcol m, tperm m (CSC slot -> CSR
+ * edge index).  Builds it first if absent. */
+gx_status gx_graph_download_transpose(gx_graph* g, int32_t* trow,
+                                      int32_t* tcol, int32_t* tperm);
+
+/* Build the transposed graph (CSC: in-edges) on the device.  Within each
+ * column, sources are in ascending order and ties keep CSR edge order, so the
+ * result is deterministic; equals graph-convert -transpose
+ * (tools/graph-convert/graph-convert.cpp:1831-1880) and replaces
+ * LC_CSR_CSC_Graph::constructIncomingEdges
+ * (libgalois/include/galois/graphs/LC_CSR_CSC_Graph.h:140-244). */
+gx_status gx_graph_build_transpose(gx_graph* g);
+
+/* ------------------------------------------------------------- .gr files
+ * Replaces FileGraph::fromMem (libgalois/src/FileGraph.cpp:44-53,104-141)
+ * and CSRGraph::readFromGR (libgpu/src/csr_graph.cu:172-261). */
+gx_status gx_gr_read(const char* path, gx_host_csr* out);
+gx_status gx_gr_write(const char* path, const gx_host_csr* g);
+void gx_host_csr_free(gx_host_csr* g);
+
+/* ------------------------------------------------------------ generators
+ * Seeded, counter-based (identical on every host and on the device).
+ * RMAT: scale, edge_factor, (a,b,c) with d = 1-a-b-c; self-loops and
+ * duplicate edges removed; symmetric = add the reverse of every edge.
+ * weight_kind: 0 none, 1 uint32 uniform [wlo,whi] per undirected pair,
+ * 2 float32 lognormal(0,1) per directed edge. */
+gx_status gx_gen_rmat(uint32_t scale, uint32_t edge_factor, double a, double b,
+                      double c, uint64_t seed, int symmetric, int weight_kind,
+                      uint32_t wlo, uint32_t whi, gx_graph** out);
+/* Road-like grid: side x side lattice, 4-neighbour edges, each undirected
+ * edge dropped with probability drop_ppm/1e6, plus shortcut_ppm/1e6 * edges
+ * random long-range undirected shortcuts; symmetric; uint32 weights
+ * uniform [wlo,whi]. */
+gx_status gx_gen_grid(uint32_t side, uint32_t drop_ppm, uint32_t shortcut_ppm,
+                      uint64_t seed, uint32_t wlo, uint32_t whi,
+                      gx_graph** out);
+/* Disjoint union of graphs (vertex ids of graph i offset by the vertex count
+ * of graphs 0..i-1); the result owns copies.  Used to run many uneven graphs
+ * in one launch. */
+gx_status gx_graph_union(gx_graph* const* parts, uint32_t count,
+                         gx_graph** out, int64_t* vertex_offsets /* count+1 */);
+
+/* Host copy of the source picker of lonestar/liblonestar/include/Lonestar/
+ * Utils.h:12-31 (mt19937 seeded 27491095, uniform over [0,n-1], rejecting
+ * degree-0 vertices).  row is the host CSR row array. */
+gx_status gx_pick_sources(int64_t n, const int32_t* row, uint32_t count,
+                          uint32_t seed, uint32_t* out);
+
+/* ------------------------------------------------------------ algorithms
+ * Outputs are DEVICE pointers owned by the caller. */
+
+/* Level-synchronous BFS (top-down, degree-binned expansion).
+ * level[v] = hops from src, GX_DIST_INFINITY if unreached.
+ * Replaces bfs gg_main (lonestar/analytics/gpu/bfs/bfs.cu:349-362) and
+ * bfs-cpu syncAlgo (lonestar/analytics/cpu/bfs/bfs.cpp:245-293). */
+gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats* st);
+
+/* Direction-optimising BFS (push while the frontier is small, pull over
+ * in-edges with a frontier bitmap otherwise; alpha/beta as in
+ * lonestar/analytics/cpu/bfs/bfsDirectionOpt.cpp:68-77, 0 = defaults 15/18).
+ * Writes levels (same values as gx_bfs) and, when parent != NULL, a valid
+ * BFS parent per reached vertex (parent[src] = src, unreached = UINT32_MAX).
+ * Replaces syncDOAlgo (bfsDirectionOpt.cpp:205-342). */
+gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32_t beta,
+                    uint32_t* level, uint32_t* parent, gx_stats* st);
+
+/* Delta-stepping SSSP on uint32 weights; bucket = dist >> shift (the
+ * reference's -delta option is this shift).  dist[v] = shortest distance,
+ * GX_DIST_INFINITY if unreached.  Replaces sssp gg_main
+ * (lonestar/analytics/gpu/sssp/sssp.cu:407-432) and deltaStepAlgo
+ * (lonestar/analytics/cpu/sssp/SSSP.cpp:119-173). */
+gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift,
+                        uint32_t* dist, gx_stats* st);
+
+/* Connected components of a symmetric graph; label[v] = minimum vertex id
+ * of v's component (the canonical result of the reference union-find,
+ * libgalois/include/galois/UnionFind.h:108-122, and LabelProp,
+ * ConnectedComponents.cpp:183-230).  Replaces cc gg_main
+ * (lonestar/analytics/gpu/connected-components/cc.cu:561-655). */
+gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* st);
+
+/* k-core of a symmetric graph by synchronous cascade peeling.
+ * degree[v] ends as the number of alive neighbours (alive: degree >= k),
+ * exactly the reference's currentDegree; alive (nullable) gets 0/1; the
+ * alive count is returned in *alive_count (nullable).  Replaces
+ * syncCascadeKCore (lonestar/analytics/cpu/k-core/kcore.cpp:124-159) and the
+ * D-IrGL KCoreStep1/2 kernels (kcore_push_cuda.py:10-87). */
+gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t* alive,
+                   uint64_t* alive_count, gx_stats* st);
+
+/* PageRank, pull over the transpose (built on demand).
+ * Semantics (normalised): x0 = 1/n; x' = (1-alpha)/n + alpha*(sum_{u->v}
+ * x[u]/outdeg(u) + D/n), D = sum of x over dangling (outdeg 0) vertices;
+ * stop when ||x'-x||_1 < tol (FP64 reduction) or after max_iter iterations.
+ * precision: 0 = FP32 with Kahan-compensated row sums, 1 = FP64.
+ * pr: device float[n] (precision 0) or double[n] (precision 1).
+ * Replaces computePRTopological (PageRank-pull.cpp:196-251) with the
+ * normalised, dangling-aware semantics. */
+#define GX_PR_FP32 0
+#define GX_PR_FP64 1
+gx_status gx_pagerank_pull(gx_graph* g, double alpha, double tol,
+                           uint32_t max_iter, int precision, void* pr,
+                           gx_stats* st);
+
+/* Reference-semantics pull PageRank (unnormalised, no dangling
+ * redistribution), FP32, the two reference variants:
+ * variant 0 = residual (PageRank-pull.cpp:132-190; value starts 0, residual
+ * starts 1-alpha, stop when no residual exceeds tol), variant 1 = Jacobi
+ * topological (value starts 1/n; x' = alpha*sum x[u]/outdeg(u) +
+ * (1-alpha)/n; stop when sum|x'-x| <= tol).  pr: device float[n]. */
+gx_status gx_pagerank_pull_ref(gx_graph* g, int variant, float alpha,
+                               float tol, uint32_t max_iter, float* pr,
+                               gx_stats* st);
+
+/* Residual-push PageRank, FP32 atomics, the synchronous residual push of
+ * lonestar/analytics/cpu/pagerank/PageRank-push.cpp:103-188 (the GPU app
+ * lonestar/analytics/gpu/pagerank/pagerank.cu:103-236 is its asynchronous
+ * form): value starts 0 and residual 1-alpha; every round each active vertex
+ * whose residual exceeds tol adds it to its value and pushes
+ * alpha*residual/outdeg to its out-neighbours; a neighbour becomes active when
+ * its residual crosses tol.  Converges to the unnormalised PageRank
+ * x = (1-alpha) + alpha*sum x[u]/outdeg(u).  pr: device float[n]. */
+gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol,
+                           uint32_t max_iter, float* pr, gx_stats* st);
+
+/* Differentiable PageRank on a float-weighted directed graph
+ * (GX_GRAPH_FWEIGHTED).  P[u,v] = w_uv / sum_v' w_uv'; dangling rows are
+ * uniform; x = alpha*P_dang^T x + (1-alpha)*t.  FP32 Kahan iteration.
+ * forward: x (device float[n]) from weights (device float[m], CSR order,
+ * overriding the graph's own when non-NULL) and teleport t (device float[n]).
+ * adjoint: given c = dL/dx, solves lambda = alpha*P_dang lambda + c and
+ * writes grad_w (device float[m], CSR order) = dL/dw and grad_t
+ * (device float[n]) = dL/dt = (1-alpha)*lambda.  x is the converged forward
+ * vector (the only state the backward pass needs). */
+gx_status gx_pagerank_weighted(gx_graph* g, const float* weights,
+                               const float* teleport, double alpha,
+                               double tol, uint32_t max_iter, float* x,
+                               gx_stats* st);
+gx_status gx_pagerank_adjoint(gx_graph* g, const float* weights,
+                              const float* x, const float* c, double alpha,
+                              double tol, uint32_t max_iter, float* grad_w,
+                              float* grad_t, gx_stats* st);
+
+/* Brandes betweenness centrality, one source per pass (plain).
+ * bc[v] (device double[n]) = sum over the given sources s != v of the
+ * dependency delta_s(v); directed convention, not halved (as
+ * LevelStructs.h:142-185 / OuterStructs.h:41-102).  Path counts sigma in
+ * FP32, dependencies in FP64.  sources: HOST array. */
+gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
+                double* bc, gx_stats* st);
+/* Same result, all sources batched: 64 sources per wavefront (one lane per
+ * source), level loops captured in a HIP graph.  sources: HOST array. */
+gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_t count,
+                        double* bc, gx_stats* st);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GALOIS_AMD_H */

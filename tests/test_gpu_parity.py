@@ -1,0 +1,318 @@
+"""GPU parity: the HIP library through the C ABI against golden vectors and the
+oracle on the same inputs.  Integer outputs are bit-exact; PageRank is within
+the tolerances stated per test."""
+from __future__ import annotations
+
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import csr_arrays
+
+pytestmark = pytest.mark.gpu
+
+INF = 2147483646
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def u32(t):
+    return t.cpu().numpy().view(np.uint32)
+
+
+def dev(gx, g, symmetric=True, weights=True):
+    row, col, w = csr_arrays(g)
+    return gx.Graph.from_csr(row, col, w if weights else None, symmetric=symmetric)
+
+
+# ------------------------------------------------------------- golden graphs
+def test_bfs_golden(gx, golden):
+    for g in golden["undirected"]:
+        G = dev(gx, g)
+        for s, exp in g["bfs"].items():
+            say(f"bfs {g['name']} src={s}")
+            lv, st = gx.bfs(G, int(s))
+            np.testing.assert_array_equal(u32(lv), np.asarray(exp, np.uint32), err_msg=g["name"])
+            lv2, par, _ = gx.bfs_do(G, int(s), parents=True)
+            np.testing.assert_array_equal(u32(lv2), np.asarray(exp, np.uint32), err_msg=g["name"])
+            _check_parents(g, int(s), u32(lv2), u32(par))
+    for g in golden["directed"]:
+        G = dev(gx, g, symmetric=False, weights=False)
+        lv, _ = gx.bfs(G, 0)
+        np.testing.assert_array_equal(u32(lv), np.asarray(g["bfs0"], np.uint32))
+        lv2, _, _ = gx.bfs_do(G, 0, alpha=1, beta=1)  # forces pull levels on the CSC
+        np.testing.assert_array_equal(u32(lv2), np.asarray(g["bfs0"], np.uint32))
+
+
+def _check_parents(g, s, lv, par):
+    row, col, _ = csr_arrays(g)
+    for v in range(len(row) - 1):
+        if lv[v] == INF:
+            assert par[v] == 0xFFFFFFFF
+        elif v == s:
+            assert par[v] == s
+        else:
+            p = par[v]
+            assert lv[p] == lv[v] - 1
+            assert v in col[row[p]:row[p + 1]]
+
+
+def test_sssp_golden(gx, golden):
+    for g in golden["undirected"]:
+        G = dev(gx, g)
+        for s, exp in g["sssp"].items():
+            for shift in (0, 2, 6, 13):
+                say(f"sssp {g['name']} src={s} shift={shift}")
+                d, _ = gx.sssp(G, int(s), shift)
+                np.testing.assert_array_equal(u32(d), np.asarray(exp, np.uint32), err_msg=g["name"])
+
+
+def test_cc_golden(gx, golden):
+    for g in golden["undirected"]:
+        say(f"cc {g['name']}")
+        lab, _ = gx.connected_components(dev(gx, g))
+        np.testing.assert_array_equal(u32(lab), np.asarray(g["cc"], np.uint32), err_msg=g["name"])
+
+
+def test_kcore_golden(gx, golden):
+    for g in golden["undirected"]:
+        G = dev(gx, g)
+        for k, exp in g["kcore"].items():
+            say(f"kcore {g['name']} k={k}")
+            deg, alive, cnt, _ = gx.kcore(G, int(k))
+            np.testing.assert_array_equal(u32(deg), np.asarray(exp["degree"], np.uint32))
+            np.testing.assert_array_equal(alive.cpu().numpy(), np.asarray(exp["alive"], np.uint8))
+            assert cnt == sum(exp["alive"])
+
+
+def test_pagerank_golden(gx, golden):
+    for g in golden["undirected"] + golden["directed"]:
+        say(f"pagerank {g['name']}")
+        G = dev(gx, g, symmetric=False, weights=False)
+        exp = np.asarray(g["pagerank"])
+        x64, st = gx.pagerank(G, tol=1e-14, max_iter=3000, precision="fp64")
+        np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-10, atol=1e-13, err_msg=g["name"])
+        assert st.residual < 1e-14
+        x32, _ = gx.pagerank(G, tol=1e-12, max_iter=300, precision="fp32")
+        # FP32 (compensated sums) vs FP64 closed form: L1 error bound 1e-6
+        assert np.abs(x32.cpu().numpy().astype(np.float64) - exp).sum() < 1e-6, g["name"]
+
+
+def test_pagerank_reference_semantics(gx, golden, oracle):
+    for g in golden["undirected"] + golden["directed"]:
+        say(f"pagerank_ref {g['name']}")
+        row, col, _ = csr_arrays(g)
+        G = dev(gx, g, symmetric=False, weights=False)
+        # residual pull is race free: bit-identical to the CPU restatement
+        for tol in (1e-3, 1e-6):
+            got, st = gx.pagerank_ref(G, "residual", tol=tol)
+            exp, it = oracle.pagerank_residual(row, col, tol=tol)
+            np.testing.assert_array_equal(got.cpu().numpy(), exp, err_msg=g["name"])
+            assert st.iterations == it
+        got, _ = gx.pagerank_ref(G, "topo", tol=1e-7)
+        exp, _ = oracle.pagerank_topo(row, col, tol=1e-7)
+        np.testing.assert_allclose(got.cpu().numpy(), exp, rtol=1e-5, atol=1e-9)
+        # push: FP32 atomics (order dependent): within the tolerance bound
+        got, _ = gx.pagerank_push(G, tol=1e-7)
+        closed = np.asarray(g["pagerank_unnormalised"])
+        np.testing.assert_allclose(got.cpu().numpy(), closed, atol=2e-5, err_msg=g["name"])
+
+
+# ------------------------------------------------------------- generators
+def _mix64(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15))
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def _hash2(seed, a, b):
+    return _mix64(_mix64(np.uint64(seed) ^ _mix64(a)) ^ (b * np.uint64(0xD1B54A32D192ED03)))
+
+
+def numpy_rmat(scale, ef, a, b, c, seed, symmetric):
+    """Independent numpy statement of the counter-based RMAT generator."""
+    with np.errstate(over="ignore"):
+        m = (1 << scale) * ef
+        i = np.arange(m, dtype=np.uint64)
+        u = np.zeros(m, np.uint64)
+        v = np.zeros(m, np.uint64)
+        ta, tab, tabc = (np.uint64(int(np.floor(p * 2 ** 32))) for p in (a, a + b, a + b + c))
+        for lvl in range(0, scale, 2):
+            h = _hash2(seed, i, np.uint64(lvl))
+            for k in range(2):
+                if lvl + k >= scale:
+                    break
+                r = (h & np.uint64(0xFFFFFFFF)) if k == 0 else (h >> np.uint64(32))
+                bit = np.uint64(1 << (scale - 1 - (lvl + k)))
+                colb = (r >= ta) & ((r < tab) | (r >= tabc))
+                rowb = r >= tab
+                v |= np.where(colb, bit, np.uint64(0))
+                u |= np.where(rowb, bit, np.uint64(0))
+    if symmetric:
+        u, v = np.concatenate([u, v]), np.concatenate([v, u])
+    keep = u != v
+    keys = np.unique(u[keep] * np.uint64(1 << scale) + v[keep])
+    src = (keys >> np.uint64(scale)).astype(np.int64)
+    dst = (keys & np.uint64((1 << scale) - 1)).astype(np.int32)
+    row = np.zeros((1 << scale) + 1, np.int64)
+    np.add.at(row, src + 1, 1)
+    return np.cumsum(row).astype(np.int32), dst
+
+
+@pytest.mark.parametrize("symmetric", [True, False])
+def test_rmat_generator_matches_numpy_statement(gx, symmetric):
+    say(f"rmat generator symmetric={symmetric}")
+    G = gx.Graph.rmat(10, 8, seed=12345, symmetric=symmetric)
+    row, col, _ = G.to_host()
+    erow, ecol = numpy_rmat(10, 8, 0.57, 0.19, 0.19, 12345, symmetric)
+    np.testing.assert_array_equal(row, erow)
+    np.testing.assert_array_equal(col, ecol)
+    # deterministic: same seed, same graph
+    row2, col2, _ = gx.Graph.rmat(10, 8, seed=12345, symmetric=symmetric).to_host()
+    np.testing.assert_array_equal(col, col2)
+
+
+def test_transpose_matches_oracle(gx, oracle):
+    say("transpose rmat12")
+    G = gx.Graph.rmat(12, 16, symmetric=False)
+    row, col, _ = G.to_host()
+    trow, tcol, perm = G.transpose_host()
+    erow, ecol, eperm = oracle.transpose(row, col)
+    np.testing.assert_array_equal(trow, erow)
+    np.testing.assert_array_equal(tcol, ecol)
+    np.testing.assert_array_equal(perm, eperm)
+
+
+def test_grid_generator_properties(gx):
+    say("grid 64")
+    G = gx.Graph.grid(64)
+    row, col, w = G.to_host()
+    n = 64 * 64
+    assert len(row) == n + 1
+    src = np.repeat(np.arange(n), np.diff(row))
+    # symmetric with symmetric weights, no self loops, sorted unique rows
+    key = src.astype(np.int64) * n + col
+    rkey = col.astype(np.int64) * n + src
+    order = np.argsort(rkey)
+    np.testing.assert_array_equal(np.sort(rkey), key)
+    np.testing.assert_array_equal(w[order], w)
+    assert (src != col).all()
+    assert (w >= 1).all() and (w <= 1000).all()
+    lattice = 2 * 64 * 63
+    assert 0.9 * 2 * lattice < len(col) < 1.05 * 2 * lattice
+
+
+# ------------------------------------------------------------- RMAT parity
+@pytest.fixture(scope="module")
+def rmat16(gx):
+    say("generate rmat16 symmetric weighted")
+    G = gx.Graph.rmat(16, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
+    row, col, w = G.to_host()
+    return G, row, col, w
+
+
+def test_rmat16_bfs_sssp(gx, oracle, rmat16):
+    G, row, col, w = rmat16
+    deg = np.diff(row)
+    for s in (int(np.argmax(deg)), 0, 12345):
+        say(f"rmat16 bfs/sssp src={s}")
+        exp = oracle.bfs(row, col, s, threads=8)
+        lv, _ = gx.bfs(G, s)
+        np.testing.assert_array_equal(u32(lv), exp)
+        lv2, _, _ = gx.bfs_do(G, s)
+        np.testing.assert_array_equal(u32(lv2), exp)
+        for shift in (4, 6):
+            d, _ = gx.sssp(G, s, shift)
+            np.testing.assert_array_equal(u32(d), oracle.sssp(row, col, w, s, shift, threads=8))
+
+
+def test_rmat16_cc_kcore(gx, oracle, rmat16):
+    G, row, col, w = rmat16
+    say("rmat16 cc")
+    lab, _ = gx.connected_components(G)
+    np.testing.assert_array_equal(u32(lab), oracle.cc(row, col, threads=8))
+    for k in (2, 10, 40):
+        say(f"rmat16 kcore k={k}")
+        deg, alive, cnt, _ = gx.kcore(G, k)
+        exp = oracle.kcore(row, col, k, threads=8)
+        np.testing.assert_array_equal(u32(deg), exp)
+        assert cnt == int((exp >= k).sum())
+
+
+def test_rmat16_pagerank(gx, oracle, rmat16):
+    G, row, col, w = rmat16
+    say("rmat16 pagerank")
+    exp, it, res = oracle.pagerank(row, col, tol=1e-9, max_iter=100, threads=8)
+    x64, st = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp64")
+    assert st.iterations == it
+    np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-11, atol=1e-17)
+    x32, st32 = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
+    err = np.abs(x32.cpu().numpy().astype(np.float64) - exp)
+    assert err.sum() < 1e-6 and st32.iterations == it
+
+
+def test_grid_parity(gx, oracle):
+    say("grid 256 parity")
+    G = gx.Graph.grid(256)
+    row, col, w = G.to_host()
+    d, _ = gx.sssp(G, 0, 6)
+    np.testing.assert_array_equal(u32(d), oracle.sssp(row, col, w, 0, 6, threads=8))
+    lab, _ = gx.connected_components(G)
+    np.testing.assert_array_equal(u32(lab), oracle.cc(row, col, threads=8))
+    deg, _, _, _ = gx.kcore(G, 3)
+    np.testing.assert_array_equal(u32(deg), oracle.kcore(row, col, 3, threads=8))
+    lv, _ = gx.bfs(G, 0)
+    np.testing.assert_array_equal(u32(lv), oracle.bfs(row, col, 0, threads=8))
+
+
+# ------------------------------------------------------------- edge cases
+def test_edge_cases(gx):
+    from galois_amd import _lib
+    say("edge cases")
+    # one vertex, no edges
+    G = gx.Graph.from_csr(np.array([0, 0], np.int32), np.zeros(0, np.int32), np.zeros(0, np.uint32),
+                          symmetric=True)
+    lv, _ = gx.bfs(G, 0)
+    assert u32(lv).tolist() == [0]
+    d, _ = gx.sssp(G, 0, 6)
+    assert u32(d).tolist() == [0]
+    lab, _ = gx.connected_components(G)
+    assert u32(lab).tolist() == [0]
+    deg, alive, cnt, _ = gx.kcore(G, 1)
+    assert cnt == 0
+    x, _ = gx.pagerank(G, precision="fp64")
+    assert abs(float(x[0]) - 1.0) < 1e-12
+    # isolated vertices around a single edge
+    row = np.array([0, 0, 1, 2, 2], np.int32)
+    col = np.array([2, 1], np.int32)
+    G = gx.Graph.from_csr(row, col, np.array([5, 5], np.uint32), symmetric=True)
+    assert u32(gx.bfs(G, 1)[0]).tolist() == [INF, 0, 1, INF]
+    assert u32(gx.sssp(G, 2, 0)[0]).tolist() == [INF, 5, 0, INF]
+    assert u32(gx.connected_components(G)[0]).tolist() == [0, 1, 1, 3]
+    # errors: out-of-range source, unweighted SSSP, asymmetric CC
+    with pytest.raises(_lib.GxError) as e:
+        gx.bfs(G, 9)
+    assert e.value.code == _lib.GX_ERR_INVALID
+    U = gx.Graph.from_csr(row, col, None, symmetric=False)
+    with pytest.raises(_lib.GxError):
+        gx.sssp(U, 0)
+    with pytest.raises(_lib.GxError):
+        gx.connected_components(U)
+    with pytest.raises(_lib.GxError):
+        gx.kcore(U, 2)
+
+
+def test_repeat_runs_are_identical(gx, rmat16):
+    G, row, col, w = rmat16
+    say("repeat determinism")
+    a = u32(gx.sssp(G, 3, 6)[0]).copy()
+    b = u32(gx.sssp(G, 3, 6)[0]).copy()
+    np.testing.assert_array_equal(a, b)
+    x1 = gx.pagerank(G, precision="fp32")[0].cpu().numpy().copy()
+    x2 = gx.pagerank(G, precision="fp32")[0].cpu().numpy().copy()
+    np.testing.assert_array_equal(x1, x2)  # fixed-order reductions: deterministic
