@@ -1,0 +1,688 @@
+// Brandes betweenness centrality: one source per pass (gx_bc) and all
+// sources batched, 64 per wavefront (gx_bc_batched).
+//
+// Reference semantics: Level BC, lonestar/analytics/cpu/
+// betweennesscentrality/LevelStructs.h:87-185 (forward: BFS levels, path
+// counts sigma accumulated from level-L predecessors; backward, deepest level
+// first: delta[v] = sigma[v] * sum_{w succ, level L+1} (1 + delta[w]) /
+// sigma[w], then bc[v] += delta[v]; the source itself is excluded), the FP64
+// Outer form OuterStructs.h:41-102 and the D-IrGL kernels
+// lonestar/analytics/distributed/betweennesscentrality/bc_level_cuda.py:
+// 43-143.  Directed convention, not halved for symmetric graphs.  Path counts
+// are FP32, dependencies FP64, bc FP64.
+//
+// Plain (gx_bc): per source a level-synchronous push forward (atomicCAS on
+// the level word, FP32 atomicAdd of sigma) and a pull backward over each
+// level's vertices (FP64 atomicAdd into the dependency of the vertex), both
+// with the load-balanced expansion of frontier.hpp.  The level lists are
+// kept back to back in one array, the stack of worklists of the reference.
+//
+// Batched (gx_bc_batched): sources are processed 64 at a time per wavefront,
+// lane j owning source j of its group; several groups run in the same
+// launches.  Per vertex and group the frontier of each level is one 64-bit
+// mask.  Forward level L -> L+1 is a pull: a wavefront takes a chunk of a
+// vertex's in-edges, loads 64 neighbour masks at once, and for every
+// neighbour with a useful bit lane j adds sigma[u][j]; new bits become the
+// vertex's level-(L+1) mask.  Backward is the matching pull over out-edges
+// against the level-(L+1) masks.  Adjacencies longer than one chunk are
+// split over several wavefronts that combine with atomics; all other
+// vertices are written by a single wavefront.  The backward level loop has a
+// fixed length once the forward sweep is done and is captured in a HIP graph
+// and replayed.
+#include <algorithm>
+#include <vector>
+
+#include "frontier.hpp"
+#include "gx_device.hpp"
+#include "gx_internal.hpp"
+
+using namespace gx;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr uint32_t kInf = GX_BC_INFINITY;
+constexpr uint32_t kHub = 4096;
+
+// ================================================================== plain
+struct PlainCounters {
+  unsigned count;
+  unsigned hub_count;
+  unsigned overflow;
+  unsigned pad;
+};
+
+__global__ void bc_plain_init_kernel(uint32_t* __restrict__ level, float* __restrict__ sigma,
+                                     double* __restrict__ delta, int64_t n) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    level[v] = kInf;
+    sigma[v] = 0.f;
+    delta[v] = 0.0;
+  }
+}
+
+__global__ void bc_plain_source_kernel(uint32_t* level, float* sigma, uint32_t* order, PlainCounters* c,
+                                       uint32_t s) {
+  if (threadIdx.x == 0) {
+    level[s] = 0;
+    sigma[s] = 1.f;
+    order[0] = s;
+    c->count = 0;
+    c->hub_count = 0;
+    c->overflow = 0;
+  }
+}
+
+__global__ void bc_plain_clear_kernel(PlainCounters* c) {
+  if (threadIdx.x == 0) {
+    c->count = 0;
+    c->hub_count = 0;
+  }
+}
+
+struct FwdOp {
+  const int32_t* col;
+  uint32_t* level;
+  float* sigma;
+  uint32_t* next;
+  PlainCounters* c;
+  uint32_t nl;
+  unsigned cap;
+  __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
+    bool won = false;
+    uint32_t v = 0;
+    if (valid) {
+      v = static_cast<uint32_t>(col[e]);
+      uint32_t old = level[v];
+      if (old == kInf) {
+        old = atomicCAS(&level[v], kInf, nl);
+        won = old == kInf;
+      }
+      if (won || old == nl) atomicAdd(&sigma[v], sigma[u]);
+    }
+    wave_append<uint32_t>(won, v, next, &c->count, cap, &c->overflow);
+  }
+};
+
+struct BwdOp {
+  const int32_t* col;
+  const uint32_t* level;
+  const float* sigma;
+  double* delta;
+  uint32_t succ;
+  __device__ __forceinline__ void operator()(bool valid, uint32_t v, int32_t e) {
+    if (!valid) return;
+    const uint32_t w = static_cast<uint32_t>(col[e]);
+    if (level[w] == succ) atomicAdd(&delta[v], (1.0 + delta[w]) / static_cast<double>(sigma[w]));
+  }
+};
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) bc_expand_kernel(const int32_t* __restrict__ row, Op op,
+                                                           const uint32_t* __restrict__ q, unsigned qcount,
+                                                           uint32_t* __restrict__ hubq, PlainCounters* c,
+                                                           unsigned cap) {
+  expand_queue<kBlock>(row, q, qcount, kHub, hubq, &c->hub_count, cap, &c->overflow, op);
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) bc_hubs_kernel(const int32_t* __restrict__ row, Op op,
+                                                         const uint32_t* __restrict__ hubq,
+                                                         const PlainCounters* c) {
+  expand_hubs<kBlock>(row, hubq, c->hub_count, op);
+}
+
+__global__ void bc_plain_finish_level_kernel(const uint32_t* __restrict__ q, unsigned cnt,
+                                             const float* __restrict__ sigma, double* __restrict__ delta,
+                                             double* __restrict__ bc) {
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t v = q[i];
+    const double d = delta[v] * static_cast<double>(sigma[v]);
+    delta[v] = d;
+    bc[v] += d;
+  }
+}
+
+__global__ void bc_plain_reset_kernel(const uint32_t* __restrict__ q, unsigned cnt, uint32_t* __restrict__ level,
+                                      float* __restrict__ sigma, double* __restrict__ delta) {
+  for (unsigned i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+    const uint32_t v = q[i];
+    level[v] = kInf;
+    sigma[v] = 0.f;
+    delta[v] = 0.0;
+  }
+}
+
+// ================================================================ batched
+// Work item = (chunk, group); chunk = a vertex and at most kChunk of its
+// edges (in-edges for the forward pull, out-edges for the backward pull).
+constexpr int kChunk = 256;
+
+struct ChunkTable {
+  uint32_t* vtx;     // owning vertex
+  int32_t* begin;    // first edge (end = min(begin + kChunk, row[vtx + 1]))
+  uint8_t* whole;    // 1 when the chunk is the vertex's whole adjacency
+  int64_t count;
+  uint32_t* multi;   // vertices with more than one chunk
+  int64_t nmulti;
+};
+
+__global__ void chunk_count_kernel(const int32_t* __restrict__ row, int64_t n, uint32_t* __restrict__ cnt,
+                                   uint32_t* __restrict__ is_multi) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t d = row[v + 1] - row[v];
+    const uint32_t c = static_cast<uint32_t>((d + kChunk - 1) / kChunk);
+    cnt[v] = c;
+    is_multi[v] = c > 1 ? 1u : 0u;
+  }
+}
+
+__global__ void chunk_fill_kernel(const int32_t* __restrict__ row, int64_t n, const uint32_t* __restrict__ first,
+                                  uint32_t* __restrict__ vtx, int32_t* __restrict__ begin,
+                                  uint8_t* __restrict__ whole) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t b = row[v], e = row[v + 1];
+    uint32_t k = first[v];
+    const uint8_t w = (e - b) <= kChunk ? 1 : 0;
+    for (int32_t x = b; x < e; x += kChunk, ++k) {
+      vtx[k] = static_cast<uint32_t>(v);
+      begin[k] = x;
+      whole[k] = w;
+    }
+  }
+}
+
+__global__ void compact_flag_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos, int64_t n,
+                                    uint32_t* __restrict__ out) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    if (flag[v]) out[pos[v]] = static_cast<uint32_t>(v);
+}
+
+__device__ __forceinline__ uint64_t readlane64(uint64_t x, int lane) {
+  const uint32_t lo = __builtin_amdgcn_readlane(static_cast<uint32_t>(x), lane);
+  const uint32_t hi = __builtin_amdgcn_readlane(static_cast<uint32_t>(x >> 32), lane);
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+struct BatchDims {
+  int64_t n;
+  int groups;
+  uint64_t last_mask;  // valid lanes of the last group
+};
+
+__device__ __forceinline__ uint64_t group_mask(const BatchDims& D, int g) {
+  return g == D.groups - 1 ? D.last_mask : ~0ull;
+}
+
+// Level-0 masks, visited masks and path counts of the sources.
+__global__ void bcb_seed_kernel(const uint32_t* __restrict__ src, int count, BatchDims D, uint64_t* __restrict__ F0,
+                                uint64_t* __restrict__ vis, float* __restrict__ sigma) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const int g = i / 64, j = i % 64;
+  const uint32_t s = src[i];
+  atomicOr(reinterpret_cast<unsigned long long*>(&F0[g * D.n + s]), 1ull << j);
+  atomicOr(reinterpret_cast<unsigned long long*>(&vis[g * D.n + s]), 1ull << j);
+  sigma[(static_cast<int64_t>(g) * D.n + s) * 64 + j] = 1.f;
+}
+
+// Forward pull for level L -> L+1.
+__global__ void __launch_bounds__(kBlock) bcb_forward_kernel(
+    const int32_t* __restrict__ trow, const int32_t* __restrict__ tcol, ChunkTable T, BatchDims D,
+    const uint64_t* __restrict__ FL, uint64_t* __restrict__ Fn, const uint64_t* __restrict__ vis,
+    float* __restrict__ sigma, unsigned long long* __restrict__ newcount) {
+  const unsigned lane = lane_id();
+  const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+  const int64_t items = T.count * D.groups;
+  unsigned long long found = 0;
+  for (int64_t it = wave; it < items; it += nwaves) {
+    const int g = static_cast<int>(it / T.count);
+    const int64_t c = it - static_cast<int64_t>(g) * T.count;
+    const uint32_t v = T.vtx[c];
+    const int64_t goff = static_cast<int64_t>(g) * D.n;
+    const uint64_t need = group_mask(D, g) & ~vis[goff + v];
+    if (need == 0) continue;
+    const int32_t eb = T.begin[c];
+    const int32_t ee = min(eb + kChunk, trow[v + 1]);
+    uint64_t acc_mask = 0;
+    float sig = 0.f;
+    for (int32_t e0 = eb; e0 < ee; e0 += kWave) {
+      const int32_t e = e0 + static_cast<int32_t>(lane);
+      uint32_t u = 0;
+      uint64_t f = 0;
+      if (e < ee) {
+        u = static_cast<uint32_t>(tcol[e]);
+        f = FL[goff + u] & need;
+      }
+      uint64_t act = __ballot(f != 0);
+      while (act) {
+        const int i = __ffsll(static_cast<unsigned long long>(act)) - 1;
+        act &= act - 1;
+        const uint64_t fi = readlane64(f, i);
+        const uint32_t ui = __builtin_amdgcn_readlane(u, i);
+        acc_mask |= fi;
+        if ((fi >> lane) & 1ull) sig += sigma[(goff + ui) * 64 + lane];
+      }
+    }
+    if (acc_mask == 0) continue;
+    float* sv = &sigma[(goff + v) * 64 + lane];
+    if (T.whole[c]) {
+      if ((acc_mask >> lane) & 1ull) *sv = sig;
+      if (lane == 0) Fn[goff + v] = acc_mask;
+    } else {
+      if ((acc_mask >> lane) & 1ull) atomicAdd(sv, sig);
+      if (lane == 0) atomicOr(reinterpret_cast<unsigned long long*>(&Fn[goff + v]), acc_mask);
+    }
+    if (lane == 0) found += __popcll(acc_mask);
+  }
+  if (lane == 0 && found) atomicAdd(newcount, found);
+}
+
+// visited |= new frontier.  Counting of new (vertex, source) pairs is done
+// by the forward kernel; multi-chunk vertices may be counted more than once
+// there, so the exact count is recomputed here.
+__global__ void __launch_bounds__(kBlock) bcb_visit_kernel(const uint64_t* __restrict__ Fn,
+                                                           uint64_t* __restrict__ vis, int64_t total,
+                                                           unsigned long long* __restrict__ exact) {
+  unsigned long long cnt = 0;
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint64_t f = Fn[i];
+    if (f) {
+      vis[i] |= f;
+      cnt += __popcll(f);
+    }
+  }
+  cnt = wave_sum(cnt);
+  if (lane_id() == 0 && cnt) atomicAdd(exact, cnt);
+}
+
+// Backward pull for level L (successors at L+1).  Whole-adjacency chunks
+// finalise delta and the per-group bc partial directly; split adjacencies
+// accumulate into delta and are finalised by bcb_multi_finish_kernel.
+__global__ void __launch_bounds__(kBlock) bcb_backward_kernel(
+    const int32_t* __restrict__ row, const int32_t* __restrict__ col, ChunkTable T, BatchDims D,
+    const uint64_t* __restrict__ FL, const uint64_t* __restrict__ FL1, const float* __restrict__ sigma,
+    double* __restrict__ delta, double* __restrict__ bcpart) {
+  const unsigned lane = lane_id();
+  const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+  const int64_t items = T.count * D.groups;
+  for (int64_t it = wave; it < items; it += nwaves) {
+    const int g = static_cast<int>(it / T.count);
+    const int64_t c = it - static_cast<int64_t>(g) * T.count;
+    const uint32_t v = T.vtx[c];
+    const int64_t goff = static_cast<int64_t>(g) * D.n;
+    const uint64_t mine = FL[goff + v];
+    if (mine == 0) continue;
+    const int32_t eb = T.begin[c];
+    const int32_t ee = min(eb + kChunk, row[v + 1]);
+    double acc = 0.0;
+    for (int32_t e0 = eb; e0 < ee; e0 += kWave) {
+      const int32_t e = e0 + static_cast<int32_t>(lane);
+      uint32_t w = 0;
+      uint64_t f = 0;
+      if (e < ee) {
+        w = static_cast<uint32_t>(col[e]);
+        f = FL1[goff + w] & mine;
+      }
+      uint64_t act = __ballot(f != 0);
+      while (act) {
+        const int i = __ffsll(static_cast<unsigned long long>(act)) - 1;
+        act &= act - 1;
+        const uint64_t fi = readlane64(f, i);
+        const uint32_t wi = __builtin_amdgcn_readlane(w, i);
+        if ((fi >> lane) & 1ull) {
+          const int64_t idx = (goff + wi) * 64 + lane;
+          acc += (1.0 + delta[idx]) / static_cast<double>(sigma[idx]);
+        }
+      }
+    }
+    const bool mylane = (mine >> lane) & 1ull;
+    const int64_t vidx = (goff + v) * 64 + lane;
+    if (T.whole[c]) {
+      double d = 0.0;
+      if (mylane) {
+        d = acc * static_cast<double>(sigma[vidx]);
+        delta[vidx] = d;
+      }
+      d = wave_sum(d);
+      if (lane == 0) bcpart[goff + v] += d;
+    } else if (mylane && acc != 0.0) {
+      atomicAdd(&delta[vidx], acc);
+    }
+  }
+}
+
+// Split adjacencies at level L: delta = sigma * accumulated sum; bc partial.
+__global__ void __launch_bounds__(kBlock) bcb_multi_finish_kernel(const uint32_t* __restrict__ multi,
+                                                                  int64_t nmulti, BatchDims D,
+                                                                  const uint64_t* __restrict__ FL,
+                                                                  const float* __restrict__ sigma,
+                                                                  double* __restrict__ delta,
+                                                                  double* __restrict__ bcpart) {
+  const unsigned lane = lane_id();
+  const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
+  const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
+  for (int64_t it = wave; it < nmulti * D.groups; it += nwaves) {
+    const int g = static_cast<int>(it / nmulti);
+    const uint32_t v = multi[it - static_cast<int64_t>(g) * nmulti];
+    const int64_t goff = static_cast<int64_t>(g) * D.n;
+    const uint64_t mine = FL[goff + v];
+    if (mine == 0) continue;
+    double d = 0.0;
+    if ((mine >> lane) & 1ull) {
+      const int64_t idx = (goff + v) * 64 + lane;
+      d = delta[idx] * static_cast<double>(sigma[idx]);
+      delta[idx] = d;
+    }
+    d = wave_sum(d);
+    if (lane == 0) bcpart[goff + v] += d;
+  }
+}
+
+// bc[v] += sum over groups of the group partials, in group order.
+__global__ void bcb_accumulate_kernel(const double* __restrict__ bcpart, BatchDims D, double* __restrict__ bc) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < D.n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    double s = 0.0;
+    for (int g = 0; g < D.groups; ++g) s += bcpart[static_cast<int64_t>(g) * D.n + v];
+    bc[v] += s;
+  }
+}
+
+template <class T>
+void fill_async(T* p, T v, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  fill_kernel<T><<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(p, v, n);
+  GX_LAUNCHED("fill_kernel");
+}
+
+// Chunk table of a CSR, cached in `buf`.
+struct CachedChunks {
+  DevBuf buf;
+  ChunkTable t{};
+  bool ready = false;
+};
+
+void build_chunks(const int32_t* row, int64_t n, int64_t m, hipStream_t s, gx_graph* g, CachedChunks& cc) {
+  if (cc.ready) return;
+  DevBuf cnt, first, flag, pos;
+  cnt.ensure((n + 1) * 4);
+  first.ensure((n + 1) * 4);
+  flag.ensure((n + 1) * 4);
+  pos.ensure((n + 1) * 4);
+  chunk_count_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(row, n, cnt.as<uint32_t>(), flag.as<uint32_t>());
+  GX_LAUNCHED("chunk_count_kernel");
+  fill_async<uint32_t>(cnt.as<uint32_t>() + n, 0u, 1, s);
+  fill_async<uint32_t>(flag.as<uint32_t>() + n, 0u, 1, s);
+  exclusive_scan_u32(cnt.as<uint32_t>(), first.as<uint32_t>(), n + 1, s);
+  exclusive_scan_u32(flag.as<uint32_t>(), pos.as<uint32_t>(), n + 1, s);
+  uint32_t nchunks = 0, nmulti = 0;
+  read_bytes(g, &nchunks, first.as<uint32_t>() + n, 4);
+  read_bytes(g, &nmulti, pos.as<uint32_t>() + n, 4);
+  const int64_t C = nchunks, M = nmulti;
+  // Layout: vtx[C] | begin[C] | multi[M] | whole[C] (bytes).
+  cc.buf.ensure(C * 4 + C * 4 + M * 4 + C + 64);
+  char* base = cc.buf.as<char>();
+  cc.t.vtx = reinterpret_cast<uint32_t*>(base);
+  cc.t.begin = reinterpret_cast<int32_t*>(base + C * 4);
+  cc.t.multi = reinterpret_cast<uint32_t*>(base + C * 8);
+  cc.t.whole = reinterpret_cast<uint8_t*>(base + C * 8 + M * 4);
+  cc.t.count = C;
+  cc.t.nmulti = M;
+  if (C > 0) {
+    chunk_fill_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(row, n, first.as<uint32_t>(), cc.t.vtx,
+                                                                    cc.t.begin, cc.t.whole);
+    GX_LAUNCHED("chunk_fill_kernel");
+  }
+  if (M > 0) {
+    compact_flag_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(flag.as<uint32_t>(), pos.as<uint32_t>(), n,
+                                                                      cc.t.multi);
+    GX_LAUNCHED("compact_flag_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  (void)m;
+  cc.ready = true;
+}
+
+// Per-graph caches of the batched BC (chunk tables of CSR and CSC).
+struct BcCache {
+  CachedChunks out, in;
+  std::vector<DevBuf> levels;  // frontier masks per level
+};
+
+}  // namespace
+
+// Graph-owned cache, created on first use, destroyed with the graph.
+struct gx_bc_cache_holder {
+  BcCache c;
+};
+
+static BcCache& bc_cache(gx_graph* g) {
+  if (!g->bc_cache) g->bc_cache = std::make_shared<gx_bc_cache_holder>();
+  return std::static_pointer_cast<gx_bc_cache_holder>(g->bc_cache)->c;
+}
+
+extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count, double* bc, gx_stats* st) {
+  GX_API_BEGIN
+  check_graph(g);
+  check_ptr(bc, "bc");
+  GX_REQUIRE(count == 0 || sources != nullptr, "sources is NULL");
+  for (uint32_t i = 0; i < count; ++i) check_vertex(g, sources[i], "source");
+  const int64_t n = g->n;
+  hipStream_t s = g->stream;
+  const unsigned cap = static_cast<unsigned>(std::max<int64_t>(n, 1));
+  uint32_t* level = g->scratch<uint32_t>(WS_BC_LEVEL, cap);
+  float* sigma = g->scratch<float>(WS_BC_SIGMA, cap);
+  double* delta = g->scratch<double>(WS_BC_DELTA, cap);
+  uint32_t* order = g->scratch<uint32_t>(WS_Q0, cap);
+  uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, cap);
+  PlainCounters* c = g->scratch<PlainCounters>(WS_COUNTERS, 1);
+  PlainCounters* host = static_cast<PlainCounters*>(g->host_pinned(sizeof(PlainCounters)));
+  RegionTimer region(s);
+  KernelTimer ktimer(s);
+  uint32_t launches = 0, levels_total = 0;
+  uint64_t edges = 0;
+  fill_async<double>(bc, 0.0, n, s);
+  bc_plain_init_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(level, sigma, delta, n);
+  GX_LAUNCHED("bc_plain_init_kernel");
+  launches += 2;
+  const unsigned grid = 256 * 8;
+  std::vector<uint32_t> lstart;
+  for (uint32_t si = 0; si < count; ++si) {
+    const uint32_t src = sources[si];
+    bc_plain_source_kernel<<<1, 64, 0, s>>>(level, sigma, order, c, src);
+    lstart.assign({0u, 1u});
+    uint32_t L = 0;
+    // Forward: level L's vertices are order[lstart[L], lstart[L+1]).
+    ktimer.begin();
+    for (;;) {
+      const uint32_t b = lstart[L], e = lstart[L + 1];
+      const unsigned cnt = e - b;
+      if (cnt == 0) break;
+      bc_plain_clear_kernel<<<1, 64, 0, s>>>(c);
+      FwdOp op{g->col, level, sigma, order + e, c, L + 1, cap - e};
+      bc_expand_kernel<FwdOp><<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(
+          g->row, op, order + b, cnt, hubq, c, cap);
+      bc_hubs_kernel<FwdOp><<<1024, kBlock, 0, s>>>(g->row, op, hubq, c);
+      GX_LAUNCHED("bc forward");
+      launches += 3;
+      GX_HIP(hipMemcpyAsync(host, c, sizeof(PlainCounters), hipMemcpyDeviceToHost, s));
+      GX_HIP(hipStreamSynchronize(s));
+      if (host->overflow) fail(GX_ERR_CAPACITY, "BC level list overflow");
+      lstart.push_back(e + host->count);
+      ++L;
+    }
+    // Levels 0..L-1 are non-empty.  Backward from L-2 down to 1.
+    const uint32_t deepest = L - 1;
+    for (int64_t lv = static_cast<int64_t>(deepest) - 1; lv >= 1; --lv) {
+      const uint32_t b = lstart[lv], e = lstart[lv + 1];
+      const unsigned cnt = e - b;
+      bc_plain_clear_kernel<<<1, 64, 0, s>>>(c);
+      BwdOp op{g->col, level, sigma, delta, static_cast<uint32_t>(lv + 1)};
+      bc_expand_kernel<BwdOp><<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(
+          g->row, op, order + b, cnt, hubq, c, cap);
+      bc_hubs_kernel<BwdOp><<<1024, kBlock, 0, s>>>(g->row, op, hubq, c);
+      bc_plain_finish_level_kernel<<<grid_for(cnt, kBlock, 4096), kBlock, 0, s>>>(order + b, cnt, sigma, delta, bc);
+      GX_LAUNCHED("bc backward");
+      launches += 4;
+    }
+    ktimer.end();
+    const uint32_t total = lstart[L];
+    bc_plain_reset_kernel<<<grid_for(total, kBlock, 4096), kBlock, 0, s>>>(order, total, level, sigma, delta);
+    GX_LAUNCHED("bc_plain_reset_kernel");
+    launches += 2;
+    levels_total += L;
+    edges += 0;
+  }
+  double ms = region.stop();
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->iterations = levels_total;
+    st->launches = launches;
+    st->edges_visited = static_cast<uint64_t>(g->m) * count;
+    st->items = count;
+    st->elapsed_ms = ms;
+    st->main_kernel_ms = ktimer.total_ms();
+    st->main_kernel_launches = ktimer.count();
+  }
+  GX_API_END
+}
+
+extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_t count, double* bc,
+                                   gx_stats* st) {
+  GX_API_BEGIN
+  check_graph(g);
+  check_ptr(bc, "bc");
+  GX_REQUIRE(count == 0 || sources != nullptr, "sources is NULL");
+  for (uint32_t i = 0; i < count; ++i) check_vertex(g, sources[i], "source");
+  const int64_t n = g->n;
+  hipStream_t s = g->stream;
+  BcCache& cache = bc_cache(g);
+  build_chunks(g->row, n, g->m, s, g, cache.out);
+  const int32_t* trow = g->row;
+  const int32_t* tcol = g->col;
+  ChunkTable tin = cache.out.t;
+  if (!(g->flags & GX_GRAPH_SYMMETRIC)) {
+    build_transpose(g);
+    trow = g->trow();
+    tcol = g->tcol();
+    build_chunks(trow, n, g->m, s, g, cache.in);
+    tin = cache.in.t;
+  }
+  const ChunkTable tout = cache.out.t;
+  // Groups per launch bounded by memory: sigma 256 B + delta 512 B + two
+  // masks per vertex per group (plus the level masks).
+  const int64_t per_group = n * (64 * 4 + 64 * 8 + 8 + 8);
+  const int64_t budget = int64_t(48) << 30;
+  int max_groups = static_cast<int>(std::max<int64_t>(1, budget / std::max<int64_t>(per_group, 1)));
+  max_groups = std::min(max_groups, 64);
+  RegionTimer region(s);
+  KernelTimer ktimer(s);
+  uint32_t launches = 0, levels_total = 0;
+  fill_async<double>(bc, 0.0, n, s);
+  ++launches;
+  unsigned long long* ctr = g->scratch<unsigned long long>(WS_COUNTERS, 4);
+  unsigned long long* host = static_cast<unsigned long long*>(g->host_pinned(4 * sizeof(unsigned long long)));
+  uint32_t* dsrc = g->scratch<uint32_t>(WS_Q3, std::max<uint32_t>(count, 1));
+  if (count > 0) GX_HIP(hipMemcpyAsync(dsrc, sources, count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
+  const unsigned wgrid = 256 * 32;  // workgroups of the persistent-style sweeps
+  for (uint32_t first = 0; first < count; first += 64u * max_groups) {
+    const uint32_t cnt = std::min<uint32_t>(count - first, 64u * max_groups);
+    BatchDims D;
+    D.n = n;
+    D.groups = static_cast<int>((cnt + 63) / 64);
+    const uint32_t tail = cnt - 64u * (D.groups - 1);
+    D.last_mask = tail == 64 ? ~0ull : ((1ull << tail) - 1);
+    const int64_t gn = static_cast<int64_t>(D.groups) * n;
+    float* sigma = g->scratch<float>(WS_BC_SIGMA, gn * 64);
+    double* delta = g->scratch<double>(WS_BC_DELTA, gn * 64);
+    uint64_t* vis = g->scratch<uint64_t>(WS_BC_MASK0, gn);
+    double* bcpart = g->scratch<double>(WS_BC_MASK1, gn);
+    fill_async<float>(sigma, 0.f, gn * 64, s);
+    fill_async<double>(delta, 0.0, gn * 64, s);
+    fill_async<uint64_t>(vis, 0ull, gn, s);
+    fill_async<double>(bcpart, 0.0, gn, s);
+    launches += 4;
+    auto level_buf = [&](size_t L) -> uint64_t* {
+      while (cache.levels.size() <= L) cache.levels.emplace_back();
+      cache.levels[L].ensure(gn * sizeof(uint64_t));
+      return cache.levels[L].as<uint64_t>();
+    };
+    uint64_t* F0 = level_buf(0);
+    fill_async<uint64_t>(F0, 0ull, gn, s);
+    bcb_seed_kernel<<<grid_for(cnt, kBlock), kBlock, 0, s>>>(dsrc + first, static_cast<int>(cnt), D, F0, vis, sigma);
+    GX_LAUNCHED("bcb_seed_kernel");
+    launches += 2;
+    // Forward sweep: level masks F[0..deepest].
+    ktimer.begin();
+    size_t L = 0;
+    for (;;) {
+      uint64_t* FL = level_buf(L);
+      uint64_t* Fn = level_buf(L + 1);
+      FL = cache.levels[L].as<uint64_t>();
+      fill_async<uint64_t>(Fn, 0ull, gn, s);
+      fill_async<unsigned long long>(ctr, 0ull, 2, s);
+      bcb_forward_kernel<<<wgrid, kBlock, 0, s>>>(trow, tcol, tin, D, FL, Fn, vis, sigma, ctr);
+      GX_LAUNCHED("bcb_forward_kernel");
+      bcb_visit_kernel<<<grid_for(gn, kBlock, 8192), kBlock, 0, s>>>(Fn, vis, gn, ctr + 1);
+      GX_LAUNCHED("bcb_visit_kernel");
+      launches += 4;
+      GX_HIP(hipMemcpyAsync(host, ctr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+      GX_HIP(hipStreamSynchronize(s));
+      if (host[1] == 0) break;
+      ++L;
+      GX_REQUIRE(L < 65535, "BC level count exceeds 65534");
+    }
+    // Levels 0..L are non-empty (L = deepest).  Backward sweep L-1 .. 1,
+    // captured once into a HIP graph and replayed.
+    const size_t deepest = L;
+    levels_total += static_cast<uint32_t>(deepest + 1);
+    if (deepest >= 2) {
+      hipGraph_t graph;
+      hipGraphExec_t exec;
+      // Capture on the graph's own non-null stream (the null stream cannot be
+      // captured); the forward work on `s` has completed (synchronised above).
+      hipStream_t cs = g->capture();
+      GX_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      for (int64_t lv = static_cast<int64_t>(deepest) - 1; lv >= 1; --lv) {
+        const uint64_t* FL = cache.levels[lv].as<uint64_t>();
+        const uint64_t* FL1 = cache.levels[lv + 1].as<uint64_t>();
+        bcb_backward_kernel<<<wgrid, kBlock, 0, cs>>>(g->row, g->col, tout, D, FL, FL1, sigma, delta, bcpart);
+        if (tout.nmulti > 0)
+          bcb_multi_finish_kernel<<<grid_for(tout.nmulti * D.groups * 64, kBlock, 8192), kBlock, 0, cs>>>(
+              tout.multi, tout.nmulti, D, FL, sigma, delta, bcpart);
+      }
+      hipError_t cap_err = hipStreamEndCapture(cs, &graph);
+      GX_HIP(cap_err);
+      GX_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+      GX_HIP(hipGraphLaunch(exec, s));
+      GX_HIP(hipStreamSynchronize(s));
+      (void)hipGraphExecDestroy(exec);
+      (void)hipGraphDestroy(graph);
+      launches += static_cast<uint32_t>((deepest - 1) * (tout.nmulti > 0 ? 2 : 1));
+    }
+    ktimer.end();
+    bcb_accumulate_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(bcpart, D, bc);
+    GX_LAUNCHED("bcb_accumulate_kernel");
+    ++launches;
+  }
+  double ms = region.stop();
+  if (st) {
+    std::memset(st, 0, sizeof(*st));
+    st->iterations = levels_total;
+    st->launches = launches;
+    st->edges_visited = static_cast<uint64_t>(g->m) * count;
+    st->items = count;
+    st->elapsed_ms = ms;
+    st->main_kernel_ms = ktimer.total_ms();
+    st->main_kernel_launches = ktimer.count();
+  }
+  GX_API_END
+}

@@ -26,6 +26,12 @@ using namespace gx;
 
 gx_graph::~gx_graph() {
   if (pinned) (void)hipHostFree(pinned);
+  if (capture_stream) (void)hipStreamDestroy(capture_stream);
+}
+
+hipStream_t gx_graph::capture() {
+  if (!capture_stream) GX_HIP(hipStreamCreateWithFlags(&capture_stream, hipStreamNonBlocking));
+  return capture_stream;
 }
 
 void* gx_graph::host_pinned(size_t bytes) {
@@ -301,8 +307,9 @@ void build_transpose(gx_graph* g) {
   const int64_t n = g->n, m = g->m;
   hipStream_t s = g->stream;
   g->trow_buf.ensure((n + 1) * sizeof(int32_t));
-  g->tcol_buf.ensure(std::max<int64_t>(m, 1) * sizeof(int32_t));
+  g->tcol_buf.ensure((m + kEdgePad) * sizeof(int32_t));
   g->tperm_buf.ensure(std::max<int64_t>(m, 1) * sizeof(int32_t));
+  fill<int32_t>(g->tcol() + m, 0, kEdgePad, s);
   if (m == 0) {
     fill<int32_t>(g->trow(), 0, n + 1, s);
     GX_HIP(hipStreamSynchronize(s));

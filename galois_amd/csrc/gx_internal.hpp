@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <cstring>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -98,6 +99,10 @@ enum Slot : int {
   WS_NUM_SLOTS
 };
 
+// The transpose's column array is allocated with this many trailing entries
+// (filled with vertex 0) so vectorised edge loads may run past m.
+constexpr int64_t kEdgePad = 4096;
+
 // Tables of the pull PageRank sweep over the transpose, built once per graph.
 struct PullPlan {
   bool ready = false;
@@ -124,7 +129,9 @@ struct gx_graph {
   gx::DevBuf outdeg_buf;  // uint32 out-degree (for transpose-based PR)
   bool has_outdeg = false;
   gx::PullPlan pull_plan;
+  std::shared_ptr<void> bc_cache;  // batched-BC tables (bc.hip)
   hipStream_t stream = nullptr;
+  hipStream_t capture_stream = nullptr;  // non-null stream for HIP graph capture
   gx::DevBuf ws[gx::WS_NUM_SLOTS];
   void* pinned = nullptr;  // small pinned host block for counters
   size_t pinned_bytes = 0;
@@ -139,6 +146,8 @@ struct gx_graph {
   }
   // Pinned host block of at least `bytes` for small device->host reads.
   void* host_pinned(size_t bytes);
+  // Default-priority stream used only to capture HIP graphs.
+  hipStream_t capture();
 };
 
 namespace gx {

@@ -43,7 +43,7 @@ using namespace gx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr int kWin = 2048;          // edges per window (rows up to kWin stay in windows)
+constexpr int kWin = 2040;          // edges per window (rows up to kWin stay in windows)
 constexpr int kPiece = 8192;        // edges per long-row piece
 constexpr int kSweepGrid = 256 * 8; // workgroups of the window sweep
 constexpr int kLongGrid = 2048;
@@ -204,6 +204,18 @@ __device__ __forceinline__ void finalize_row(uint32_t r, Real sum, const PrState
   }
 }
 
+// One workgroup per window.  Lane t owns the 16 consecutive edge slots
+// [A + 16t, A + 16t + 16) (A = window start rounded down to 4 so the column
+// loads are 16-byte vectors), gathers their y values into registers and sums
+// them row segment by row segment.  A row that lies inside a lane's slots is
+// finalised by that lane; a row that crosses lanes is finalised by the lane
+// holding its first edge, which adds the continuation partials that the later
+// lanes left in LDS, in lane order (deterministic).  Empty rows get x' = base.
+// Windows with more rows than kRowCap fall back to a lane per row.
+constexpr int kPerLane = 16;
+constexpr int kRowCap = 1024;
+static_assert(kPerLane * kBlock >= 2 * kWin + 3, "window must fit the lanes' slots");
+
 template <class Real>
 __global__ void __launch_bounds__(kBlock) pr_window_kernel(
     const int32_t* __restrict__ trow, const int32_t* __restrict__ tcol, const int32_t* __restrict__ win_rows,
@@ -212,33 +224,109 @@ __global__ void __launch_bounds__(kBlock) pr_window_kernel(
     double* __restrict__ part_resid, double* __restrict__ part_dang) {
   if (state->converged) return;
   const PrState S = *state;
-  __shared__ Real vals[2 * kWin];
+  __shared__ int32_t s_off[kRowCap + 1];
+  __shared__ Real s_cs[kBlock];
+  __shared__ Real s_cc[kBlock];
   __shared__ double red[kBlock / kWave];
   double resid = 0, dang = 0;
-  const unsigned tid = threadIdx.x, lane = lane_id(), wid = tid / kWave;
+  const int tid = threadIdx.x;
   for (int64_t k = blockIdx.x; k < nwin; k += gridDim.x) {
     const int32_t r0 = win_rows[k], r1 = win_rows[k + 1];
     if (r0 == r1) continue;
     int32_t rl = r1;
-    if (trow[r1] - trow[r1 - 1] > kWin) rl = r1 - 1;  // long last row: handled by the piece sweep
+    if (trow[r1] - trow[r1 - 1] > kWin) rl = r1 - 1;  // long last row: piece sweep
+    const int32_t nrows = rl - r0;
+    if (nrows == 0) continue;
     const int32_t e0 = trow[r0];
     const int32_t nnz = trow[rl] - e0;
-    for (int32_t i = tid; i < nnz; i += kBlock) vals[i] = y[tcol[e0 + i]];
-    __syncthreads();
-    const int32_t nrows = rl - r0;
-    if (nrows >= kBlock / 2) {
+    if (nrows > kRowCap) {
+      // Rare (long runs of empty rows): a lane per row.
       for (int32_t r = r0 + tid; r < rl; r += kBlock) {
         Acc<Real> a;
-        for (int32_t i = trow[r] - e0, end = trow[r + 1] - e0; i < end; ++i) a.add(vals[i]);
+        for (int32_t i = trow[r], end = trow[r + 1]; i < end; ++i) a.add(y[tcol[i]]);
         finalize_row<Real>(r, a.value(), S, x_old, x_new, y_new, outdeg, resid, dang);
       }
-    } else {
-      for (int32_t r = r0 + wid; r < rl; r += kBlock / kWave) {
-        Acc<Real> a;
-        for (int32_t i = trow[r] - e0 + lane, end = trow[r + 1] - e0; i < end; i += kWave) a.add(vals[i]);
-        a = wave_reduce(a);
-        if (lane == 0) finalize_row<Real>(r, a.value(), S, x_old, x_new, y_new, outdeg, resid, dang);
+      continue;
+    }
+    for (int32_t i = tid; i <= nrows; i += kBlock) s_off[i] = trow[r0 + i] - e0;
+    __syncthreads();
+    // Empty rows.
+    for (int32_t i = tid; i < nrows; i += kBlock)
+      if (s_off[i] == s_off[i + 1]) finalize_row<Real>(r0 + i, Real(0), S, x_old, x_new, y_new, outdeg, resid, dang);
+    // This lane's slots, relative to e0: [lo, hi) within [0, nnz).
+    const int32_t A = e0 & ~3;
+    const int32_t base_rel = A - e0 + tid * kPerLane;  // may be negative for lane 0
+    const int32_t lo = max(base_rel, 0), hi = min(base_rel + kPerLane, nnz);
+    Real v[kPerLane];
+    if (base_rel < nnz) {
+      const int4* p = reinterpret_cast<const int4*>(tcol + A + tid * kPerLane);
+#pragma unroll
+      for (int q = 0; q < kPerLane / 4; ++q) {
+        const int4 c = p[q];
+        v[4 * q + 0] = y[c.x];
+        v[4 * q + 1] = y[c.y];
+        v[4 * q + 2] = y[c.z];
+        v[4 * q + 3] = y[c.w];
       }
+    }
+    // Segmented pass over this lane's slots (no barrier inside: lanes of a
+    // wavefront may diverge here).
+    const bool active = lo < hi;
+    bool owner_open = false;  // this lane holds the first edge of a row that continues
+    int ri = 0;
+    Acc<Real> cont;
+    if (active) {
+      // Row (local index) holding slot lo: last row with offset <= lo.
+      int a = 0, b = nrows - 1;
+      while (a < b) {
+        const int mid = (a + b + 1) >> 1;
+        if (s_off[mid] <= lo) a = mid; else b = mid - 1;
+      }
+      ri = a;
+      int32_t rend = s_off[ri + 1];
+      bool started_here = s_off[ri] >= lo;
+      Acc<Real> acc;
+#pragma unroll
+      for (int q = 0; q < kPerLane; ++q) {
+        const int32_t i = base_rel + q;
+        if (i < lo || i >= hi) continue;
+        while (i >= rend) {
+          // Row ri ended before slot i.
+          if (started_here)
+            finalize_row<Real>(r0 + ri, acc.value(), S, x_old, x_new, y_new, outdeg, resid, dang);
+          else
+            cont = acc;
+          acc = Acc<Real>();
+          ++ri;
+          while (s_off[ri + 1] == s_off[ri]) ++ri;  // skip empty rows
+          rend = s_off[ri + 1];
+          started_here = true;
+        }
+        acc.add(v[q]);
+      }
+      if (!started_here) {
+        cont = acc;  // whole range inside a row that began in an earlier lane
+      } else if (rend <= hi) {
+        finalize_row<Real>(r0 + ri, acc.value(), S, x_old, x_new, y_new, outdeg, resid, dang);
+      } else {
+        owner_open = true;
+        cont = acc;
+      }
+    }
+    s_cs[tid] = cont.s;
+    if constexpr (sizeof(Real) == 4) s_cc[tid] = cont.c;
+    __syncthreads();
+    if (owner_open) {
+      // Add the continuation partials of the lanes covering the rest of the row.
+      const int32_t rend = s_off[ri + 1];
+      const int last_lane = (rend - 1 - (A - e0)) / kPerLane;
+      for (int t2 = tid + 1; t2 <= last_lane; ++t2) {
+        Acc<Real> o;
+        o.s = s_cs[t2];
+        if constexpr (sizeof(Real) == 4) o.c = s_cc[t2];
+        cont.merge(o);
+      }
+      finalize_row<Real>(r0 + ri, cont.value(), S, x_old, x_new, y_new, outdeg, resid, dang);
     }
     __syncthreads();
   }

@@ -15,13 +15,3 @@ extern "C" gx_status gx_pagerank_adjoint(gx_graph*, const float*, const float*, 
   fail(GX_ERR_INVALID, "gx_pagerank_adjoint: not implemented");
   GX_API_END
 }
-extern "C" gx_status gx_bc(gx_graph*, const uint32_t*, uint32_t, double*, gx_stats*) {
-  GX_API_BEGIN
-  fail(GX_ERR_INVALID, "gx_bc: not implemented");
-  GX_API_END
-}
-extern "C" gx_status gx_bc_batched(gx_graph*, const uint32_t*, uint32_t, double*, gx_stats*) {
-  GX_API_BEGIN
-  fail(GX_ERR_INVALID, "gx_bc_batched: not implemented");
-  GX_API_END
-}

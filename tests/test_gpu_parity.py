@@ -316,3 +316,75 @@ def test_repeat_runs_are_identical(gx, rmat16):
     x1 = gx.pagerank(G, precision="fp32")[0].cpu().numpy().copy()
     x2 = gx.pagerank(G, precision="fp32")[0].cpu().numpy().copy()
     np.testing.assert_array_equal(x1, x2)  # fixed-order reductions: deterministic
+
+
+# ------------------------------------------------------------- PageRank shapes
+def test_pagerank_long_rows_and_empty_runs(gx, oracle):
+    """Rows longer than a window (piece sweep), windows with more rows than
+    the LDS row table (lane-per-row fallback), dangling vertices."""
+    say("pagerank long rows / empty runs")
+    rng = np.random.default_rng(7)
+    n = 20000
+    src = list(range(1, 5000)) + list(rng.integers(5000, 6000, 4000))
+    dst = [0] * 4999 + list(rng.integers(5000, 6000, 4000))
+    src += [0, 0, 17]
+    dst += [19999, 5000, 12000]
+    keys = np.unique(np.array(src, np.int64) * n + np.array(dst, np.int64))
+    s, d = keys // n, keys % n
+    keep = s != d
+    s, d = s[keep], d[keep]
+    row = np.zeros(n + 1, np.int64)
+    np.add.at(row, s + 1, 1)
+    row = np.cumsum(row).astype(np.int32)
+    col = d.astype(np.int32)
+    G = gx.Graph.from_csr(row, col, None, symmetric=False)
+    exp, it, _ = oracle.pagerank(row, col, tol=1e-12, max_iter=200)
+    x64, st = gx.pagerank(G, tol=1e-12, max_iter=200, precision="fp64")
+    assert st.iterations == it
+    np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-11, atol=1e-18)
+    x32, _ = gx.pagerank(G, tol=1e-12, max_iter=200, precision="fp32")
+    assert np.abs(x32.cpu().numpy().astype(np.float64) - exp).sum() < 1e-6
+
+
+# ------------------------------------------------------------- BC
+def _bc_close(got, exp, rel=1e-6):
+    got = np.asarray(got, np.float64)
+    exp = np.asarray(exp, np.float64)
+    scale = np.maximum(np.abs(exp), 1.0)
+    assert np.all(np.abs(got - exp) <= rel * scale), np.max(np.abs(got - exp) / scale)
+
+
+def test_bc_golden(gx, golden):
+    for g in golden["undirected"]:
+        G = dev(gx, g, weights=False)
+        n = g["n"]
+        for fn in (gx.betweenness_centrality, gx.betweenness_centrality_batched):
+            say(f"bc {g['name']} {fn.__name__}")
+            bc, _ = fn(G, np.arange(n))
+            _bc_close(bc.cpu().numpy(), g["bc_all"])
+            bc, _ = fn(G, g["bc_sources"])
+            _bc_close(bc.cpu().numpy(), g["bc_subset"])
+
+
+def test_bc_directed_and_duplicates(gx, golden, oracle):
+    for g in golden["directed"]:
+        row, col, _ = csr_arrays(g)
+        G = dev(gx, g, symmetric=False, weights=False)
+        srcs = np.array([0, 3, 3, 7, 11, 0], np.uint32)
+        exp = oracle.bc(row, col, srcs)
+        for fn in (gx.betweenness_centrality, gx.betweenness_centrality_batched):
+            say(f"bc directed {g['name']} {fn.__name__}")
+            _bc_close(fn(G, srcs)[0].cpu().numpy(), exp)
+
+
+def test_bc_rmat16_plain_vs_batched(gx, oracle, rmat16):
+    G, row, col, w = rmat16
+    srcs = gx.pick_sources(row, 150)
+    say("bc rmat16 oracle")
+    exp = oracle.bc(row, col, srcs, threads=8)
+    say("bc rmat16 plain")
+    plain, _ = gx.betweenness_centrality(G, srcs)
+    _bc_close(plain.cpu().numpy(), exp)
+    say("bc rmat16 batched")
+    batched, st = gx.betweenness_centrality_batched(G, srcs)
+    _bc_close(batched.cpu().numpy(), exp)
