@@ -26,7 +26,7 @@ __all__ = [
     "Graph", "GxError", "Stats", "bfs", "bfs_do", "sssp", "connected_components", "connectedComponents",
     "kcore", "pagerank", "pagerank_ref", "pagerank_push", "pagerank_weighted", "pagerank_adjoint",
     "betweenness_centrality", "betweenness_centrality_batched", "pick_sources", "read_gr", "write_gr",
-    "GX_DIST_INFINITY", "GX_BC_INFINITY", "set_profiling",
+    "GX_DIST_INFINITY", "GX_BC_INFINITY", "set_profiling", "differentiable_pagerank",
 ]
 
 KRAND_SEED = 27491095  # lonestar/liblonestar/include/Lonestar/Utils.h:31
@@ -315,24 +315,32 @@ def pagerank_push(g: Graph, alpha: float = 0.85, tol: float = 1e-3, max_iter: in
     return pr[:g.n], _stats(st)
 
 
-def pagerank_weighted(g: Graph, weights, teleport, alpha: float = 0.85, tol: float = 1e-9, max_iter: int = 200):
-    """Forward of the differentiable PageRank (float weights and teleport device tensors)."""
+def pagerank_weighted(g: Graph, weights, teleport, alpha: float = 0.85, tol: float = 1e-9, max_iter: int = 200,
+                      precision: str = "fp32"):
+    """Forward of the differentiable PageRank: weights (float32 device tensor, CSR
+    edge order, or None for the graph's own) and teleport (float32 device
+    tensor).  Returns (x, Stats); x is float32 or float64 per `precision`."""
     torch = _torch()
-    x = _empty(g.n, torch.float32)
+    fp64 = precision == "fp64"
+    x = _empty(g.n, torch.float64 if fp64 else torch.float32)
     st = gx_stats()
-    check(lib().gx_pagerank_weighted(g.handle, _ptr(weights), _ptr(teleport), alpha, tol, max_iter, _ptr(x),
-                                     C.byref(st)))
+    check(lib().gx_pagerank_weighted(g.handle, _ptr(weights), _ptr(teleport), alpha, tol, max_iter, int(fp64),
+                                     _ptr(x), C.byref(st)))
     return x[:g.n], _stats(st)
 
 
-def pagerank_adjoint(g: Graph, weights, x, c, alpha: float = 0.85, tol: float = 1e-9, max_iter: int = 200):
-    """Adjoint of pagerank_weighted for L = c.x: returns (grad_w, grad_t, Stats)."""
+def pagerank_adjoint(g: Graph, weights, x, c, alpha: float = 0.85, tol: float = 1e-9, max_iter: int = 200,
+                     precision: str = "fp32"):
+    """Adjoint of pagerank_weighted for L with dL/dx = c: returns (grad_w, grad_t, Stats)."""
     torch = _torch()
-    gw = _empty(g.m, torch.float32)
-    gt = _empty(g.n, torch.float32)
+    fp64 = precision == "fp64"
+    dt = torch.float64 if fp64 else torch.float32
+    x = x.to(dt).contiguous()
+    gw = _empty(g.m, dt)
+    gt = _empty(g.n, dt)
     st = gx_stats()
-    check(lib().gx_pagerank_adjoint(g.handle, _ptr(weights), _ptr(x), _ptr(c), alpha, tol, max_iter, _ptr(gw),
-                                    _ptr(gt), C.byref(st)))
+    check(lib().gx_pagerank_adjoint(g.handle, _ptr(weights), _ptr(x), _ptr(c.float().contiguous()), alpha, tol,
+                                    max_iter, int(fp64), _ptr(gw), _ptr(gt), C.byref(st)))
     return gw[:g.m], gt[:g.n], _stats(st)
 
 
@@ -353,3 +361,11 @@ def betweenness_centrality(g: Graph, sources):
 def betweenness_centrality_batched(g: Graph, sources):
     """Brandes BC with all sources batched (64 per wavefront); returns (bc float64, Stats)."""
     return _bc(lib().gx_bc_batched, g, sources)
+
+
+def differentiable_pagerank(g: Graph, weights, teleport, alpha: float = 0.85, tol: float = 1e-9,
+                            max_iter: int = 200, precision: str = "fp32"):
+    """PageRank as a torch.autograd function of the edge weights and the teleport
+    vector (see galois_amd.autograd)."""
+    from .autograd import pagerank_fn
+    return pagerank_fn(g, weights, teleport, alpha, tol, max_iter, precision)

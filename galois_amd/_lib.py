@@ -103,8 +103,8 @@ SIGNATURES: dict[str, tuple] = {
     "gx_pagerank_pull": (INT, [P, D, D, U32, INT, P, PST]),
     "gx_pagerank_pull_ref": (INT, [P, INT, F, F, U32, P, PST]),
     "gx_pagerank_push": (INT, [P, F, F, U32, P, PST]),
-    "gx_pagerank_weighted": (INT, [P, P, P, D, D, U32, P, PST]),
-    "gx_pagerank_adjoint": (INT, [P, P, P, P, D, D, U32, P, P, PST]),
+    "gx_pagerank_weighted": (INT, [P, P, P, D, D, U32, INT, P, PST]),
+    "gx_pagerank_adjoint": (INT, [P, P, P, P, D, D, U32, INT, P, P, PST]),
     "gx_bc": (INT, [P, P, U32, P, PST]),
     "gx_bc_batched": (INT, [P, P, U32, P, PST]),
 }

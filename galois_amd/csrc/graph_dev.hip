@@ -290,7 +290,7 @@ void ensure_outdeg(gx_graph* g) {
 }
 
 // Source vertex of every CSR edge (uint32[m]) into `src`.
-static void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipStream_t s) {
+void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipStream_t s) {
   if (m == 0) return;
   fill<uint32_t>(src, 0u, m, s);
   row_heads_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(row, src, n);
@@ -345,6 +345,38 @@ void build_transpose(gx_graph* g) {
   GX_LAUNCHED("lo_bits_kernel");
   GX_HIP(hipStreamSynchronize(s));
   g->has_transpose = true;
+}
+
+__global__ void degree_keys_kernel(const int32_t* __restrict__ row, int64_t n, uint64_t* __restrict__ keys) {
+  int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; v < n; v += stride) {
+    const uint32_t d = static_cast<uint32_t>(row[v + 1] - row[v]);
+    keys[v] = (static_cast<uint64_t>(0x7fffffffu - d) << 31) | static_cast<uint64_t>(v);
+  }
+}
+
+__global__ void rank_from_sorted_kernel(const uint64_t* __restrict__ keys, int64_t n, const int32_t* __restrict__ row,
+                                        uint32_t* __restrict__ old_of_new, uint32_t* __restrict__ new_of_old,
+                                        uint32_t* __restrict__ outdeg_new) {
+  int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) {
+    const uint32_t v = static_cast<uint32_t>(keys[i] & 0x7fffffffull);
+    old_of_new[i] = v;
+    new_of_old[v] = static_cast<uint32_t>(i);
+    outdeg_new[i] = static_cast<uint32_t>(row[v + 1] - row[v]);
+  }
+}
+
+// Relabelled in-edge keys: (new dst << bits) | new src.
+__global__ void relabel_keys_kernel(const int32_t* __restrict__ col, const uint32_t* __restrict__ src,
+                                    const uint32_t* __restrict__ new_of_old, uint64_t* __restrict__ keys, int64_t m,
+                                    int bits) {
+  int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; e < m; e += stride)
+    keys[e] = (static_cast<uint64_t>(new_of_old[col[e]]) << bits) | new_of_old[src[e]];
 }
 
 // Sort `cnt` edge keys (u << bits | v), drop duplicates and self-loops, and
@@ -402,6 +434,70 @@ static void csr_from_keys(gx_graph* g, DevBuf& keys, int64_t cnt, int bits, int6
     fill<int32_t>(g->row, 0, n + 1, s);
   }
   GX_HIP(hipStreamSynchronize(s));
+}
+
+void build_degree_order(gx_graph* g) {
+  DegreeOrder& D = g->degree_order;
+  if (D.ready) return;
+  const int64_t n = g->n, m = g->m;
+  hipStream_t s = g->stream;
+  const int bits = bits_for(n);
+  D.new_of_old.ensure(std::max<int64_t>(n, 1) * 4);
+  D.old_of_new.ensure(std::max<int64_t>(n, 1) * 4);
+  D.outdeg.ensure(std::max<int64_t>(n, 1) * 4);
+  D.trow.ensure((n + 1) * 4);
+  D.tcol.ensure((m + kEdgePad) * 4);
+  fill<int32_t>(D.tcol.as<int32_t>() + m, 0, kEdgePad, s);
+  {
+    DevBuf k0, k1;
+    k0.ensure(std::max<int64_t>(n, 1) * 8);
+    k1.ensure(std::max<int64_t>(n, 1) * 8);
+    degree_keys_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(g->row, n, k0.as<uint64_t>());
+    GX_LAUNCHED("degree_keys_kernel");
+    size_t tmp = 0;
+    GX_HIP(rocprim::radix_sort_keys(nullptr, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(n), 0,
+                                    62, s));
+    DevBuf t;
+    t.ensure(tmp);
+    GX_HIP(rocprim::radix_sort_keys(t.p, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(n), 0, 62,
+                                    s));
+    rank_from_sorted_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), n, g->row,
+                                                                       D.old_of_new.as<uint32_t>(),
+                                                                       D.new_of_old.as<uint32_t>(),
+                                                                       D.outdeg.as<uint32_t>());
+    GX_LAUNCHED("rank_from_sorted_kernel");
+  }
+  if (m == 0) {
+    fill<int32_t>(D.trow.as<int32_t>(), 0, n + 1, s);
+  } else {
+    DevBuf src, k0, k1;
+    src.ensure(m * 4);
+    k0.ensure(m * 8);
+    k1.ensure(m * 8);
+    edge_sources(g->row, n, m, src.as<uint32_t>(), s);
+    relabel_keys_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(g->col, src.as<uint32_t>(),
+                                                                   D.new_of_old.as<uint32_t>(), k0.as<uint64_t>(), m,
+                                                                   bits);
+    GX_LAUNCHED("relabel_keys_kernel");
+    src.release();
+    size_t tmp = 0;
+    GX_HIP(rocprim::radix_sort_keys(nullptr, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(m), 0,
+                                    2 * bits, s));
+    DevBuf t;
+    t.ensure(tmp);
+    GX_HIP(rocprim::radix_sort_keys(t.p, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(m), 0,
+                                    2 * bits, s));
+    t.release();
+    k0.release();
+    offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), m, bits, n,
+                                                                          D.trow.as<int32_t>());
+    GX_LAUNCHED("offsets_from_sorted_kernel");
+    lo_bits_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), D.tcol.as<int32_t>(), m,
+                                                              (uint64_t(1) << bits) - 1);
+    GX_LAUNCHED("lo_bits_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  D.ready = true;
 }
 
 static void add_pair_weights(gx_graph* g, uint64_t seed, uint32_t lo, uint32_t hi) {

@@ -110,6 +110,16 @@ struct PullPlan {
   DevBuf buf;
 };
 
+// The pull graph of PageRank in degree-sorted vertex order: new id = rank of
+// the vertex by (out-degree descending, id ascending).  Sources that are
+// gathered most often get the lowest ids, so the hot part of the gathered
+// vector is a few contiguous megabytes.  trow/tcol are the transpose in new
+// ids (tcol padded by kEdgePad), outdeg is in new order.
+struct DegreeOrder {
+  bool ready = false;
+  DevBuf new_of_old, old_of_new, trow, tcol, outdeg;
+};
+
 }  // namespace gx
 
 // The opaque graph handle of the C ABI.
@@ -128,7 +138,9 @@ struct gx_graph {
   // Derived per-vertex data, built on demand.
   gx::DevBuf outdeg_buf;  // uint32 out-degree (for transpose-based PR)
   bool has_outdeg = false;
-  gx::PullPlan pull_plan;
+  gx::PullPlan pull_plan;            // degree-ordered transpose (PageRank pull)
+  gx::PullPlan csc_plan, csr_plan;   // plain transpose / CSR (weighted PageRank and adjoint)
+  gx::DegreeOrder degree_order;
   std::shared_ptr<void> bc_cache;  // batched-BC tables (bc.hip)
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // non-null stream for HIP graph capture
@@ -230,6 +242,9 @@ inline unsigned grid_for(int64_t items, int block, int64_t max_blocks = 1 << 20)
 // Device-side helpers exported by graph_dev.hip for other translation units.
 void ensure_outdeg(gx_graph* g);    // g->outdeg_buf = out-degree per vertex
 void build_transpose(gx_graph* g);  // CSC + permutation (deterministic)
+void build_degree_order(gx_graph* g);  // g->degree_order (relabelled transpose)
+// Source vertex of every CSR edge into src[m].
+void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipStream_t s);
 // Exclusive prefix sum of `count` uint32 values into uint32 `out` (may alias).
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s);
 // Device->host copy of a few bytes through the graph's pinned block.

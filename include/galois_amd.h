@@ -245,23 +245,28 @@ gx_status gx_pagerank_pull_ref(gx_graph* g, int variant, float alpha,
 gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol,
                            uint32_t max_iter, float* pr, gx_stats* st);
 
-/* Differentiable PageRank on a float-weighted directed graph
- * (GX_GRAPH_FWEIGHTED).  P[u,v] = w_uv / sum_v' w_uv'; dangling rows are
- * uniform; x = alpha*P_dang^T x + (1-alpha)*t.  FP32 Kahan iteration.
- * forward: x (device float[n]) from weights (device float[m], CSR order,
- * overriding the graph's own when non-NULL) and teleport t (device float[n]).
- * adjoint: given c = dL/dx, solves lambda = alpha*P_dang lambda + c and
- * writes grad_w (device float[m], CSR order) = dL/dw and grad_t
- * (device float[n]) = dL/dt = (1-alpha)*lambda.  x is the converged forward
- * vector (the only state the backward pass needs). */
-gx_status gx_pagerank_weighted(gx_graph* g, const float* weights,
-                               const float* teleport, double alpha,
-                               double tol, uint32_t max_iter, float* x,
-                               gx_stats* st);
-gx_status gx_pagerank_adjoint(gx_graph* g, const float* weights,
-                              const float* x, const float* c, double alpha,
-                              double tol, uint32_t max_iter, float* grad_w,
-                              float* grad_t, gx_stats* st);
+/* Differentiable PageRank on a weighted directed graph.
+ * P[u,v] = w_uv / sum_v' w_uv'; rows with no out-edge (or zero weight sum)
+ * are dangling and spread uniformly; x = alpha*P^T x + alpha*(D/n) +
+ * (1-alpha)*t, D = dangling mass, solved by Jacobi iteration to
+ * sum|x'-x| < tol or max_iter (x0 = 1/n).  weights: device float[m] in CSR
+ * edge order (NULL = the graph's own float weights); teleport: device
+ * float[n].  precision GX_PR_FP32 (compensated sums) or GX_PR_FP64; x is a
+ * device float[n] or double[n] accordingly.
+ * Extends computePRTopological (PageRank-pull.cpp:196-251) to edge weights
+ * and a teleport vector; no reference counterpart for the gradient. */
+gx_status gx_pagerank_weighted(gx_graph* g, const float* weights, const float* teleport, double alpha,
+                               double tol, uint32_t max_iter, int precision, void* x, gx_stats* st);
+/* Adjoint of gx_pagerank_weighted for a loss L with dL/dx = c (device
+ * float[n]): solves lambda = c + alpha*(P lambda + (sum(lambda)/n) on dangling
+ * rows) by the transposed power iteration on the same CSR, then writes
+ * grad_w[e=(u,v)] = alpha*x[u]*(lambda[v] - (P lambda)[u]) / s_u (CSR edge
+ * order) and grad_t = (1-alpha)*lambda.  x is the converged forward vector
+ * (the only forward state the backward pass needs); x, grad_w, grad_t are
+ * float or double per `precision`. */
+gx_status gx_pagerank_adjoint(gx_graph* g, const float* weights, const void* x, const float* c, double alpha,
+                              double tol, uint32_t max_iter, int precision, void* grad_w, void* grad_t,
+                              gx_stats* st);
 
 /* Brandes betweenness centrality, one source per pass (plain).
  * bc[v] (device double[n]) = sum over the given sources s != v of the

@@ -363,28 +363,30 @@ int oracle_kcore(int64_t n, const int32_t* row, const int32_t* col, uint32_t k, 
 //   D = sum_{outdeg(u)=0} x[u];  stop when sum|x'-x| < tol or max_iter.
 // Pinned against a direct sparse solve of (I - a*M^T) x = (1-a)/n (scipy
 // spsolve fixtures) and sum(x) = 1.
-int oracle_pagerank_fp64(int64_t n, const int32_t* row, const int32_t* col, double alpha, double tol,
-                         uint32_t max_iter, double* x_out, uint32_t* iters_out, double* resid_out, int threads) {
+int oracle_pagerank_fp64_csc(int64_t n, const uint32_t* outdeg, const int32_t* trow, const int32_t* tcol,
+                             double alpha, double tol, uint32_t max_iter, double* x_out, uint32_t* iters_out,
+                             double* resid_out, int threads) {
   Pool pool(threads);
-  std::vector<int32_t> trow, tcol;
-  transpose(n, row, col, trow, tcol, nullptr);
   std::vector<double> x(n, 1.0 / static_cast<double>(n)), y(n), xn(n);
-  std::vector<double> part(pool.size());
   uint32_t it = 0;
   double resid = 0;
+  // Blocked so that per-block partial sums add up in a fixed order whatever
+  // the thread count.
+  const int64_t blk = 4096, nblk = (n + blk - 1) / blk;
+  std::vector<double> bres(nblk, 0.0), bdang(nblk, 0.0);
   while (it < max_iter) {
+    do_all(pool, 0, nblk, 1, [&](int64_t b, int) {
+      double dg = 0;
+      for (int64_t u = b * blk; u < std::min(n, (b + 1) * blk); ++u) {
+        const uint32_t d = outdeg[u];
+        if (d == 0) dg += x[u];
+        y[u] = d ? x[u] / d : 0.0;
+      }
+      bdang[b] = dg;
+    });
     double dang = 0;
-    for (int64_t u = 0; u < n; ++u) {
-      int32_t d = row[u + 1] - row[u];
-      if (d == 0) dang += x[u];
-      y[u] = d ? x[u] / d : 0.0;
-    }
+    for (double d : bdang) dang += d;
     const double base = (1.0 - alpha) / n + alpha * dang / n;
-    std::fill(part.begin(), part.end(), 0.0);
-    // Blocked so that the per-block partial sums of |x'-x| add up in a fixed
-    // order regardless of the thread count.
-    const int64_t blk = 4096, nblk = (n + blk - 1) / blk;
-    std::vector<double> bres(nblk, 0.0);
     do_all(pool, 0, nblk, 1, [&](int64_t b, int) {
       double r = 0;
       for (int64_t v = b * blk; v < std::min(n, (b + 1) * blk); ++v) {
@@ -405,6 +407,16 @@ int oracle_pagerank_fp64(int64_t n, const int32_t* row, const int32_t* col, doub
   if (iters_out) *iters_out = it;
   if (resid_out) *resid_out = resid;
   return 0;
+}
+
+int oracle_pagerank_fp64(int64_t n, const int32_t* row, const int32_t* col, double alpha, double tol,
+                         uint32_t max_iter, double* x_out, uint32_t* iters_out, double* resid_out, int threads) {
+  std::vector<int32_t> trow, tcol;
+  transpose(n, row, col, trow, tcol, nullptr);
+  std::vector<uint32_t> outdeg(n);
+  for (int64_t u = 0; u < n; ++u) outdeg[u] = static_cast<uint32_t>(row[u + 1] - row[u]);
+  return oracle_pagerank_fp64_csc(n, outdeg.data(), trow.data(), tcol.data(), alpha, tol, max_iter, x_out,
+                                  iters_out, resid_out, threads);
 }
 
 // Pull-residual PageRank in FP32, computePRResidual

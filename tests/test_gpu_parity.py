@@ -388,3 +388,100 @@ def test_bc_rmat16_plain_vs_batched(gx, oracle, rmat16):
     say("bc rmat16 batched")
     batched, st = gx.betweenness_centrality_batched(G, srcs)
     _bc_close(batched.cpu().numpy(), exp)
+
+
+# ------------------------------------------------------------- differentiable PageRank
+def _norm_rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def test_weighted_pagerank_golden(gx, golden):
+    import torch
+    for g in golden["weighted"]:
+        say(f"weighted pagerank {g['name']}")
+        row = np.asarray(g["row"], np.int32)
+        col = np.asarray(g["col"], np.int32)
+        w = np.asarray(g["w"], np.float32)
+        G = gx.Graph.from_csr(row, col, w, symmetric=False)
+        wt = torch.tensor(w, device="cuda")
+        t = torch.tensor(np.asarray(g["t"], np.float32), device="cuda")
+        c = torch.tensor(np.asarray(g["c"], np.float32), device="cuda")
+        x64, _ = gx.pagerank_weighted(G, wt, t, alpha=g["alpha"], tol=1e-15, max_iter=2000, precision="fp64")
+        # the golden system was built from the same float32 weights
+        np.testing.assert_allclose(x64.cpu().numpy(), g["x"], rtol=1e-6)
+        gw, gt, _ = gx.pagerank_adjoint(G, wt, x64, c, alpha=g["alpha"], tol=1e-15, max_iter=2000,
+                                        precision="fp64")
+        np.testing.assert_allclose(gw.cpu().numpy(), g["grad_w"], rtol=1e-4, atol=1e-7)
+        np.testing.assert_allclose(gt.cpu().numpy(), g["grad_t"], rtol=1e-4, atol=1e-7)
+
+
+def test_adjoint_vs_oracle_and_finite_differences(gx, oracle):
+    import torch
+    say("adjoint rmat12 lognormal")
+    G = gx.Graph.rmat(12, 8, symmetric=False, weights="lognormal")
+    row, col, w = G.to_host()
+    n, m = G.n, G.m
+    rng = np.random.default_rng(5)
+    t = rng.random(n).astype(np.float32)
+    t /= t.sum()
+    c = rng.normal(size=n).astype(np.float32)
+    wt, tt, ct = (torch.tensor(a, device="cuda") for a in (w, t, c))
+    x_or = oracle.pagerank_weighted(row, col, w, t, alpha=0.85, tol=1e-15, max_iter=3000)
+    gw_or, gt_or = oracle.pagerank_adjoint(row, col, w, x_or, c, alpha=0.85, tol=1e-15, max_iter=3000)
+    x64, _ = gx.pagerank_weighted(G, wt, tt, tol=1e-15, max_iter=3000, precision="fp64")
+    np.testing.assert_allclose(x64.cpu().numpy(), x_or, rtol=1e-9)
+    gw64, gt64, _ = gx.pagerank_adjoint(G, wt, x64, ct, tol=1e-15, max_iter=3000, precision="fp64")
+    np.testing.assert_allclose(gw64.cpu().numpy(), gw_or, rtol=1e-8, atol=1e-14)
+    np.testing.assert_allclose(gt64.cpu().numpy(), gt_or, rtol=1e-8, atol=1e-14)
+    # FP32 forward + adjoint against the FP64 gradient: 1e-5 (norm-relative)
+    x32, st_f = gx.pagerank_weighted(G, wt, tt, tol=1e-9, max_iter=300, precision="fp32")
+    gw32, gt32, st_b = gx.pagerank_adjoint(G, wt, x32, ct, tol=1e-9, max_iter=300, precision="fp32")
+    assert _norm_rel(gw32.cpu().numpy(), gw_or) < 1e-5
+    assert _norm_rel(gt32.cpu().numpy(), gt_or) < 1e-5
+    # central finite differences (eps = 1e-4) of the FP64 oracle on sampled edges
+    idx = rng.choice(m, 40, replace=False)
+    for e in idx:
+        wp = w.astype(np.float64).copy()
+        wm = wp.copy()
+        wp[e] += 1e-4
+        wm[e] -= 1e-4
+        fd = (np.dot(c, oracle.pagerank_weighted(row, col, wp, t, tol=1e-15, max_iter=3000))
+              - np.dot(c, oracle.pagerank_weighted(row, col, wm, t, tol=1e-15, max_iter=3000))) / 2e-4
+        assert abs(fd - gw64.cpu().numpy()[e]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9
+
+
+def test_autograd_function(gx):
+    import torch
+    say("autograd")
+    G = gx.Graph.rmat(10, 8, symmetric=False, weights="lognormal")
+    _, _, w = G.to_host()
+    rng = np.random.default_rng(9)
+    w = torch.tensor(w, device="cuda", requires_grad=True)
+    t = torch.full((G.n,), 1.0 / G.n, device="cuda", requires_grad=True)
+    c = torch.tensor(rng.normal(size=G.n).astype(np.float32), device="cuda")
+    x = gx.differentiable_pagerank(G, w, t, tol=1e-9, max_iter=300)
+    loss = (c * x).sum()
+    loss.backward()
+    gw, gt, _ = gx.pagerank_adjoint(G, w.detach(), x.detach(), c, tol=1e-9, max_iter=300)
+    np.testing.assert_array_equal(w.grad.cpu().numpy(), gw.cpu().numpy())
+    np.testing.assert_array_equal(t.grad.cpu().numpy(), gt.cpu().numpy())
+
+
+# ------------------------------------------------------------- uneven graphs in one launch
+def test_batched_bc_over_union_of_uneven_graphs(gx, oracle):
+    say("union of uneven graphs")
+    rng = np.random.default_rng(3)
+    parts = [gx.Graph.rmat(int(rng.integers(7, 12)), int(rng.integers(4, 17)), seed=100 + i, symmetric=True)
+             for i in range(12)]
+    U, offs = gx.Graph.union(parts)
+    srcs = []
+    exp = np.zeros(U.n)
+    for i, P in enumerate(parts):
+        row, col, _ = P.to_host()
+        s = gx.pick_sources(row, 3, seed=7 + i)
+        srcs.extend((s + offs[i]).tolist())
+        exp[offs[i]:offs[i + 1]] = oracle.bc(row, col, s)
+    bc, st = gx.betweenness_centrality_batched(U, np.array(srcs, np.uint32))
+    _bc_close(bc.cpu().numpy(), exp)
