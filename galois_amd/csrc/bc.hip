@@ -42,7 +42,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kInf = GX_BC_INFINITY;
-constexpr uint32_t kHub = 4096;
+constexpr uint32_t kHub = 1024;
 
 // ================================================================== plain
 struct PlainCounters {
@@ -81,30 +81,54 @@ __global__ void bc_plain_clear_kernel(PlainCounters* c) {
   }
 }
 
+constexpr int kQCap = 2048;
+
+// Forward discovery from level L: claim unreached neighbours for L+1 and stage
+// them in LDS (the next level's list).
 struct FwdOp {
   const int32_t* col;
   uint32_t* level;
-  float* sigma;
   uint32_t* next;
   PlainCounters* c;
+  LdsQueue<kQCap>* sq;
   uint32_t nl;
   unsigned cap;
-  __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
+  __device__ __forceinline__ void operator()(bool valid, uint32_t, int32_t e) {
     bool won = false;
     uint32_t v = 0;
     if (valid) {
       v = static_cast<uint32_t>(col[e]);
-      uint32_t old = level[v];
-      if (old == kInf) {
-        old = atomicCAS(&level[v], kInf, nl);
-        won = old == kInf;
-      }
-      if (won || old == nl) atomicAdd(&sigma[v], sigma[u]);
+      won = level[v] == kInf && atomicCAS(&level[v], kInf, nl) == kInf;
     }
-    wave_append<uint32_t>(won, v, next, &c->count, cap, &c->overflow);
+    lq_push(*sq, won, v);
   }
+  __device__ __forceinline__ void flush(bool force) { lq_flush(*sq, next, &c->count, cap, &c->overflow, force); }
+  __device__ __forceinline__ void after_step() { flush(false); }
 };
 
+// Path counts of level L+1 by a pull over in-edges: sigma[v] = sum of
+// sigma[u] over in-neighbours at level L.  Consecutive lanes hold edges of the
+// same vertex, so a wavefront segmented sum precedes one atomicAdd per run.
+struct SigmaOp {
+  const int32_t* col;
+  const uint32_t* level;
+  float* sigma;
+  uint32_t pred;
+  __device__ __forceinline__ void operator()(bool valid, uint32_t v, int32_t e) {
+    float x = 0.f;
+    if (valid) {
+      const uint32_t u = static_cast<uint32_t>(col[e]);
+      if (level[u] == pred) x = sigma[u];
+    }
+    bool tail;
+    const float sum = wave_segmented_sum(valid ? v : 0xffffffffu, x, tail);
+    if (valid && tail && sum != 0.f) atomicAdd(&sigma[v], sum);
+  }
+  __device__ __forceinline__ void after_step() {}
+};
+
+// Dependencies of level L: delta[v] += sum over successors w at L+1 of
+// (1 + delta[w]) / sigma[w], segmented per vertex before the atomicAdd.
 struct BwdOp {
   const int32_t* col;
   const uint32_t* level;
@@ -112,10 +136,16 @@ struct BwdOp {
   double* delta;
   uint32_t succ;
   __device__ __forceinline__ void operator()(bool valid, uint32_t v, int32_t e) {
-    if (!valid) return;
-    const uint32_t w = static_cast<uint32_t>(col[e]);
-    if (level[w] == succ) atomicAdd(&delta[v], (1.0 + delta[w]) / static_cast<double>(sigma[w]));
+    double x = 0.0;
+    if (valid) {
+      const uint32_t w = static_cast<uint32_t>(col[e]);
+      if (level[w] == succ) x = (1.0 + delta[w]) / static_cast<double>(sigma[w]);
+    }
+    bool tail;
+    const double sum = wave_segmented_sum(valid ? v : 0xffffffffu, x, tail);
+    if (valid && tail && sum != 0.0) atomicAdd(&delta[v], sum);
   }
+  __device__ __forceinline__ void after_step() {}
 };
 
 template <class Op>
@@ -129,8 +159,31 @@ __global__ void __launch_bounds__(kBlock) bc_expand_kernel(const int32_t* __rest
 template <class Op>
 __global__ void __launch_bounds__(kBlock) bc_hubs_kernel(const int32_t* __restrict__ row, Op op,
                                                          const uint32_t* __restrict__ hubq,
-                                                         const PlainCounters* c) {
-  expand_hubs<kBlock>(row, hubq, c->hub_count, op);
+                                                         const PlainCounters* c, unsigned cap,
+                                                         const uint32_t* __restrict__ hub_off) {
+  if (c->hub_count == 0) return;
+  expand_hubs<kBlock>(row, hubq, c->hub_count, cap, hub_off, op);
+}
+
+__global__ void __launch_bounds__(kBlock) bc_fwd_expand_kernel(const int32_t* __restrict__ row, FwdOp op,
+                                                               const uint32_t* __restrict__ q, unsigned qcount,
+                                                               uint32_t* __restrict__ hubq) {
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  op.sq = &sq;
+  expand_queue<kBlock>(row, q, qcount, kHub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  op.flush(true);
+}
+
+__global__ void __launch_bounds__(kBlock) bc_fwd_hubs_kernel(const int32_t* __restrict__ row, FwdOp op,
+                                                             const uint32_t* __restrict__ hubq,
+                                                             const uint32_t* __restrict__ hub_off) {
+  if (op.c->hub_count == 0) return;
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  op.sq = &sq;
+  expand_hubs<kBlock>(row, hubq, op.c->hub_count, op.cap, hub_off, op);
+  op.flush(true);
 }
 
 __global__ void bc_plain_finish_level_kernel(const uint32_t* __restrict__ q, unsigned cnt,
@@ -483,6 +536,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
   double* delta = g->scratch<double>(WS_BC_DELTA, cap);
   uint32_t* order = g->scratch<uint32_t>(WS_Q0, cap);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, cap);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
   PlainCounters* c = g->scratch<PlainCounters>(WS_COUNTERS, 1);
   PlainCounters* host = static_cast<PlainCounters*>(g->host_pinned(sizeof(PlainCounters)));
   RegionTimer region(s);
@@ -494,30 +548,52 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
   GX_LAUNCHED("bc_plain_init_kernel");
   launches += 2;
   const unsigned grid = 256 * 8;
+  // In-edges for the path-count pull (the CSR itself for symmetric graphs).
+  const int32_t* irow = g->row;
+  const int32_t* icol = g->col;
+  if (!(g->flags & GX_GRAPH_SYMMETRIC)) {
+    build_transpose(g);
+    irow = g->trow();
+    icol = g->tcol();
+  }
   std::vector<uint32_t> lstart;
   for (uint32_t si = 0; si < count; ++si) {
     const uint32_t src = sources[si];
     bc_plain_source_kernel<<<1, 64, 0, s>>>(level, sigma, order, c, src);
     lstart.assign({0u, 1u});
     uint32_t L = 0;
-    // Forward: level L's vertices are order[lstart[L], lstart[L+1]).
+    // Forward: level L's vertices are order[lstart[L], lstart[L+1]).  Each
+    // level is a discovery push followed by the path-count pull of the new
+    // level.
     ktimer.begin();
     for (;;) {
       const uint32_t b = lstart[L], e = lstart[L + 1];
       const unsigned cnt = e - b;
       if (cnt == 0) break;
       bc_plain_clear_kernel<<<1, 64, 0, s>>>(c);
-      FwdOp op{g->col, level, sigma, order + e, c, L + 1, cap - e};
-      bc_expand_kernel<FwdOp><<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(
-          g->row, op, order + b, cnt, hubq, c, cap);
-      bc_hubs_kernel<FwdOp><<<1024, kBlock, 0, s>>>(g->row, op, hubq, c);
+      FwdOp op{g->col, level, order + e, c, nullptr, L + 1, cap - e};
+      bc_fwd_expand_kernel<<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(g->row, op,
+                                                                                               order + b, cnt, hubq);
+      hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+      bc_fwd_hubs_kernel<<<grid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
       GX_LAUNCHED("bc forward");
       launches += 3;
       GX_HIP(hipMemcpyAsync(host, c, sizeof(PlainCounters), hipMemcpyDeviceToHost, s));
       GX_HIP(hipStreamSynchronize(s));
       if (host->overflow) fail(GX_ERR_CAPACITY, "BC level list overflow");
-      lstart.push_back(e + host->count);
+      const unsigned ncnt = host->count;
+      lstart.push_back(e + ncnt);
       ++L;
+      if (ncnt > 0) {
+        bc_plain_clear_kernel<<<1, 64, 0, s>>>(c);
+        SigmaOp sop{icol, level, sigma, L - 1};
+        bc_expand_kernel<SigmaOp><<<std::min<unsigned>(grid, grid_for(ncnt, kBlock)), kBlock, 0, s>>>(
+            irow, sop, order + e, ncnt, hubq, c, cap);
+        hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(irow, hubq, &c->hub_count, cap, hub_off);
+        bc_hubs_kernel<SigmaOp><<<grid, kBlock, 0, s>>>(irow, sop, hubq, c, cap, hub_off);
+        GX_LAUNCHED("bc sigma");
+        launches += 3;
+      }
     }
     // Levels 0..L-1 are non-empty.  Backward from L-2 down to 1.
     const uint32_t deepest = L - 1;
@@ -528,7 +604,8 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
       BwdOp op{g->col, level, sigma, delta, static_cast<uint32_t>(lv + 1)};
       bc_expand_kernel<BwdOp><<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(
           g->row, op, order + b, cnt, hubq, c, cap);
-      bc_hubs_kernel<BwdOp><<<1024, kBlock, 0, s>>>(g->row, op, hubq, c);
+      hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+      bc_hubs_kernel<BwdOp><<<grid, kBlock, 0, s>>>(g->row, op, hubq, c, cap, hub_off);
       bc_plain_finish_level_kernel<<<grid_for(cnt, kBlock, 4096), kBlock, 0, s>>>(order + b, cnt, sigma, delta, bc);
       GX_LAUNCHED("bc backward");
       launches += 4;

@@ -11,11 +11,19 @@
 // app's hook/jump scheme (lonestar/analytics/gpu/connected-components/
 // cc.cu:244-357) leaves non-minimum roots; this kernel set does not.
 //
-// Design (gfx950): one hooking pass over all edges, load balanced with the
-// workgroup-local expansion of frontier.hpp over the identity frontier (hubs
-// spread over the grid); find() uses path halving with relaxed agent-scope
-// atomics; linking is one atomicCAS on the higher root.  A final sweep writes
-// the fully compressed root of every vertex.
+// Design (gfx950): the sampling scheme of the reference's Afforest variant
+// (ConnectedComponents.cpp:654-759) on a lock-free GPU union-find:
+//  1. link every vertex with its first two neighbours (two vertex-parallel
+//     passes), then compress;
+//  2. estimate the largest component from 1024 hashed samples of roots;
+//  3. hook the remaining edges of every vertex outside that component (load
+//     balanced with frontier.hpp over a compacted vertex list; hubs spread
+//     over the grid).  Edges of vertices inside it need no hooking: the graph
+//     is symmetric, so their other endpoint hooks the edge if it matters;
+//  4. compress.
+// find() uses path halving with relaxed agent-scope atomics; linking is one
+// atomicCAS of the higher root onto the lower one, so roots stay component
+// minima and the labels are the canonical min labels.
 #include <algorithm>
 
 #include "frontier.hpp"
@@ -27,7 +35,7 @@ using namespace gx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kHub = 4096;
+constexpr uint32_t kHub = 1024;
 
 __device__ __forceinline__ uint32_t ld(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -71,26 +79,97 @@ __global__ void cc_init_kernel(uint32_t* __restrict__ parent, int64_t n) {
   for (; v < n; v += stride) parent[v] = static_cast<uint32_t>(v);
 }
 
+// Sampling pass r: link v with its r-th neighbour.
+__global__ void cc_sample_kernel(const int32_t* __restrict__ row, const int32_t* __restrict__ col,
+                                 uint32_t* __restrict__ parent, int64_t n, int r) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int32_t b = row[v];
+    if (row[v + 1] - b > r) unite(parent, static_cast<uint32_t>(v), static_cast<uint32_t>(col[b + r]));
+  }
+}
+
+// Roots of 1024 hashed sample vertices; the most frequent one is taken as the
+// largest component (single workgroup, counts by comparison in LDS).
+__global__ void __launch_bounds__(256) cc_largest_kernel(uint32_t* __restrict__ parent, int64_t n,
+                                                         uint32_t* __restrict__ out) {
+  constexpr int kSamples = 1024;
+  __shared__ uint32_t roots[kSamples];
+  __shared__ unsigned best_cnt[256];
+  __shared__ uint32_t best_root[256];
+  for (int i = threadIdx.x; i < kSamples; i += 256) {
+    const uint64_t v = (mix64(0x43434343u + static_cast<uint64_t>(i)) >> 11) % static_cast<uint64_t>(n);
+    roots[i] = find_root(parent, static_cast<uint32_t>(v));
+  }
+  __syncthreads();
+  unsigned bc = 0;
+  uint32_t br = 0;
+  for (int i = threadIdx.x; i < kSamples; i += 256) {
+    unsigned c = 0;
+    for (int j = 0; j < kSamples; ++j) c += roots[j] == roots[i];
+    if (c > bc || (c == bc && roots[i] < br)) {
+      bc = c;
+      br = roots[i];
+    }
+  }
+  best_cnt[threadIdx.x] = bc;
+  best_root[threadIdx.x] = br;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int t = 1; t < 256; ++t)
+      if (best_cnt[t] > bc || (best_cnt[t] == bc && best_root[t] < br)) {
+        bc = best_cnt[t];
+        br = best_root[t];
+      }
+    *out = br;
+  }
+}
+
+constexpr int kQCap = 2048;
+
+// Vertices outside the largest component that still have unhooked edges.
+__global__ void __launch_bounds__(kBlock) cc_rest_list_kernel(const int32_t* __restrict__ row,
+                                                              uint32_t* __restrict__ parent, int64_t n,
+                                                              const uint32_t* __restrict__ largest,
+                                                              uint32_t* __restrict__ q, unsigned* count,
+                                                              unsigned cap, unsigned* overflow) {
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  const uint32_t c = *largest;
+  for (int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x); b < n;
+       b += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const int64_t v = b + threadIdx.x;
+    bool keep = false;
+    if (v < n && row[v + 1] - row[v] > 2) keep = find_root(parent, static_cast<uint32_t>(v)) != c;
+    lq_push(sq, keep, static_cast<uint32_t>(v));
+    lq_flush(sq, q, count, cap, overflow, false);
+  }
+  lq_flush(sq, q, count, cap, overflow, true);
+}
+
 struct HookOp {
+  const int32_t* row;
   const int32_t* col;
   uint32_t* parent;
+  __device__ __forceinline__ void after_step() {}
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
-    if (!valid) return;
-    const uint32_t v = static_cast<uint32_t>(col[e]);
-    if (v < u) unite(parent, u, v);
+    if (!valid || e - row[u] < 2) return;  // the first two edges were sampled
+    unite(parent, u, static_cast<uint32_t>(col[e]));
   }
 };
 
-__global__ void __launch_bounds__(kBlock) cc_hook_kernel(const int32_t* __restrict__ row, HookOp op, unsigned n,
+__global__ void __launch_bounds__(kBlock) cc_hook_kernel(const int32_t* __restrict__ row, HookOp op,
+                                                         const uint32_t* __restrict__ q, const unsigned* count,
                                                          uint32_t* __restrict__ hubq, unsigned* hub_count,
-                                                         unsigned* overflow) {
-  expand_queue<kBlock>(row, nullptr, n, kHub, hubq, hub_count, n, overflow, op);
+                                                         unsigned cap, unsigned* overflow) {
+  expand_queue<kBlock>(row, q, *count, kHub, hubq, hub_count, cap, overflow, op);
 }
 
 __global__ void __launch_bounds__(kBlock) cc_hubs_kernel(const int32_t* __restrict__ row, HookOp op,
                                                          const uint32_t* __restrict__ hubq,
-                                                         const unsigned* hub_count) {
-  expand_hubs<kBlock>(row, hubq, *hub_count, op);
+                                                         const unsigned* hub_count, unsigned cap,
+                                                         const uint32_t* __restrict__ hub_off) {
+  expand_hubs<kBlock>(row, hubq, *hub_count, cap, hub_off, op);
 }
 
 __global__ void cc_compress_kernel(uint32_t* __restrict__ parent, uint32_t* __restrict__ label, int64_t n) {
@@ -114,30 +193,40 @@ extern "C" gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* stp) {
   hipStream_t s = g->stream;
   const unsigned cap = static_cast<unsigned>(std::max<int64_t>(n, 1));
   uint32_t* parent = g->scratch<uint32_t>(WS_V0, cap);
+  uint32_t* rest = g->scratch<uint32_t>(WS_Q1, cap);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, cap);
-  unsigned* ctr = g->scratch<unsigned>(WS_COUNTERS, 4);  // [0] hubs, [1] overflow
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
+  unsigned* ctr = g->scratch<unsigned>(WS_COUNTERS, 4);  // [0] hubs, [1] overflow, [2] rest, [3] largest
   RegionTimer region(s);
   KernelTimer ktimer(s);
+  const unsigned vgrid = grid_for(n, kBlock, 4096);
   if (n > 0) {
     cc_zero_kernel<<<1, 64, 0, s>>>(ctr);
-    cc_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(parent, n);
+    cc_init_kernel<<<vgrid, kBlock, 0, s>>>(parent, n);
     GX_LAUNCHED("cc_init_kernel");
-    HookOp op{g->col, parent};
     ktimer.begin();
-    cc_hook_kernel<<<grid_for(n, kBlock, 256 * 16), kBlock, 0, s>>>(g->row, op, static_cast<unsigned>(n), hubq,
-                                                                      &ctr[0], &ctr[1]);
+    for (int r = 0; r < 2; ++r) cc_sample_kernel<<<vgrid, kBlock, 0, s>>>(g->row, g->col, parent, n, r);
+    cc_compress_kernel<<<vgrid, kBlock, 0, s>>>(parent, label, n);
+    cc_largest_kernel<<<1, 256, 0, s>>>(parent, n, &ctr[3]);
+    cc_rest_list_kernel<<<vgrid, kBlock, 0, s>>>(g->row, parent, n, &ctr[3], rest, &ctr[2], cap, &ctr[1]);
+    HookOp op{g->row, g->col, parent};
+    cc_hook_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, rest, &ctr[2], hubq, &ctr[0], cap, &ctr[1]);
     GX_LAUNCHED("cc_hook_kernel");
-    cc_hubs_kernel<<<1024, kBlock, 0, s>>>(g->row, op, hubq, &ctr[0]);
+    hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &ctr[0], cap, hub_off);
+    cc_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, &ctr[0], cap, hub_off);
     ktimer.end();
     GX_LAUNCHED("cc_hubs_kernel");
-    cc_compress_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(parent, label, n);
+    cc_compress_kernel<<<vgrid, kBlock, 0, s>>>(parent, label, n);
     GX_LAUNCHED("cc_compress_kernel");
+    unsigned hc[2];
+    read_bytes(g, hc, ctr, sizeof(hc));
+    if (hc[1]) fail(GX_ERR_CAPACITY, "CC vertex list overflow");
   }
   double ms = region.stop();
   if (stp) {
     std::memset(stp, 0, sizeof(*stp));
     stp->iterations = 1;
-    stp->launches = n > 0 ? 5 : 0;
+    stp->launches = n > 0 ? 10 : 0;
     stp->edges_visited = static_cast<uint64_t>(g->m);
     stp->items = static_cast<uint64_t>(n);
     stp->elapsed_ms = ms;

@@ -12,7 +12,9 @@
 //
 // The per-edge operation is called by every lane of the workgroup the same
 // number of times (with `valid` false for padding lanes), so it may use
-// wavefront-collective operations (ballot, shuffles).
+// wavefront-collective operations (ballot, shuffles).  After every step of
+// BLOCK edges, every lane calls op.after_step() (workgroup-uniform), where an
+// operation may flush its LDS-staged output (LdsQueue below).
 #pragma once
 
 #include "gx_device.hpp"
@@ -65,25 +67,132 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
         edge = s_start[lo] + static_cast<int32_t>(e - s_off[lo]);
       }
       op(valid, u, edge);
+      op.after_step();
     }
     __syncthreads();
   }
 }
 
-// Edges of the hub vertices hubq[0..hubs), spread over every workgroup.
+// Step offsets of the hub list for expand_hubs: off[h] = sum over h' < h of
+// ceil(deg(hubq[h']) / BLOCK), off[hubs] = total steps, with hubs =
+// min(*hub_count, cap) (the count runs past cap when the hub queue
+// overflowed; the caller reports that).  One workgroup; `off` holds cap + 1.
+template <int BLOCK>
+__global__ void __launch_bounds__(1024) hub_offsets_kernel(const int32_t* __restrict__ row,
+                                                           const uint32_t* __restrict__ hubq,
+                                                           const unsigned* __restrict__ hub_count, unsigned cap,
+                                                           uint32_t* __restrict__ off) {
+  __shared__ uint32_t lds[1024 / kWave + 1];
+  const unsigned hubs = min(*hub_count, cap);
+  uint32_t carry = 0;
+  for (unsigned base = 0; base < hubs; base += 1024) {
+    const unsigned i = base + threadIdx.x;
+    uint32_t c = 0;
+    if (i < hubs) {
+      const uint32_t v = hubq[i];
+      c = static_cast<uint32_t>((row[v + 1] - row[v] + BLOCK - 1) / BLOCK);
+    }
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<1024>(c, lds, &total);
+    if (i < hubs) off[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) off[hubs] = carry;
+}
+
+// Edges of the hub vertices hubq[0..hubs), cut into BLOCK-edge steps that
+// are spread over every workgroup (step -> hub by binary search over `off`
+// from hub_offsets_kernel).
 template <int BLOCK, class Op>
 __device__ __forceinline__ void expand_hubs(const int32_t* __restrict__ row, const uint32_t* __restrict__ hubq,
-                                            unsigned hubs, Op& op) {
-  for (unsigned h = 0; h < hubs; ++h) {
-    const uint32_t v = hubq[h];
-    const int32_t start = row[v], end = row[v + 1];
-    for (int64_t e0 = start + static_cast<int64_t>(blockIdx.x) * BLOCK; e0 < end;
-         e0 += static_cast<int64_t>(gridDim.x) * BLOCK) {
-      const int64_t e = e0 + threadIdx.x;
-      const bool valid = e < end;
-      op(valid, v, valid ? static_cast<int32_t>(e) : 0);
+                                            unsigned hub_count, unsigned cap, const uint32_t* __restrict__ off,
+                                            Op& op) {
+  const unsigned hubs = min(hub_count, cap);
+  if (hubs == 0) return;
+  const uint32_t total = off[hubs];
+  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
+    unsigned lo = 0, hi = hubs - 1;  // last h with off[h] <= c
+    while (lo < hi) {
+      const unsigned mid = (lo + hi + 1) >> 1;
+      if (off[mid] <= c) lo = mid; else hi = mid - 1;
     }
+    const uint32_t v = hubq[lo];
+    const int64_t e = row[v] + static_cast<int64_t>(c - off[lo]) * BLOCK + threadIdx.x;
+    const bool valid = e < row[v + 1];
+    op(valid, v, valid ? static_cast<int32_t>(e) : 0);
+    op.after_step();
   }
+}
+
+// Workgroup-private output queue in LDS.  Lanes append with one LDS atomic
+// per wavefront; the workgroup moves the items to the global queue with one
+// global atomic per flush (coalesced stores), instead of one global atomic
+// per wavefront per step on a single hot counter.
+template <int CAP>
+struct LdsQueue {
+  uint32_t items[CAP];
+  unsigned count;
+  unsigned gbase;
+};
+
+template <int CAP>
+__device__ __forceinline__ void lq_init(LdsQueue<CAP>& q) {
+  if (threadIdx.x == 0) q.count = 0;
+  __syncthreads();
+}
+
+// Room is guaranteed by flushing whenever fewer than one step's worth
+// (blockDim.x items) of space is left.
+template <int CAP>
+__device__ __forceinline__ void lq_push(LdsQueue<CAP>& q, bool pred, uint32_t v) {
+  const uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  const unsigned leader = __ffsll(static_cast<unsigned long long>(mask)) - 1;
+  unsigned base = 0;
+  if (lane_id() == leader) base = atomicAdd(&q.count, static_cast<unsigned>(__popcll(mask)));
+  base = __shfl(base, leader);
+  if (pred) q.items[base + lanes_below(mask)] = v;
+}
+
+// Workgroup-uniform: every lane of the workgroup must call it.  Flushes when
+// `force` or when the queue is nearly full.
+template <int CAP>
+__device__ __forceinline__ void lq_flush(LdsQueue<CAP>& q, uint32_t* __restrict__ gq, unsigned* gcount, unsigned cap,
+                                         unsigned* overflow, bool force) {
+  __syncthreads();
+  const unsigned cnt = q.count;
+  __syncthreads();
+  if (!force && cnt + blockDim.x <= CAP) return;
+  if (cnt == 0) return;
+  if (threadIdx.x == 0) q.gbase = atomicAdd(gcount, cnt);
+  __syncthreads();
+  const unsigned base = q.gbase;
+  for (unsigned i = threadIdx.x; i < cnt; i += blockDim.x) {
+    if (base + i < cap)
+      gq[base + i] = q.items[i];
+    else
+      atomicOr(overflow, 1u);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) q.count = 0;
+  __syncthreads();
+}
+
+// Segmented inclusive sum across a wavefront where equal `key`s form
+// contiguous runs of lanes (invalid lanes carry a key no valid lane has).
+// Returns the run total in the last lane of each run (is_tail set).
+template <class T>
+__device__ __forceinline__ T wave_segmented_sum(uint32_t key, T v, bool& is_tail) {
+  const unsigned lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const T ov = __shfl_up(v, off);
+    const uint32_t ok = __shfl_up(key, off);
+    if (lane >= static_cast<unsigned>(off) && ok == key) v += ov;
+  }
+  const uint32_t nk = __shfl_down(key, 1);
+  is_tail = (lane == kWave - 1) || (nk != key);
+  return v;
 }
 
 }  // namespace gx

@@ -95,7 +95,7 @@ struct DevBuf {
 enum Slot : int {
   WS_Q0 = 0, WS_Q1, WS_Q2, WS_Q3, WS_COUNTERS, WS_BITMAP0, WS_BITMAP1,
   WS_V0, WS_V1, WS_V2, WS_V3, WS_V4, WS_V5, WS_REDUCE, WS_BC_LEVEL, WS_BC_SIGMA,
-  WS_BC_DELTA, WS_BC_MASK0, WS_BC_MASK1, WS_BC_MASK2, WS_HOST_PINNED,
+  WS_BC_DELTA, WS_BC_MASK0, WS_BC_MASK1, WS_BC_MASK2, WS_HOST_PINNED, WS_HUBOFF,
   WS_NUM_SLOTS
 };
 
@@ -119,6 +119,28 @@ struct DegreeOrder {
   bool ready = false;
   DevBuf new_of_old, old_of_new, trow, tcol, outdeg;
 };
+
+// A fixed kernel sequence captured once into an executable HIP graph and
+// replayed; re-captured when `key` (identity of the baked kernel arguments)
+// changes.  Used for the device-driven level/bucket loops.
+struct CapturedBatch {
+  hipGraphExec_t exec = nullptr;
+  uint64_t key = 0;
+  CapturedBatch() = default;
+  CapturedBatch(const CapturedBatch&) = delete;
+  CapturedBatch& operator=(const CapturedBatch&) = delete;
+  ~CapturedBatch() { reset(); }
+  void reset() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    exec = nullptr;
+    key = 0;
+  }
+};
+
+inline uint64_t mix_key(uint64_t h, uint64_t v) {
+  h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+  return h;
+}
 
 }  // namespace gx
 
@@ -144,6 +166,7 @@ struct gx_graph {
   std::shared_ptr<void> bc_cache;  // batched-BC tables (bc.hip)
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // non-null stream for HIP graph capture
+  gx::CapturedBatch sssp_batch, bfs_batch, kcore_batch;
   gx::DevBuf ws[gx::WS_NUM_SLOTS];
   void* pinned = nullptr;  // small pinned host block for counters
   size_t pinned_bytes = 0;
@@ -160,6 +183,24 @@ struct gx_graph {
   void* host_pinned(size_t bytes);
   // Default-priority stream used only to capture HIP graphs.
   hipStream_t capture();
+  // Replays `batch` (capturing `body(stream)` first when `key` changed) on
+  // the graph's stream.
+  template <class F>
+  void replay(gx::CapturedBatch& batch, uint64_t key, F body) {
+    if (!batch.exec || batch.key != key) {
+      batch.reset();
+      hipStream_t cs = capture();
+      GX_HIP(hipStreamBeginCapture(cs, hipStreamCaptureModeThreadLocal));
+      body(cs);
+      hipGraph_t graph;
+      GX_HIP(hipStreamEndCapture(cs, &graph));
+      hipError_t e = hipGraphInstantiate(&batch.exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      GX_HIP(e);
+      batch.key = key;
+    }
+    GX_HIP(hipGraphLaunch(batch.exec, stream));
+  }
 };
 
 namespace gx {

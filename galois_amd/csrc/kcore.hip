@@ -27,7 +27,7 @@ using namespace gx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kHub = 4096;
+constexpr uint32_t kHub = 1024;
 
 struct KcoreCounters {
   unsigned count[2];
@@ -65,11 +65,14 @@ __global__ void __launch_bounds__(kBlock) kcore_init_kernel(const int32_t* __res
   }
 }
 
+constexpr int kQCap = 2048;
+
 struct PeelOp {
   const int32_t* col;
   uint32_t* degree;
   uint32_t* qout;
   KcoreCounters* c;
+  LdsQueue<kQCap>* sq;
   uint32_t k;
   int out;
   unsigned cap;
@@ -80,20 +83,34 @@ struct PeelOp {
       v = static_cast<uint32_t>(col[e]);
       dies = atomicSub(&degree[v], 1u) == k;
     }
-    wave_append<uint32_t>(dies, v, qout, &c->count[out], cap, &c->overflow);
+    lq_push(*sq, dies, v);
   }
+  __device__ __forceinline__ void flush(bool force) {
+    lq_flush(*sq, qout, &c->count[out], cap, &c->overflow, force);
+  }
+  __device__ __forceinline__ void after_step() { flush(false); }
 };
 
 __global__ void __launch_bounds__(kBlock) kcore_peel_kernel(const int32_t* __restrict__ row, PeelOp op,
                                                             const uint32_t* __restrict__ qin, int in,
                                                             uint32_t* __restrict__ hubq) {
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  op.sq = &sq;
   const unsigned cnt = op.c->count[in];
   expand_queue<kBlock>(row, qin, cnt, kHub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  op.flush(true);
 }
 
 __global__ void __launch_bounds__(kBlock) kcore_hubs_kernel(const int32_t* __restrict__ row, PeelOp op,
-                                                            const uint32_t* __restrict__ hubq) {
-  expand_hubs<kBlock>(row, hubq, op.c->hub_count, op);
+                                                            const uint32_t* __restrict__ hubq,
+                                                            const uint32_t* __restrict__ hub_off) {
+  if (op.c->hub_count == 0) return;
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  op.sq = &sq;
+  expand_hubs<kBlock>(row, hubq, op.c->hub_count, op.cap, hub_off, op);
+  op.flush(true);
 }
 
 __global__ void kcore_round_begin_kernel(KcoreCounters* c, int out) {
@@ -131,6 +148,7 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   const unsigned cap = static_cast<unsigned>(std::max<int64_t>(n, 1));
   uint32_t* q[2] = {g->scratch<uint32_t>(WS_Q0, cap), g->scratch<uint32_t>(WS_Q1, cap)};
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, cap);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
   KcoreCounters* c = g->scratch<KcoreCounters>(WS_COUNTERS, 1);
   KcoreCounters* host = static_cast<KcoreCounters*>(g->host_pinned(sizeof(KcoreCounters)));
   RegionTimer region(s);
@@ -148,12 +166,13 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
     items += count;
     const int out = in ^ 1;
     kcore_round_begin_kernel<<<1, 64, 0, s>>>(c, out);
-    PeelOp op{g->col, degree, q[out], c, k, out, cap};
+    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap};
     ktimer.begin();
     kcore_peel_kernel<<<std::min<unsigned>(256 * 8, grid_for(count, kBlock)), kBlock, 0, s>>>(g->row, op, q[in], in,
                                                                                               hubq);
     GX_LAUNCHED("kcore_peel_kernel");
-    kcore_hubs_kernel<<<1024, kBlock, 0, s>>>(g->row, op, hubq);
+    hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+    kcore_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
     ktimer.end();
     GX_LAUNCHED("kcore_hubs_kernel");
     launches += 3;

@@ -461,6 +461,8 @@ __global__ void __launch_bounds__(kBlock) push_take_kernel(const uint32_t* __res
   }
 }
 
+constexpr int kQCap = 2048;
+
 struct PushOp {
   const int32_t* col;
   const float* delta;
@@ -468,6 +470,7 @@ struct PushOp {
   uint32_t* stamp;
   uint32_t* next;
   PushCounters* c;
+  LdsQueue<kQCap>* sq;
   float tol;
   uint32_t round;
   int out;
@@ -482,19 +485,33 @@ struct PushOp {
       const float old = atomicAdd(&residual[d], dl);
       if (old <= tol && old + dl >= tol) act = atomicExch(&stamp[d], round) != round;
     }
-    wave_append<uint32_t>(act, d, next, &c->count[out], cap, &c->overflow);
+    lq_push(*sq, act, d);
   }
+  __device__ __forceinline__ void flush(bool force) {
+    lq_flush(*sq, next, &c->count[out], cap, &c->overflow, force);
+  }
+  __device__ __forceinline__ void after_step() { flush(false); }
 };
 
 __global__ void __launch_bounds__(kBlock) push_spread_kernel(const int32_t* __restrict__ row, PushOp op,
                                                              const uint32_t* __restrict__ pushers, int in,
                                                              uint32_t* __restrict__ hubq) {
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  op.sq = &sq;
   expand_queue<kBlock>(row, pushers, op.c->count[in], 4096u, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  op.flush(true);
 }
 
 __global__ void __launch_bounds__(kBlock) push_hubs_kernel(const int32_t* __restrict__ row, PushOp op,
-                                                           const uint32_t* __restrict__ hubq) {
-  expand_hubs<kBlock>(row, hubq, op.c->hub_count, op);
+                                                           const uint32_t* __restrict__ hubq,
+                                                           const uint32_t* __restrict__ hub_off) {
+  if (op.c->hub_count == 0) return;
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  op.sq = &sq;
+  expand_hubs<kBlock>(row, hubq, op.c->hub_count, op.cap, hub_off, op);
+  op.flush(true);
 }
 
 __global__ void iota_kernel(uint32_t* __restrict__ q, int64_t n) {
@@ -534,6 +551,7 @@ extern "C" gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol, uint3
   uint32_t* active = g->scratch<uint32_t>(WS_Q0, n);
   uint32_t* pushers = g->scratch<uint32_t>(WS_Q1, n);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, n);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, n + 1);
   float* residual = g->scratch<float>(WS_V0, n);
   float* delta = g->scratch<float>(WS_V1, n);
   uint32_t* stamp = g->scratch<uint32_t>(WS_V2, n);
@@ -556,9 +574,10 @@ extern "C" gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol, uint3
     push_take_kernel<<<std::min<unsigned>(kSweepGrid, grid_for(count, kBlock)), kBlock, 0, s>>>(
         active, c, 0, pr, residual, delta, g->row, alpha, tol, pushers, 1, cap);
     push_set_kernel<<<1, 64, 0, s>>>(c, 0, 0u);
-    PushOp op{g->col, delta, residual, stamp, active, c, tol, it, 0, cap};
+    PushOp op{g->col, delta, residual, stamp, active, c, nullptr, tol, it, 0, cap};
     push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq);
-    push_hubs_kernel<<<1024, kBlock, 0, s>>>(g->row, op, hubq);
+    hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+    push_hubs_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
     ktimer.end();
     GX_LAUNCHED("push_hubs_kernel");
     launches += 5;

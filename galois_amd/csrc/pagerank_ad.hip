@@ -230,6 +230,7 @@ struct GradOp {
   const Real* inv_s;
   Real* gw;
   Real alpha;
+  __device__ __forceinline__ void after_step() {}
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     if (!valid) return;
     gw[e] = alpha * x[u] * (lam[col[e]] - pl[u]) * inv_s[u];
@@ -246,8 +247,9 @@ __global__ void __launch_bounds__(kBlock) grad_edges_kernel(const int32_t* __res
 template <class Real>
 __global__ void __launch_bounds__(kBlock) grad_hubs_kernel(const int32_t* __restrict__ row, GradOp<Real> op,
                                                            const uint32_t* __restrict__ hubq,
-                                                           const unsigned* hub_count) {
-  expand_hubs<kBlock>(row, hubq, *hub_count, op);
+                                                           const unsigned* hub_count, unsigned cap,
+                                                           const uint32_t* __restrict__ hub_off) {
+  expand_hubs<kBlock>(row, hubq, *hub_count, cap, hub_off, op);
 }
 
 template <class Real>
@@ -361,6 +363,7 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   Real* piece_c = g->scratch<Real>(WS_V3, std::max<int64_t>(P.npieces, 1));
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, n);
   unsigned* hubc = g->scratch<unsigned>(WS_Q3, 4);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, n + 1);
   AdState* state = g->scratch<AdState>(WS_COUNTERS, 1);
   AdState* host = static_cast<AdState*>(g->host_pinned(sizeof(AdState)));
   RegionTimer region(s);
@@ -393,7 +396,8 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   GradOp<Real> op{g->col, x, lam_f, pl, inv_s, grad_w, static_cast<Real>(alpha)};
   grad_edges_kernel<Real><<<grid_for(n, kBlock, 256 * 16), kBlock, 0, s>>>(g->row, op, static_cast<unsigned>(n), hubq,
                                                                           hubc, hubc + 1);
-  grad_hubs_kernel<Real><<<1024, kBlock, 0, s>>>(g->row, op, hubq, hubc);
+  hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, hubc, static_cast<unsigned>(n), hub_off);
+  grad_hubs_kernel<Real><<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hubc, static_cast<unsigned>(n), hub_off);
   grad_t_kernel<Real><<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(lam_f, n, static_cast<Real>(1.0 - alpha), grad_t);
   GX_LAUNCHED("adjoint gradients");
   launches += 4;

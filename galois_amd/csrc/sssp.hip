@@ -10,7 +10,7 @@
 //
 // Design (gfx950)
 //  - near frontier: vertices whose new distance falls in the current bucket;
-//    each inner round relaxes all of them with the load-balanced expansion of
+//    each round relaxes all of them with the load-balanced expansion of
 //    frontier.hpp and atomicMin on the distance word; a vertex enters a
 //    round's queue once (stamp word per vertex).  A vertex is expanded only
 //    if its distance changed since it was last expanded (`done` word), the
@@ -18,8 +18,13 @@
 //  - far pile: vertices improved to a later bucket; each vertex is in the far
 //    pile at most once (farmark word), so the pile is bounded by n.  When the
 //    near frontier drains, one pass finds the smallest live far bucket and a
-//    second pass moves that bucket into the near frontier and compacts the
-//    rest.
+//    second pass moves that bucket into the near frontier and keeps the rest.
+//  - The loop is driven on the device: every step is the same fixed kernel
+//    sequence (begin, filter, relax, hubs, far-min, far-split, end) whose
+//    kernels read the step's mode from a device state word and return at
+//    once when it is not theirs.  Steps are captured once into a HIP graph
+//    (kStepsPerGraph steps) and replayed until the state says done, so the
+//    host synchronises once per graph replay instead of once per round.
 #include <algorithm>
 
 #include "frontier.hpp"
@@ -32,17 +37,47 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kInf = GX_DIST_INFINITY;
-constexpr uint32_t kHub = 4096;
+constexpr uint32_t kHub = 1024;
+constexpr int kGrid = 256 * 8;
+constexpr int kStepsPerGraph = 16;
 
-struct SsspCounters {
-  unsigned near_count[2];
-  unsigned far_count[2];
+enum Mode : unsigned { kIdle = 0, kRelax = 1, kAdvance = 2 };
+
+struct SsspState {
+  unsigned near_count[2];  // near queues (double buffered by q)
+  unsigned far_count[2];   // far piles (double buffered by far_sel)
+  unsigned filt_count;     // filtered near items of the current round
   unsigned hub_count;
   unsigned overflow;
   unsigned min_bucket;
-  unsigned pad;
+  unsigned mode;
+  unsigned done;
+  unsigned q;              // near queue read this step
+  unsigned far_sel;        // far pile appended to
+  unsigned bucket;
+  unsigned round;
+  unsigned phase;
+  unsigned steps;
   unsigned long long edges;
   unsigned long long items;
+};
+
+struct Bufs {
+  const int32_t* row;
+  const int32_t* col;
+  const uint32_t* w;
+  uint32_t* dist;
+  uint32_t* stamp;
+  uint32_t* farmark;
+  uint32_t* done;
+  uint32_t* nearq[2];
+  uint32_t* farq[2];
+  uint32_t* filt;
+  uint32_t* hubq;
+  SsspState* S;
+  uint32_t shift;
+  unsigned cap;
+  uint32_t* hub_off;
 };
 
 __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
@@ -58,160 +93,239 @@ __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restri
   }
 }
 
-__global__ void sssp_reset_kernel(SsspCounters* c, uint32_t* nearq, uint32_t src) {
+__global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src) {
   if (threadIdx.x == 0) {
-    c->near_count[0] = 1;
-    c->near_count[1] = 0;
-    c->far_count[0] = 0;
-    c->far_count[1] = 0;
-    c->hub_count = 0;
-    c->overflow = 0;
-    c->min_bucket = 0xffffffffu;
-    c->edges = 0;
-    c->items = 0;
-    nearq[0] = src;
+    S->near_count[0] = 1;
+    S->near_count[1] = 0;
+    S->far_count[0] = S->far_count[1] = 0;
+    S->filt_count = 0;
+    S->hub_count = 0;
+    S->overflow = 0;
+    S->min_bucket = 0xffffffffu;
+    S->mode = kIdle;
+    S->done = 0;
+    S->q = 0;
+    S->far_sel = 0;
+    S->bucket = 0;
+    S->round = 0;
+    S->phase = 0;
+    S->steps = 0;
+    S->edges = 0;
+    S->items = 0;
+    nearq0[0] = src;
   }
 }
 
-__global__ void sssp_round_begin_kernel(SsspCounters* c, int nxt) {
-  if (threadIdx.x == 0) {
-    c->near_count[nxt] = 0;
-    c->hub_count = 0;
+// Chooses the step's mode and clears the counters the step appends to.
+__global__ void step_begin_kernel(SsspState* S) {
+  if (threadIdx.x != 0) return;
+  if (S->done || S->overflow) {
+    S->mode = kIdle;
+    return;
   }
+  S->round += 1;
+  S->filt_count = 0;
+  S->hub_count = 0;
+  S->near_count[S->q ^ 1] = 0;
+  if (S->near_count[S->q] > 0) {
+    S->mode = kRelax;
+  } else if (S->far_count[S->far_sel] > 0) {
+    S->mode = kAdvance;
+    S->phase += 1;
+    S->min_bucket = 0xffffffffu;
+    S->far_count[S->far_sel ^ 1] = 0;
+  } else {
+    S->mode = kIdle;
+    S->done = 1;
+  }
+  S->steps += 1;
 }
+
+// Relax step, part 1: drop near items whose distance has not changed since
+// their last expansion; mark the rest expanded at their current distance.
+__global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B);
+
+constexpr int kQCap = 2048;
 
 struct RelaxOp {
-  const int32_t* col;
-  const uint32_t* w;
-  uint32_t* dist;
-  uint32_t* stamp;
-  uint32_t* farmark;
-  uint32_t* nearq;
+  Bufs B;
+  LdsQueue<kQCap>* qn;  // staged near pushes
+  LdsQueue<kQCap>* qf;  // staged far pushes
+  uint32_t* nearq_out;
+  unsigned* near_out_count;
   uint32_t* farq;
-  SsspCounters* c;
-  int nxt, far_sel;
-  uint32_t shift, bucket, round, phase;
-  unsigned cap;
-
+  unsigned* far_count;
+  uint32_t bucket, round, phase;
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     bool to_near = false, to_far = false;
     uint32_t v = 0;
     if (valid) {
-      v = static_cast<uint32_t>(col[e]);
-      const uint32_t nd = dist[u] + w[e];
-      if (nd < dist[v]) {
-        const uint32_t old = atomicMin(&dist[v], nd);
+      v = static_cast<uint32_t>(B.col[e]);
+      const uint32_t nd = B.dist[u] + B.w[e];
+      if (nd < B.dist[v]) {
+        const uint32_t old = atomicMin(&B.dist[v], nd);
         if (nd < old) {
-          if ((nd >> shift) <= bucket)
-            to_near = atomicExch(&stamp[v], round) != round;
+          if ((nd >> B.shift) <= bucket)
+            to_near = atomicExch(&B.stamp[v], round) != round;
           else
-            to_far = atomicExch(&farmark[v], phase) != phase;
+            to_far = atomicExch(&B.farmark[v], phase) != phase;
         }
       }
     }
-    wave_append<uint32_t>(to_near, v, nearq, &c->near_count[nxt], cap, &c->overflow);
-    wave_append<uint32_t>(to_far, v, farq, &c->far_count[far_sel], cap, &c->overflow);
+    lq_push(*qn, to_near, v);
+    lq_push(*qf, to_far, v);
   }
+  __device__ __forceinline__ void flush(bool force) {
+    lq_flush(*qn, nearq_out, near_out_count, B.cap, &B.S->overflow, force);
+    lq_flush(*qf, farq, far_count, B.cap, &B.S->overflow, force);
+  }
+  __device__ __forceinline__ void after_step() { flush(false); }
 };
 
-// Drop near items whose distance has not changed since their last expansion;
-// mark the rest as expanded at their current distance.
-__global__ void __launch_bounds__(kBlock) sssp_filter_kernel(const int32_t* __restrict__ row,
-                                                             const uint32_t* __restrict__ qin,
-                                                             SsspCounters* c, int cur,
-                                                             const uint32_t* __restrict__ dist,
-                                                             uint32_t* __restrict__ done, uint32_t* __restrict__ qout,
-                                                             int out_sel, unsigned cap) {
-  const unsigned cnt = c->near_count[cur];
+__device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn, LdsQueue<kQCap>* qf) {
+  SsspState* S = B.S;
+  const unsigned q = S->q, fs = S->far_sel;
+  return RelaxOp{B, qn, qf, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.farq[fs], &S->far_count[fs], S->bucket,
+                 S->round, S->phase};
+}
+
+// Relax step, part 2: expand the filtered items.
+__global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
+  SsspState* S = B.S;
+  if (S->mode != kRelax) return;
+  __shared__ LdsQueue<kQCap> qn, qf;
+  lq_init(qn);
+  lq_init(qf);
+  RelaxOp op = make_relax(B, &qn, &qf);
+  expand_queue<kBlock>(B.row, B.filt, S->filt_count, kHub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
+  op.flush(true);
+}
+
+__global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
+  SsspState* S = B.S;
+  if (S->mode != kRelax || S->hub_count == 0) return;
+  __shared__ LdsQueue<kQCap> qn, qf;
+  lq_init(qn);
+  lq_init(qf);
+  RelaxOp op = make_relax(B, &qn, &qf);
+  expand_hubs<kBlock>(B.row, B.hubq, S->hub_count, B.cap, B.hub_off, op);
+  op.flush(true);
+}
+
+__global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
+  SsspState* S = B.S;
+  if (S->mode != kRelax) return;
+  __shared__ LdsQueue<kQCap> qo;
+  lq_init(qo);
+  const unsigned q = S->q;
+  const unsigned cnt = S->near_count[q];
+  const uint32_t* in = B.nearq[q];
   unsigned long long edges = 0, items = 0;
   for (unsigned b = blockIdx.x * kBlock; b < cnt; b += gridDim.x * kBlock) {
     const unsigned i = b + threadIdx.x;
     bool keep = false;
     uint32_t u = 0;
     if (i < cnt) {
-      u = qin[i];
-      const uint32_t d = dist[u];
-      keep = done[u] != d;
+      u = in[i];
+      const uint32_t d = B.dist[u];
+      keep = B.done[u] != d;
       if (keep) {
-        done[u] = d;
-        edges += static_cast<unsigned long long>(row[u + 1] - row[u]);
+        B.done[u] = d;
+        edges += static_cast<unsigned long long>(B.row[u + 1] - B.row[u]);
         items += 1;
       }
     }
-    wave_append<uint32_t>(keep, u, qout, &c->near_count[out_sel], cap, &c->overflow);
+    lq_push(qo, keep, u);
+    lq_flush(qo, B.filt, &S->filt_count, B.cap, &S->overflow, false);
   }
+  lq_flush(qo, B.filt, &S->filt_count, B.cap, &S->overflow, true);
   edges = wave_sum(edges);
   items = wave_sum(items);
   if (lane_id() == 0 && items) {
-    atomicAdd(&c->edges, edges);
-    atomicAdd(&c->items, items);
+    atomicAdd(&S->edges, edges);
+    atomicAdd(&S->items, items);
   }
 }
 
-__global__ void __launch_bounds__(kBlock) sssp_relax_kernel(const int32_t* __restrict__ row, RelaxOp op,
-                                                            const uint32_t* __restrict__ qin, int cur,
-                                                            uint32_t* __restrict__ hubq) {
-  const unsigned qcount = op.c->near_count[cur];
-  expand_queue<kBlock>(row, qin, qcount, kHub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
-}
-
-__global__ void __launch_bounds__(kBlock) sssp_hubs_kernel(const int32_t* __restrict__ row, RelaxOp op,
-                                                           const uint32_t* __restrict__ hubq) {
-  expand_hubs<kBlock>(row, hubq, op.c->hub_count, op);
-}
-
-// Smallest bucket among far items still above the current bucket.
-__global__ void __launch_bounds__(kBlock) far_min_kernel(const uint32_t* __restrict__ farq, SsspCounters* c,
-                                                         int sel, const uint32_t* __restrict__ dist, uint32_t shift,
-                                                         uint32_t bucket) {
-  const unsigned cnt = c->far_count[sel];
+// Advance step, part 1: smallest bucket among far items above the current.
+__global__ void __launch_bounds__(kBlock) far_min_kernel(Bufs B) {
+  SsspState* S = B.S;
+  if (S->mode != kAdvance) return;
+  const unsigned fs = S->far_sel;
+  const unsigned cnt = S->far_count[fs];
+  const uint32_t* farq = B.farq[fs];
+  const uint32_t bucket = S->bucket;
   uint32_t best = 0xffffffffu;
   for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock) {
-    const uint32_t b = dist[farq[i]] >> shift;
+    const uint32_t b = B.dist[farq[i]] >> B.shift;
     if (b > bucket && b < best) best = b;
   }
   best = wave_min(best);
-  if (lane_id() == 0 && best != 0xffffffffu) atomicMin(&c->min_bucket, best);
+  if (lane_id() == 0 && best != 0xffffffffu) atomicMin(&S->min_bucket, best);
 }
 
-// Move far items of bucket `next` into the near queue, keep later ones
-// (one copy per vertex), drop settled ones.
-__global__ void __launch_bounds__(kBlock) far_split_kernel(const uint32_t* __restrict__ farq, int sel,
-                                                           uint32_t* __restrict__ farq_out,
-                                                           uint32_t* __restrict__ nearq, int near_sel,
-                                                           SsspCounters* c, const uint32_t* __restrict__ dist,
-                                                           uint32_t* __restrict__ stamp,
-                                                           uint32_t* __restrict__ farmark, uint32_t shift,
-                                                           uint32_t next, uint32_t round, uint32_t phase,
-                                                           unsigned cap) {
-  const unsigned cnt = c->far_count[sel];
+// Advance step, part 2: far items of the new bucket go to the near queue,
+// later ones are kept (one copy per vertex), settled ones are dropped.
+__global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
+  SsspState* S = B.S;
+  if (S->mode != kAdvance) return;
+  const uint32_t next = S->min_bucket;
+  if (next == 0xffffffffu) return;
+  __shared__ LdsQueue<kQCap> qn, qk;
+  lq_init(qn);
+  lq_init(qk);
+  const unsigned fs = S->far_sel, q = S->q;
+  const unsigned cnt = S->far_count[fs];
+  const uint32_t* farq = B.farq[fs];
+  const uint32_t round = S->round, phase = S->phase;
   for (unsigned b0 = blockIdx.x * kBlock; b0 < cnt; b0 += gridDim.x * kBlock) {
     const unsigned i = b0 + threadIdx.x;
     bool to_near = false, keep = false;
     uint32_t v = 0;
     if (i < cnt) {
       v = farq[i];
-      const uint32_t b = dist[v] >> shift;
+      const uint32_t b = B.dist[v] >> B.shift;
       if (b == next)
-        to_near = atomicExch(&stamp[v], round) != round;
+        to_near = atomicExch(&B.stamp[v], round) != round;
       else if (b > next)
-        keep = atomicExch(&farmark[v], phase) != phase;
+        keep = atomicExch(&B.farmark[v], phase) != phase;
     }
-    wave_append<uint32_t>(to_near, v, nearq, &c->near_count[near_sel], cap, &c->overflow);
-    wave_append<uint32_t>(keep, v, farq_out, &c->far_count[sel ^ 1], cap, &c->overflow);
+    lq_push(qn, to_near, v);
+    lq_push(qk, keep, v);
+    lq_flush(qn, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.cap, &S->overflow, false);
+    lq_flush(qk, B.farq[fs ^ 1], &S->far_count[fs ^ 1], B.cap, &S->overflow, false);
   }
+  lq_flush(qn, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.cap, &S->overflow, true);
+  lq_flush(qk, B.farq[fs ^ 1], &S->far_count[fs ^ 1], B.cap, &S->overflow, true);
 }
 
-__global__ void far_begin_kernel(SsspCounters* c, int sel, int near_sel) {
-  if (threadIdx.x == 0) {
-    c->far_count[sel ^ 1] = 0;
-    c->near_count[near_sel] = 0;
-    c->min_bucket = 0xffffffffu;
+__global__ void step_end_kernel(SsspState* S) {
+  if (threadIdx.x != 0) return;
+  if (S->mode == kRelax) {
+    S->q ^= 1;
+  } else if (S->mode == kAdvance) {
+    if (S->min_bucket == 0xffffffffu) {
+      S->done = 1;  // every far item is settled
+    } else {
+      S->bucket = S->min_bucket;
+      S->far_count[S->far_sel] = 0;
+      S->far_sel ^= 1;
+      S->q ^= 1;
+    }
   }
+  S->mode = kIdle;
 }
 
-__global__ void far_clear_kernel(SsspCounters* c, int sel) {
-  if (threadIdx.x == 0) c->far_count[sel] = 0;
+void enqueue_step(const Bufs& B, hipStream_t s) {
+  step_begin_kernel<<<1, 64, 0, s>>>(B.S);
+  filter_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  relax_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off);
+  hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  far_min_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  far_split_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  step_end_kernel<<<1, 64, 0, s>>>(B.S);
+  GX_LAUNCHED("sssp step");
 }
 
 }  // namespace
@@ -226,91 +340,57 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   const int64_t n = g->n;
   hipStream_t s = g->stream;
   const unsigned cap = static_cast<unsigned>(std::max<int64_t>(n, 1));
-  uint32_t* nearq[2] = {g->scratch<uint32_t>(WS_Q0, cap), g->scratch<uint32_t>(WS_Q1, cap)};
-  uint32_t* farq[2] = {g->scratch<uint32_t>(WS_Q2, cap), g->scratch<uint32_t>(WS_Q3, cap)};
-  uint32_t* hubq = g->scratch<uint32_t>(WS_V0, cap);
-  uint32_t* stamp = g->scratch<uint32_t>(WS_V1, cap);
-  uint32_t* farmark = g->scratch<uint32_t>(WS_V2, cap);
-  uint32_t* done = g->scratch<uint32_t>(WS_V3, cap);
-  uint32_t* filtered = g->scratch<uint32_t>(WS_V4, cap);
-  SsspCounters* c = g->scratch<SsspCounters>(WS_COUNTERS, 1);
-  SsspCounters* host = static_cast<SsspCounters*>(g->host_pinned(sizeof(SsspCounters)));
+  Bufs B;
+  B.row = g->row;
+  B.col = g->col;
+  B.w = static_cast<const uint32_t*>(g->w);
+  B.dist = dist;
+  B.nearq[0] = g->scratch<uint32_t>(WS_Q0, cap);
+  B.nearq[1] = g->scratch<uint32_t>(WS_Q1, cap);
+  B.farq[0] = g->scratch<uint32_t>(WS_Q2, cap);
+  B.farq[1] = g->scratch<uint32_t>(WS_Q3, cap);
+  B.hubq = g->scratch<uint32_t>(WS_V0, cap);
+  B.stamp = g->scratch<uint32_t>(WS_V1, cap);
+  B.farmark = g->scratch<uint32_t>(WS_V2, cap);
+  B.done = g->scratch<uint32_t>(WS_V3, cap);
+  B.filt = g->scratch<uint32_t>(WS_V4, cap);
+  B.S = g->scratch<SsspState>(WS_COUNTERS, 1);
+  B.shift = shift;
+  B.cap = cap;
+  B.hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
+  SsspState* host = static_cast<SsspState*>(g->host_pinned(sizeof(SsspState)));
 
   RegionTimer region(s);
   KernelTimer ktimer(s);
-  uint32_t launches = 2;
-  sssp_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(dist, stamp, farmark, done, n, src);
+  sssp_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(dist, B.stamp, B.farmark, B.done, n, src);
   GX_LAUNCHED("sssp_init_kernel");
-  sssp_reset_kernel<<<1, 64, 0, s>>>(c, nearq[0], src);
+  sssp_reset_kernel<<<1, 64, 0, s>>>(B.S, B.nearq[0], src);
   GX_LAUNCHED("sssp_reset_kernel");
-
-  const unsigned relax_grid = 256 * 8;
-  uint32_t bucket = 0, round = 0, phase = 0, rounds_total = 0;
-  int cur = 0, far_sel = 0;
-  unsigned near_count = 1, far_count = 0;
-  // Each inner round: filter nearq[cur] into `filtered` (counter
-  // near_count[cur^1]), then relax `filtered`, appending the next round's
-  // items to nearq[cur] (counter near_count[cur]).
+  uint64_t key = 0;
+  for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w, (const void*)B.dist,
+                        (const void*)B.nearq[0], (const void*)B.nearq[1], (const void*)B.farq[0],
+                        (const void*)B.farq[1], (const void*)B.hubq, (const void*)B.stamp, (const void*)B.farmark,
+                        (const void*)B.done, (const void*)B.filt, (const void*)B.S, (const void*)B.hub_off})
+    key = mix_key(key, reinterpret_cast<uintptr_t>(p));
+  key = mix_key(mix_key(key, shift), cap);
+  uint32_t replays = 0;
   for (;;) {
-    while (near_count > 0) {
-      // Filter stale items into `filtered` (counter near_count[cur^1]).
-      const int nxt = cur ^ 1;
-      sssp_round_begin_kernel<<<1, 64, 0, s>>>(c, nxt);
-      sssp_filter_kernel<<<std::min<unsigned>(relax_grid, grid_for(near_count, kBlock)), kBlock, 0, s>>>(
-          g->row, nearq[cur], c, cur, dist, done, filtered, nxt, cap);
-      GX_LAUNCHED("sssp_filter_kernel");
-      // Relax the filtered items; their count is now near_count[nxt]; the
-      // relax writes the next round's items into nearq[cur] / near_count[cur].
-      ++round;
-      RelaxOp op{g->col, static_cast<const uint32_t*>(g->w), dist, stamp, farmark, nearq[cur], farq[far_sel], c,
-                 cur, far_sel, shift, bucket, round, phase, cap};
-      sssp_round_begin_kernel<<<1, 64, 0, s>>>(c, cur);
-      ktimer.begin();
-      sssp_relax_kernel<<<std::min<unsigned>(relax_grid, grid_for(near_count, kBlock)), kBlock, 0, s>>>(
-          g->row, op, filtered, nxt, hubq);
-      GX_LAUNCHED("sssp_relax_kernel");
-      sssp_hubs_kernel<<<1024, kBlock, 0, s>>>(g->row, op, hubq);
-      ktimer.end();
-      GX_LAUNCHED("sssp_hubs_kernel");
-      launches += 5;
-      ++rounds_total;
-      GX_HIP(hipMemcpyAsync(host, c, sizeof(SsspCounters), hipMemcpyDeviceToHost, s));
-      GX_HIP(hipStreamSynchronize(s));
-      if (host->overflow) fail(GX_ERR_CAPACITY, "SSSP queue overflow");
-      near_count = host->near_count[cur];
-      far_count = host->far_count[far_sel];
-    }
-    if (far_count == 0) break;
-    // Next bucket = smallest live far bucket.
-    far_begin_kernel<<<1, 64, 0, s>>>(c, far_sel, cur);
-    far_min_kernel<<<std::min<unsigned>(relax_grid, grid_for(far_count, kBlock)), kBlock, 0, s>>>(
-        farq[far_sel], c, far_sel, dist, shift, bucket);
-    GX_LAUNCHED("far_min_kernel");
-    GX_HIP(hipMemcpyAsync(host, c, sizeof(SsspCounters), hipMemcpyDeviceToHost, s));
-    GX_HIP(hipStreamSynchronize(s));
-    launches += 2;
-    if (host->min_bucket == 0xffffffffu) break;  // every far item is settled
-    bucket = host->min_bucket;
-    ++round;
-    ++phase;
-    far_split_kernel<<<std::min<unsigned>(relax_grid, grid_for(far_count, kBlock)), kBlock, 0, s>>>(
-        farq[far_sel], far_sel, farq[far_sel ^ 1], nearq[cur], cur, c, dist, stamp, farmark, shift, bucket, round,
-        phase, cap);
-    GX_LAUNCHED("far_split_kernel");
-    far_clear_kernel<<<1, 64, 0, s>>>(c, far_sel);
-    launches += 2;
-    far_sel ^= 1;
-    GX_HIP(hipMemcpyAsync(host, c, sizeof(SsspCounters), hipMemcpyDeviceToHost, s));
+    ktimer.begin();
+    g->replay(g->sssp_batch, key, [&](hipStream_t cs) {
+      for (int i = 0; i < kStepsPerGraph; ++i) enqueue_step(B, cs);
+    });
+    ktimer.end();
+    ++replays;
+    GX_HIP(hipMemcpyAsync(host, B.S, sizeof(SsspState), hipMemcpyDeviceToHost, s));
     GX_HIP(hipStreamSynchronize(s));
     if (host->overflow) fail(GX_ERR_CAPACITY, "SSSP queue overflow");
-    near_count = host->near_count[cur];
-    far_count = host->far_count[far_sel];
+    if (host->done) break;
   }
   double ms = region.stop();
   if (st) {
     std::memset(st, 0, sizeof(*st));
-    st->iterations = rounds_total;
-    st->launches = launches;
+    st->iterations = host->steps;
+    st->launches = 2 + replays * kStepsPerGraph * 8;
     st->edges_visited = host->edges;
     st->items = host->items;
     st->elapsed_ms = ms;
