@@ -10,8 +10,8 @@ full PageRank solve; value = GTEPS = edges x iterations / time.
 
 Also reported, one line of JSON in total:
   roofline      the dominant kernel (the pull sweep) against the 8 TB/s HBM
-                peak, algorithmic bytes 4m + 16n per sweep, timed with HIP
-                events on the library's stream;
+                peak, algorithmic bytes 4m + 24n + 4 per sweep, timed with HIP
+                events on the library's stream during the timed steps;
   cpu_baseline  the CPU restatement (oracle, FP64 pull) timed on this host's
                 cores on a bounded sample (a few PageRank iterations on the
                 same graph);
@@ -48,6 +48,19 @@ def log(msg: str) -> None:
 def sync():
     import torch
     torch.cuda.synchronize()
+
+
+def measured_traffic(scale: int):
+    """HBM bytes per pull sweep from the committed rocprofv3 PMC summary
+    (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy kernel), or
+    None when no summary for this graph is present."""
+    f = ROOT / "profiles" / "pagerank_pull_rmat26_traffic.json"
+    if scale != 26 or not f.exists():
+        return None
+    try:
+        return json.loads(f.read_text())["traffic_bytes_per_sweep"]
+    except (ValueError, KeyError):
+        return None
 
 
 # ------------------------------------------------------------ CPU baseline
@@ -188,28 +201,37 @@ def main() -> int:
     for _ in range(args.warmup):
         step()
     sync()
+    # The timed steps run with the library's HIP-event timers on (one event
+    # pair per pull sweep, on the stream the sweeps are launched on), so the
+    # roofline below comes from the same region as `value`.
+    gx.set_profiling(True)
     t0 = time.perf_counter()
-    iters = 0
+    iters = sweeps = 0
+    sweep_ms_total = 0.0
     for _ in range(args.steps):
         x32, st = step()
         iters += st.iterations
+        sweeps += st.main_kernel_launches
+        sweep_ms_total += st.main_kernel_ms
     sync()
     dt = time.perf_counter() - t0
+    gx.set_profiling(False)
     value = m * iters / dt / 1e9
     ms_per_step = dt * 1e3 / args.steps
     log(f"pagerank fp32: {ms_per_step:.1f} ms/step, {iters / args.steps:.0f} iterations, {value:.2f} GTEPS")
 
-    # Roofline of the dominant kernel: HIP-event timed pull sweeps.
-    gx.set_profiling(True)
-    x32p, stp = step()
-    gx.set_profiling(False)
-    sweep_ms = stp.main_kernel_ms / max(stp.main_kernel_launches, 1)
-    algo_bytes = 4 * m + 16 * n
+    # Roofline of the dominant kernel, the pull sweep (hot-source pass + cold
+    # pass + spanning-row completion, bracketed together per iteration).
+    # Algorithmic bytes of one sweep: 4m column ids + 4(n+1) row offsets +
+    # 4n y (read once) + 4n x_old + 4n out-degrees + 8n (x', y' written).
+    sweep_ms = sweep_ms_total / max(sweeps, 1)
+    algo_bytes = 4 * m + 4 * (n + 1) + 20 * n
     achieved = algo_bytes / (sweep_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": None,
-                "kernel": "pr_sweep_kernel<float>", "bytes_per_launch": algo_bytes,
-                "avg_launch_ms": round(sweep_ms, 4), "launches": stp.main_kernel_launches}
+                "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": measured_traffic(args.scale),
+                "kernel": "pull sweep: rs::sweep_kernel<float,PrSrc,PartFin> + rs::sweep_kernel<float,PrSrc,PrFin>"
+                          " + rs::span_finalize_kernel (x2)",
+                "bytes_per_launch": algo_bytes, "avg_launch_ms": round(sweep_ms, 4), "launches": sweeps}
 
     # FP64 run and the FP32-vs-FP64 error.
     x64, st64 = gx.pagerank(G, alpha=0.85, tol=1e-9, max_iter=100, precision="fp64")
@@ -224,7 +246,7 @@ def main() -> int:
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline_pagerank(G, iters=3)
-    del G, x32, x64, x32p
+    del G, x32, x64
     torch.cuda.empty_cache()
     kernels = {} if args.no_kernels else kernel_table(gx, args.quick)
     kernels["pagerank_rmat26"] = pr_err

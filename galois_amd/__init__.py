@@ -62,6 +62,18 @@ def set_profiling(on: bool) -> None:
     lib().gx_set_profiling(1 if on else 0)
 
 
+def tune_set(key: str, value: int) -> None:
+    """Set a tuning knob (gx_tune_set); -1 restores the built-in default."""
+    check(lib().gx_tune_set(key.encode(), int(value)))
+
+
+def tune_get(key: str) -> int:
+    """Current value of a tuning knob; -1 means the built-in default."""
+    v = C.c_int64()
+    check(lib().gx_tune_get(key.encode(), C.byref(v)))
+    return v.value
+
+
 class Graph:
     """A CSR graph resident in device memory (owns a gx_graph handle)."""
 

@@ -59,7 +59,33 @@ def build_library(jobs: int | None = None, verbose: bool = False) -> Path:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}", file=sys.stderr)
+    build_apps(jobs)
     return LIB
+
+
+APPS = PKG / "apps"
+BIN = PKG / "bin"
+# Lonestar app entry points: source stem -> executable name of the reference.
+APP_NAMES = {"bfs": "bfs", "sssp": "sssp", "connected_components": "connected-components", "kcore": "k-core",
+             "pagerank_pull": "pagerank-pull", "betweennesscentrality": "betweennesscentrality"}
+
+
+def _build_app(src: Path) -> Path:
+    exe = BIN / APP_NAMES[src.stem]
+    cmd = [HIPCC, *COMMON, "-I", str(PKG.parent / "include"), str(src), "-o", str(exe), f"-L{PKG}", "-lgalois_amd",
+           "-Wl,-rpath,$ORIGIN/.."]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        raise RuntimeError(f"app build failed: {' '.join(cmd)}\n{res.stdout}\n{res.stderr}")
+    return exe
+
+
+def build_apps(jobs: int | None = None) -> list[Path]:
+    """Build the lonestar-compatible app executables into galois_amd/bin/."""
+    BIN.mkdir(exist_ok=True)
+    srcs = sorted(APPS.glob("*.cpp"))
+    with cf.ThreadPoolExecutor(max_workers=jobs or len(srcs) or 1) as ex:
+        return list(ex.map(_build_app, srcs))
 
 
 if __name__ == "__main__":

@@ -79,6 +79,8 @@ SIGNATURES: dict[str, tuple] = {
     "gx_last_error": (C.c_size_t, [C.c_char_p, C.c_size_t]),
     "gx_init": (INT, [INT]),
     "gx_set_profiling": (None, [INT]),
+    "gx_tune_set": (INT, [C.c_char_p, C.c_int64]),
+    "gx_tune_get": (INT, [C.c_char_p, C.POINTER(C.c_int64)]),
     "gx_graph_create": (INT, [I64, I64, P, P, P, U32, PG]),
     "gx_graph_create_device": (INT, [I64, I64, P, P, P, U32, PG]),
     "gx_graph_free": (INT, [P]),
@@ -120,6 +122,11 @@ def lib() -> C.CDLL:
             raise ImportError(
                 f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
                 "(there is no fallback implementation)")
+        # torch ships its own HIP runtime under the same soname as ROCm's.
+        # Loading torch first makes this library bind to that one copy; if
+        # ROCm's copy were loaded first, torch would later bind to it and
+        # fail to see the GPU.
+        import torch  # noqa: F401
         handle = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(handle, name)

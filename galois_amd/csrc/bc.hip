@@ -42,7 +42,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kInf = GX_BC_INFINITY;
-constexpr uint32_t kHub = 1024;
 
 // ================================================================== plain
 struct PlainCounters {
@@ -152,8 +151,8 @@ template <class Op>
 __global__ void __launch_bounds__(kBlock) bc_expand_kernel(const int32_t* __restrict__ row, Op op,
                                                            const uint32_t* __restrict__ q, unsigned qcount,
                                                            uint32_t* __restrict__ hubq, PlainCounters* c,
-                                                           unsigned cap) {
-  expand_queue<kBlock>(row, q, qcount, kHub, hubq, &c->hub_count, cap, &c->overflow, op);
+                                                           unsigned cap, uint32_t hub) {
+  expand_queue<kBlock>(row, q, qcount, hub, hubq, &c->hub_count, cap, &c->overflow, op);
 }
 
 template <class Op>
@@ -167,11 +166,11 @@ __global__ void __launch_bounds__(kBlock) bc_hubs_kernel(const int32_t* __restri
 
 __global__ void __launch_bounds__(kBlock) bc_fwd_expand_kernel(const int32_t* __restrict__ row, FwdOp op,
                                                                const uint32_t* __restrict__ q, unsigned qcount,
-                                                               uint32_t* __restrict__ hubq) {
+                                                               uint32_t* __restrict__ hubq, uint32_t hub) {
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
   op.sq = &sq;
-  expand_queue<kBlock>(row, q, qcount, kHub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  expand_queue<kBlock>(row, q, qcount, hub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
   op.flush(true);
 }
 
@@ -548,6 +547,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
   GX_LAUNCHED("bc_plain_init_kernel");
   launches += 2;
   const unsigned grid = 256 * 8;
+  const uint32_t hub = hub_degree();
   // In-edges for the path-count pull (the CSR itself for symmetric graphs).
   const int32_t* irow = g->row;
   const int32_t* icol = g->col;
@@ -573,7 +573,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
       bc_plain_clear_kernel<<<1, 64, 0, s>>>(c);
       FwdOp op{g->col, level, order + e, c, nullptr, L + 1, cap - e};
       bc_fwd_expand_kernel<<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(g->row, op,
-                                                                                               order + b, cnt, hubq);
+                                                                                               order + b, cnt, hubq, hub);
       hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
       bc_fwd_hubs_kernel<<<grid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
       GX_LAUNCHED("bc forward");
@@ -588,7 +588,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
         bc_plain_clear_kernel<<<1, 64, 0, s>>>(c);
         SigmaOp sop{icol, level, sigma, L - 1};
         bc_expand_kernel<SigmaOp><<<std::min<unsigned>(grid, grid_for(ncnt, kBlock)), kBlock, 0, s>>>(
-            irow, sop, order + e, ncnt, hubq, c, cap);
+            irow, sop, order + e, ncnt, hubq, c, cap, hub);
         hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(irow, hubq, &c->hub_count, cap, hub_off);
         bc_hubs_kernel<SigmaOp><<<grid, kBlock, 0, s>>>(irow, sop, hubq, c, cap, hub_off);
         GX_LAUNCHED("bc sigma");
@@ -603,7 +603,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
       bc_plain_clear_kernel<<<1, 64, 0, s>>>(c);
       BwdOp op{g->col, level, sigma, delta, static_cast<uint32_t>(lv + 1)};
       bc_expand_kernel<BwdOp><<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(
-          g->row, op, order + b, cnt, hubq, c, cap);
+          g->row, op, order + b, cnt, hubq, c, cap, hub);
       hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
       bc_hubs_kernel<BwdOp><<<grid, kBlock, 0, s>>>(g->row, op, hubq, c, cap, hub_off);
       bc_plain_finish_level_kernel<<<grid_for(cnt, kBlock, 4096), kBlock, 0, s>>>(order + b, cnt, sigma, delta, bc);

@@ -39,7 +39,6 @@ constexpr int kBlock = 256;
 constexpr int kQCap = 2048;
 constexpr uint32_t kInf = GX_DIST_INFINITY;
 constexpr uint32_t kNoParent = 0xffffffffu;
-constexpr uint32_t kHub = 1024;
 constexpr int kGrid = 256 * 8;
 constexpr int kLevelsPerGraph = 8;
 
@@ -68,6 +67,7 @@ struct Bufs {
   BfsState* S;
   unsigned cap;
   uint32_t* hub_off;
+  uint32_t hub;  // frontier.hub_degree
 };
 
 __global__ void bfs_init_kernel(uint32_t* __restrict__ level, uint32_t* __restrict__ parent, int64_t n,
@@ -157,7 +157,7 @@ __global__ void __launch_bounds__(kBlock) td_expand_kernel(Bufs B) {
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
   TdOp op{B, &sq, S->level, 0ull};
-  expand_queue<kBlock>(B.row, B.q[S->q], S->count[S->q], kHub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
+  expand_queue<kBlock>(B.row, B.q[S->q], S->count[S->q], B.hub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
   td_finish(op, nullptr);
 }
 
@@ -285,7 +285,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   const unsigned cap = static_cast<unsigned>(n);
   Bufs B{g->row, g->col, level, nullptr, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, n + 1)};
+         g->scratch<uint32_t>(WS_HUBOFF, n + 1), hub_degree()};
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));
   RegionTimer region(s);
   KernelTimer ktimer(s);
@@ -296,7 +296,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.level, (const void*)B.q[0],
                         (const void*)B.q[1], (const void*)B.hubq, (const void*)B.S, (const void*)B.hub_off})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
-  key = mix_key(key, cap);
+  key = mix_key(mix_key(key, cap), B.hub);
   uint32_t replays = 0;
   for (;;) {
     ktimer.begin();
@@ -350,7 +350,7 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   const unsigned cap = static_cast<unsigned>(n);
   Bufs B{g->row, g->col, level, parent, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, n + 1)};
+         g->scratch<uint32_t>(WS_HUBOFF, n + 1), hub_degree()};
   const int64_t words = (n + 31) / 32;
   uint32_t* bm[2] = {g->scratch<uint32_t>(WS_BITMAP0, words), g->scratch<uint32_t>(WS_BITMAP1, words)};
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));

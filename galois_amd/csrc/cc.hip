@@ -35,7 +35,6 @@ using namespace gx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kHub = 1024;
 
 __device__ __forceinline__ uint32_t ld(const uint32_t* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -161,8 +160,8 @@ struct HookOp {
 __global__ void __launch_bounds__(kBlock) cc_hook_kernel(const int32_t* __restrict__ row, HookOp op,
                                                          const uint32_t* __restrict__ q, const unsigned* count,
                                                          uint32_t* __restrict__ hubq, unsigned* hub_count,
-                                                         unsigned cap, unsigned* overflow) {
-  expand_queue<kBlock>(row, q, *count, kHub, hubq, hub_count, cap, overflow, op);
+                                                         unsigned cap, unsigned* overflow, uint32_t hub) {
+  expand_queue<kBlock>(row, q, *count, hub, hubq, hub_count, cap, overflow, op);
 }
 
 __global__ void __launch_bounds__(kBlock) cc_hubs_kernel(const int32_t* __restrict__ row, HookOp op,
@@ -210,7 +209,8 @@ extern "C" gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* stp) {
     cc_largest_kernel<<<1, 256, 0, s>>>(parent, n, &ctr[3]);
     cc_rest_list_kernel<<<vgrid, kBlock, 0, s>>>(g->row, parent, n, &ctr[3], rest, &ctr[2], cap, &ctr[1]);
     HookOp op{g->row, g->col, parent};
-    cc_hook_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, rest, &ctr[2], hubq, &ctr[0], cap, &ctr[1]);
+    cc_hook_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, rest, &ctr[2], hubq, &ctr[0], cap, &ctr[1],
+                                                 hub_degree());
     GX_LAUNCHED("cc_hook_kernel");
     hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &ctr[0], cap, hub_off);
     cc_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, &ctr[0], cap, hub_off);

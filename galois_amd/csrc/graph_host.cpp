@@ -36,6 +36,50 @@ extern "C" size_t gx_last_error(char* buf, size_t len) {
 
 extern "C" void gx_set_profiling(int on) { g_profiling.store(on ? 1 : 0); }
 
+namespace gx {
+namespace {
+struct Knob {
+  const char* key;
+  std::atomic<int64_t> value;
+  int64_t lo, hi;
+};
+Knob g_knobs[] = {
+    {"frontier.hub_degree", {-1}, 64, int64_t(1) << 30},
+    {"pr.hot_sources", {-1}, 0, int64_t(1) << 31},
+};
+Knob* find_knob(const char* key) {
+  if (!key) return nullptr;
+  for (Knob& k : g_knobs)
+    if (std::strcmp(k.key, key) == 0) return &k;
+  return nullptr;
+}
+}  // namespace
+
+int64_t tune_value(const char* key, int64_t dflt) {
+  Knob* k = find_knob(key);
+  const int64_t v = k ? k->value.load() : -1;
+  return v < 0 ? dflt : v;
+}
+}  // namespace gx
+
+extern "C" gx_status gx_tune_set(const char* key, int64_t value) {
+  GX_API_BEGIN
+  Knob* k = find_knob(key);
+  GX_REQUIRE(k, std::string("unknown tuning key ") + (key ? key : "(null)"));
+  GX_REQUIRE(value == -1 || (value >= k->lo && value <= k->hi), std::string("value out of range for ") + key);
+  k->value.store(value);
+  GX_API_END
+}
+
+extern "C" gx_status gx_tune_get(const char* key, int64_t* value) {
+  GX_API_BEGIN
+  Knob* k = find_knob(key);
+  GX_REQUIRE(k, std::string("unknown tuning key ") + (key ? key : "(null)"));
+  check_ptr(value, "value");
+  *value = k->value.load();
+  GX_API_END
+}
+
 extern "C" gx_status gx_init(int device) {
   GX_API_BEGIN
   int count = 0;

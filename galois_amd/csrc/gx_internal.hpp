@@ -106,7 +106,7 @@ constexpr int64_t kEdgePad = 4096;
 // Tables of the pull PageRank sweep over the transpose, built once per graph.
 struct PullPlan {
   bool ready = false;
-  int64_t nwin = 0, nlong = 0, npieces = 0;
+  int64_t ntiles = 0;
   DevBuf buf;
 };
 
@@ -118,6 +118,11 @@ struct PullPlan {
 struct DegreeOrder {
   bool ready = false;
   DevBuf new_of_old, old_of_new, trow, tcol, outdeg;
+  // Hot/cold split of the transpose rows by source id (< hot / >= hot):
+  // two CSRs over the same rows (PageRank pull).
+  int64_t hot = -1, hot_edges = 0;
+  DevBuf rowA, colA, rowB, colB;
+  PullPlan planA, planB;
 };
 
 // A fixed kernel sequence captured once into an executable HIP graph and
@@ -206,6 +211,10 @@ struct gx_graph {
 namespace gx {
 
 bool profiling_enabled();
+// Current value of a tuning knob (gx_tune_set), `dflt` when unset.
+int64_t tune_value(const char* key, int64_t dflt);
+// Degree above which a frontier vertex is expanded by the whole grid.
+inline uint32_t hub_degree() { return static_cast<uint32_t>(tune_value("frontier.hub_degree", 1024)); }
 
 // Accumulates the device time of the launches of one kernel with HIP events
 // (enabled by gx_set_profiling).  Events are recorded on the graph's stream,

@@ -27,7 +27,6 @@ using namespace gx;
 namespace {
 
 constexpr int kBlock = 256;
-constexpr uint32_t kHub = 1024;
 
 struct KcoreCounters {
   unsigned count[2];
@@ -76,6 +75,7 @@ struct PeelOp {
   uint32_t k;
   int out;
   unsigned cap;
+  uint32_t hub;  // frontier.hub_degree
   __device__ __forceinline__ void operator()(bool valid, uint32_t, int32_t e) {
     bool dies = false;
     uint32_t v = 0;
@@ -98,7 +98,7 @@ __global__ void __launch_bounds__(kBlock) kcore_peel_kernel(const int32_t* __res
   lq_init(sq);
   op.sq = &sq;
   const unsigned cnt = op.c->count[in];
-  expand_queue<kBlock>(row, qin, cnt, kHub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  expand_queue<kBlock>(row, qin, cnt, op.hub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
   op.flush(true);
 }
 
@@ -166,7 +166,7 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
     items += count;
     const int out = in ^ 1;
     kcore_round_begin_kernel<<<1, 64, 0, s>>>(c, out);
-    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap};
+    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap, hub_degree()};
     ktimer.begin();
     kcore_peel_kernel<<<std::min<unsigned>(256 * 8, grid_for(count, kBlock)), kBlock, 0, s>>>(g->row, op, q[in], in,
                                                                                               hubq);

@@ -15,15 +15,14 @@
 //   after max_iter iterations.
 //
 // Design of the pull sweep (gfx950, HBM/latency bound, no MFMA)
-//  - The transpose (CSC) is cut into edge windows of W edges.  Rows are
-//    assigned to the window holding their first in-edge, so a window's rows
-//    are consecutive vertex ids and its edges are one contiguous range of at
-//    most 2W.  One workgroup per window streams tcol coalesced, gathers y into
-//    LDS, then reduces rows out of LDS: a lane per row when the window holds
-//    many rows, a wavefront per row (strided lanes + wave reduction) when it
-//    holds few.  Rows longer than W are cut into pieces of P edges reduced by
-//    whole workgroups; their per-piece partials are combined in piece order.
-//    The window table and the piece table are built once per graph.
+//  - The pull runs over the transpose in degree order (vertices relabelled
+//    by out-degree, descending), so the most-gathered y entries share cache
+//    lines; results are mapped back to the caller's ids at the end.
+//  - The row sums are the merge-path tile sweep of rowsweep.hpp: tiles of a
+//    fixed number of row ends + edges, columns staged with 16-byte loads,
+//    every gather of a tile in flight at once, per-lane walks in LDS, and
+//    coalesced per-row finalisation (x', y' = x'/outdeg, residual, dangling
+//    mass).  Rows spanning tiles are completed by a second small launch.
 //  - FP32 mode: compensated (Neumaier) row sums in registers and
 //    TwoSum-merged wave reductions; x and y stored in FP32; the residual and
 //    dangling mass are accumulated in FP64 per workgroup and reduced in a
@@ -32,6 +31,7 @@
 //    reads a device `converged` flag and returns at once when it is set, so
 //    surplus launches after convergence are no-ops.
 #include <algorithm>
+#include <cstdlib>
 #include <cmath>
 
 #include "frontier.hpp"
@@ -46,7 +46,6 @@ namespace {
 using rs::Acc;
 using rs::block_sum;
 using rs::kBlock;
-using rs::kLongGrid;
 using rs::kSweepGrid;
 
 struct PrState {
@@ -76,16 +75,24 @@ struct PrFin {
   Real* __restrict__ x_new;
   Real* __restrict__ y_new;
   const uint32_t* __restrict__ outdeg;
+  const Real* __restrict__ part;  // row sums of the hot pass (null: single pass)
   Real base, alpha;
   __device__ __forceinline__ void load() {
     base = static_cast<Real>(state->base);
     alpha = static_cast<Real>(state->alpha);
   }
-  __device__ __forceinline__ void operator()(uint32_t r, Real sum, double& resid, double& dang) const {
-    const Real xn = base + alpha * sum;
-    resid += fabs(static_cast<double>(xn) - static_cast<double>(rs::ld_stream(&x_old[r])));
+  struct Pre {
+    Real xo, pa;
+    uint32_t d;
+  };
+  __device__ __forceinline__ Pre pre(uint32_t r) const {
+    return Pre{rs::ld_stream(&x_old[r]), part ? rs::ld_stream(&part[r]) : Real(0), rs::ld_stream(&outdeg[r])};
+  }
+  __device__ __forceinline__ void operator()(uint32_t r, Real sum, const Pre& p, double& resid, double& dang) const {
+    const Real xn = base + alpha * (p.pa + sum);
+    resid += fabs(static_cast<double>(xn) - static_cast<double>(p.xo));
     __builtin_nontemporal_store(xn, &x_new[r]);
-    const uint32_t d = rs::ld_stream(&outdeg[r]);
+    const uint32_t d = p.d;
     if (d) {
       y_new[r] = xn / static_cast<Real>(d);
     } else {
@@ -94,6 +101,60 @@ struct PrFin {
     }
   }
 };
+
+// Hot pass of the split pull: the row sum over hot sources is stored.
+template <class Real>
+struct PartFin {
+  Real* __restrict__ part;
+  struct Pre {};
+  __device__ __forceinline__ void load() {}
+  __device__ __forceinline__ Pre pre(uint32_t) const { return Pre{}; }
+  __device__ __forceinline__ void operator()(uint32_t r, Real sum, const Pre&, double&, double&) const {
+    __builtin_nontemporal_store(sum, &part[r]);
+  }
+};
+
+// Hot/cold split: per row, the number of in-edges from sources < hot (a
+// prefix: sources are ascending within a row) and the rest.
+__global__ void split_count_kernel(const int32_t* __restrict__ rowoff, const int32_t* __restrict__ col, int64_t n,
+                                   int32_t hot, uint32_t* __restrict__ ca, uint32_t* __restrict__ cb) {
+  for (int64_t r = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; r <= n;
+       r += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    if (r == n) {
+      ca[n] = cb[n] = 0;
+      continue;
+    }
+    const int32_t b = rowoff[r], e = rowoff[r + 1];
+    int32_t lo = b, hi = e;  // first edge with col >= hot
+    while (lo < hi) {
+      const int32_t mid = lo + ((hi - lo) >> 1);
+      if (col[mid] < hot) lo = mid + 1; else hi = mid;
+    }
+    ca[r] = static_cast<uint32_t>(lo - b);
+    cb[r] = static_cast<uint32_t>(e - lo);
+  }
+}
+
+// One wavefront per row copies its hot prefix and cold suffix.
+__global__ void __launch_bounds__(kBlock) split_copy_kernel(const int32_t* __restrict__ rowoff,
+                                                            const int32_t* __restrict__ col, int64_t n,
+                                                            const int32_t* __restrict__ rowA,
+                                                            const int32_t* __restrict__ rowB,
+                                                            int32_t* __restrict__ colA, int32_t* __restrict__ colB) {
+  const int64_t waves = static_cast<int64_t>(gridDim.x) * (kBlock / kWave);
+  const unsigned lane = lane_id();
+  for (int64_t r = blockIdx.x * static_cast<int64_t>(kBlock / kWave) + threadIdx.x / kWave; r < n; r += waves) {
+    const int32_t b = rowoff[r], e = rowoff[r + 1];
+    const int32_t a0 = rowA[r], k = rowA[r + 1] - a0, b0 = rowB[r];
+    for (int32_t i = static_cast<int32_t>(lane); i < e - b; i += kWave) {
+      const int32_t c = col[b + i];
+      if (i < k)
+        colA[a0 + i] = c;
+      else
+        colB[b0 + i - k] = c;
+    }
+  }
+}
 
 // Fixed-order reduction of the per-workgroup partials; sets the next base,
 // the convergence flag and the buffer index of the newest iterate.
@@ -177,25 +238,73 @@ __global__ void pr_unpermute_kernel(const Real* __restrict__ x, const uint32_t* 
 
 }  // namespace
 
+// Hot/cold split of the degree-ordered transpose at source id `hot`.
+static void build_split(gx_graph* g, int64_t hot) {
+  DegreeOrder& D = g->degree_order;
+  if (D.hot == hot) return;
+  const int64_t n = g->n, m = g->m;
+  hipStream_t s = g->stream;
+  DevBuf ca, cb;
+  ca.ensure((n + 1) * 4);
+  cb.ensure((n + 1) * 4);
+  split_count_kernel<<<grid_for(n + 1, kBlock, 8192), kBlock, 0, s>>>(
+      D.trow.as<int32_t>(), D.tcol.as<int32_t>(), n, static_cast<int32_t>(hot), ca.as<uint32_t>(), cb.as<uint32_t>());
+  GX_LAUNCHED("split_count_kernel");
+  D.rowA.ensure((n + 1) * 4);
+  D.rowB.ensure((n + 1) * 4);
+  exclusive_scan_u32(ca.as<uint32_t>(), D.rowA.as<uint32_t>(), n + 1, s);
+  exclusive_scan_u32(cb.as<uint32_t>(), D.rowB.as<uint32_t>(), n + 1, s);
+  uint32_t ma = 0;
+  read_bytes(g, &ma, D.rowA.as<uint32_t>() + n, 4);
+  D.colA.ensure(std::max<int64_t>(ma, 1) * 4);
+  D.colB.ensure(std::max<int64_t>(m - ma, 1) * 4);
+  split_copy_kernel<<<4096, kBlock, 0, s>>>(D.trow.as<int32_t>(), D.tcol.as<int32_t>(), n, D.rowA.as<int32_t>(),
+                                            D.rowB.as<int32_t>(), D.colA.as<int32_t>(), D.colB.as<int32_t>());
+  GX_LAUNCHED("split_copy_kernel");
+  GX_HIP(hipStreamSynchronize(s));
+  D.planA.ready = D.planB.ready = false;
+  D.hot = hot;
+  D.hot_edges = ma;
+}
+
+// Sources gathered in the hot pass (knob "pr.hot_sources"; default: the top
+// ids of the degree order whose y fills 2 MB, half of one XCD's 4 MB L2).
+// 0 = single pass.
+static int64_t hot_sources(int64_t n, size_t real_bytes) {
+  int64_t h = static_cast<int64_t>((size_t(2) << 20) / real_bytes);
+  h = tune_value("pr.hot_sources", h);
+  return (h <= 0 || h >= n) ? 0 : h;
+}
+
 // Normalised pull PageRank over the degree-ordered transpose.
 template <class Real>
 static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, Real* pr, gx_stats* st) {
   const int64_t n = g->n;
   hipStream_t s = g->stream;
   build_degree_order(g);
-  const DegreeOrder& D = g->degree_order;
-  const rs::RowPlan P = rs::build_plan(g, D.trow.as<int32_t>(), D.tcol.as<int32_t>(), n, g->m, g->pull_plan);
+  DegreeOrder& D = g->degree_order;
+  const int64_t hot = hot_sources(n, sizeof(Real));
+  rs::RowPlan P{}, PA{};
+  if (hot > 0) {
+    build_split(g, hot);
+    PA = rs::build_plan(g, D.rowA.as<int32_t>(), D.colA.as<int32_t>(), n, D.hot_edges, D.planA);
+    P = rs::build_plan(g, D.rowB.as<int32_t>(), D.colB.as<int32_t>(), n, g->m - D.hot_edges, D.planB);
+  } else {
+    P = rs::build_plan(g, D.trow.as<int32_t>(), D.tcol.as<int32_t>(), n, g->m, g->pull_plan);
+  }
+  Real* part = hot > 0 ? g->scratch<Real>(WS_V3, n) : nullptr;
+  Real* carryA = hot > 0 ? g->scratch<Real>(WS_V4, 3 * PA.ntiles) : nullptr;
+  double* partsA = hot > 0 ? g->scratch<double>(WS_V5, 2 * rs::kParts) : nullptr;
   const uint32_t* outdeg = D.outdeg.as<uint32_t>();
   Real* xb = g->scratch<Real>(WS_V0, 2 * n);
   Real* yb = g->scratch<Real>(WS_V1, 2 * n);
   Real* x[2] = {xb, xb + n};
   Real* y[2] = {yb, yb + n};
-  const int nparts = kSweepGrid + kLongGrid;
+  const int nparts = rs::kParts;
   double* parts = g->scratch<double>(WS_REDUCE, 2 * nparts);
   double* part_resid = parts;
   double* part_dang = parts + nparts;
-  Real* piece_s = g->scratch<Real>(WS_V2, std::max<int64_t>(P.npieces, 1));
-  Real* piece_c = g->scratch<Real>(WS_V3, std::max<int64_t>(P.npieces, 1));
+  Real* carry = g->scratch<Real>(WS_V2, 3 * P.ntiles);
   PrState* state = g->scratch<PrState>(WS_COUNTERS, 1);
   PrState* host = static_cast<PrState*>(g->host_pinned(sizeof(PrState)));
 
@@ -218,12 +327,19 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
     for (; it < end; ++it) {
       const int nxt = cur ^ 1;
       PrSrc<Real> src{y[cur]};
-      PrFin<Real> fin{state, x[cur], x[nxt], y[nxt], outdeg, Real(0), Real(0)};
-      const int np = rs::launch_sweep<Real>(P, src, fin, &state->converged, part_resid, part_dang, piece_s, piece_c,
-                                           s, &ktimer);
+      PrFin<Real> fin{state, x[cur], x[nxt], y[nxt], outdeg, part, Real(0), Real(0)};
+      ktimer.begin();
+      if (hot > 0) {
+        rs::launch_sweep<Real>(PA, src, PartFin<Real>{part}, &state->converged, partsA, partsA + rs::kParts, carryA,
+                               s, nullptr);
+        launches += 2;
+      }
+      const int np = rs::launch_sweep<Real>(P, src, fin, &state->converged, part_resid, part_dang, carry, s,
+                                           nullptr);
+      ktimer.end();
       pr_reduce_kernel<<<1, kBlock, 0, s>>>(state, part_resid, part_dang, np, static_cast<unsigned>(nxt));
       GX_LAUNCHED("pr_reduce_kernel");
-      launches += (np > kSweepGrid) ? 3 : 2;
+      launches += 3;
       cur = nxt;
     }
     GX_HIP(hipMemcpyAsync(host, state, sizeof(PrState), hipMemcpyDeviceToHost, s));
@@ -495,11 +611,11 @@ struct PushOp {
 
 __global__ void __launch_bounds__(kBlock) push_spread_kernel(const int32_t* __restrict__ row, PushOp op,
                                                              const uint32_t* __restrict__ pushers, int in,
-                                                             uint32_t* __restrict__ hubq) {
+                                                             uint32_t* __restrict__ hubq, uint32_t hub) {
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
   op.sq = &sq;
-  expand_queue<kBlock>(row, pushers, op.c->count[in], 4096u, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  expand_queue<kBlock>(row, pushers, op.c->count[in], hub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
   op.flush(true);
 }
 
@@ -575,7 +691,7 @@ extern "C" gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol, uint3
         active, c, 0, pr, residual, delta, g->row, alpha, tol, pushers, 1, cap);
     push_set_kernel<<<1, 64, 0, s>>>(c, 0, 0u);
     PushOp op{g->col, delta, residual, stamp, active, c, nullptr, tol, it, 0, cap};
-    push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq);
+    push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq, hub_degree());
     hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
     push_hubs_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
     ktimer.end();

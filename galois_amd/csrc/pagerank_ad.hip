@@ -30,7 +30,6 @@ namespace {
 
 using rs::block_sum;
 using rs::kBlock;
-using rs::kLongGrid;
 using rs::kSweepGrid;
 
 struct AdState {
@@ -97,11 +96,16 @@ struct FwdFin {
     base = static_cast<Real>(state->base);
     alpha = static_cast<Real>(state->alpha);
   }
-  __device__ __forceinline__ void operator()(uint32_t r, Real sum, double& resid, double& dang) const {
-    const Real xn = (Real(1) - alpha) * static_cast<Real>(t[r]) + base + alpha * sum;
-    resid += fabs(static_cast<double>(xn) - static_cast<double>(x_old[r]));
+  struct Pre {
+    float t;
+    Real is, xo;
+  };
+  __device__ __forceinline__ Pre pre(uint32_t r) const { return Pre{t[r], inv_s[r], x_old[r]}; }
+  __device__ __forceinline__ void operator()(uint32_t r, Real sum, const Pre& p, double& resid, double& dang) const {
+    const Real xn = (Real(1) - alpha) * static_cast<Real>(p.t) + base + alpha * sum;
+    resid += fabs(static_cast<double>(xn) - static_cast<double>(p.xo));
     x_new[r] = xn;
-    const Real is = inv_s[r];
+    const Real is = p.is;
     y_new[r] = xn * is;
     if (is == Real(0)) dang += static_cast<double>(xn);
   }
@@ -124,15 +128,27 @@ struct AdjFin {
     base = static_cast<Real>(state->base);
     alpha = static_cast<Real>(state->alpha);
   }
-  __device__ __forceinline__ void operator()(uint32_t r, Real sum, double& resid, double& tot) const {
-    const Real is = inv_s[r];
+  struct Pre {
+    Real is, lo;
+    float c;
+  };
+  __device__ __forceinline__ Pre pre(uint32_t r) const {
+    Pre q{inv_s[r], Real(0), 0.f};
+    if (!final_pass) {
+      q.lo = lam_old[r];
+      q.c = c[r];
+    }
+    return q;
+  }
+  __device__ __forceinline__ void operator()(uint32_t r, Real sum, const Pre& q, double& resid, double& tot) const {
+    const Real is = q.is;
     const Real p = (is == Real(0)) ? base : sum * is;
     if (final_pass) {
       pl[r] = p;
       return;
     }
-    const Real ln = static_cast<Real>(c[r]) + alpha * p;
-    resid += fabs(static_cast<double>(ln) - static_cast<double>(lam_old[r]));
+    const Real ln = static_cast<Real>(q.c) + alpha * p;
+    resid += fabs(static_cast<double>(ln) - static_cast<double>(q.lo));
     lam_new[r] = ln;
     tot += static_cast<double>(ln);
   }
@@ -240,8 +256,8 @@ struct GradOp {
 template <class Real>
 __global__ void __launch_bounds__(kBlock) grad_edges_kernel(const int32_t* __restrict__ row, GradOp<Real> op,
                                                             unsigned n, uint32_t* __restrict__ hubq,
-                                                            unsigned* hub_count, unsigned* overflow) {
-  expand_queue<kBlock>(row, nullptr, n, 4096u, hubq, hub_count, n, overflow, op);
+                                                            unsigned* hub_count, unsigned* overflow, uint32_t hub) {
+  expand_queue<kBlock>(row, nullptr, n, hub, hubq, hub_count, n, overflow, op);
 }
 
 template <class Real>
@@ -285,7 +301,7 @@ uint32_t iterate(gx_graph* g, AdState* state, AdState* host, double* p0, double*
       const int np = sweep(cur, nxt);
       ad_reduce_kernel<<<1, kBlock, 0, s>>>(state, p0, p1, np, mode, static_cast<unsigned>(nxt));
       GX_LAUNCHED("ad_reduce_kernel");
-      launches += (np > kSweepGrid) ? 3 : 2;
+      launches += 3;
       cur = nxt;
     }
     GX_HIP(hipMemcpyAsync(host, state, sizeof(AdState), hipMemcpyDeviceToHost, s));
@@ -310,10 +326,9 @@ void run_forward(gx_graph* g, const float* w, const float* t, double alpha, doub
   Real* yb = g->scratch<Real>(WS_V1, 2 * n);
   Real* xs[2] = {xb, xb + n};
   Real* ys[2] = {yb, yb + n};
-  const int nparts = kSweepGrid + kLongGrid;
+  const int nparts = rs::kParts;
   double* parts = g->scratch<double>(WS_REDUCE, 2 * nparts);
-  Real* piece_s = g->scratch<Real>(WS_V2, std::max<int64_t>(P.npieces, 1));
-  Real* piece_c = g->scratch<Real>(WS_V3, std::max<int64_t>(P.npieces, 1));
+  Real* carry = g->scratch<Real>(WS_V2, 3 * P.ntiles);
   AdState* state = g->scratch<AdState>(WS_COUNTERS, 1);
   AdState* host = static_cast<AdState*>(g->host_pinned(sizeof(AdState)));
   RegionTimer region(s);
@@ -328,7 +343,7 @@ void run_forward(gx_graph* g, const float* w, const float* t, double alpha, doub
   auto sweep = [&](int cur, int nxt) {
     WSrc<Real> src{ys[cur], wt};
     FwdFin<Real> fin{state, t, inv_s, xs[cur], xs[nxt], ys[nxt], Real(0), Real(0)};
-    return rs::launch_sweep<Real>(P, src, fin, &state->converged, parts, parts + nparts, piece_s, piece_c, s,
+    return rs::launch_sweep<Real>(P, src, fin, &state->converged, parts, parts + nparts, carry, s,
                                   &ktimer);
   };
   const uint32_t iters = iterate(g, state, host, parts, parts + nparts, 0, max_iter, sweep, launches);
@@ -357,10 +372,9 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   Real* lb = g->scratch<Real>(WS_V0, 2 * n);
   Real* pl = g->scratch<Real>(WS_V1, n);
   Real* lam[2] = {lb, lb + n};
-  const int nparts = kSweepGrid + kLongGrid;
+  const int nparts = rs::kParts;
   double* parts = g->scratch<double>(WS_REDUCE, 2 * nparts);
-  Real* piece_s = g->scratch<Real>(WS_V2, std::max<int64_t>(P.npieces, 1));
-  Real* piece_c = g->scratch<Real>(WS_V3, std::max<int64_t>(P.npieces, 1));
+  Real* carry = g->scratch<Real>(WS_V2, 3 * P.ntiles);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, n);
   unsigned* hubc = g->scratch<unsigned>(WS_Q3, 4);
   uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, n + 1);
@@ -379,7 +393,7 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   auto sweep = [&](int cur, int nxt) {
     WSrc<Real> src{lam[cur], wsrc};
     AdjFin<Real> fin{state, c, inv_s, lam[cur], lam[nxt], pl, 0, Real(0), Real(0)};
-    return rs::launch_sweep<Real>(P, src, fin, &state->converged, parts, parts + nparts, piece_s, piece_c, s,
+    return rs::launch_sweep<Real>(P, src, fin, &state->converged, parts, parts + nparts, carry, s,
                                   &ktimer);
   };
   const uint32_t iters = iterate(g, state, host, parts, parts + nparts, 1, max_iter, sweep, launches);
@@ -389,13 +403,13 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   {
     WSrc<Real> src{lam_f, wsrc};
     AdjFin<Real> fin{state, c, inv_s, lam_f, nullptr, pl, 1, Real(0), Real(0)};
-    rs::launch_sweep<Real>(P, src, fin, nullptr, parts, parts + nparts, piece_s, piece_c, s, nullptr);
+    rs::launch_sweep<Real>(P, src, fin, nullptr, parts, parts + nparts, carry, s, nullptr);
     launches += 2;
   }
   zero_u32_kernel<<<1, 64, 0, s>>>(hubc, 2);
   GradOp<Real> op{g->col, x, lam_f, pl, inv_s, grad_w, static_cast<Real>(alpha)};
   grad_edges_kernel<Real><<<grid_for(n, kBlock, 256 * 16), kBlock, 0, s>>>(g->row, op, static_cast<unsigned>(n), hubq,
-                                                                          hubc, hubc + 1);
+                                                                          hubc, hubc + 1, hub_degree());
   hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, hubc, static_cast<unsigned>(n), hub_off);
   grad_hubs_kernel<Real><<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hubc, static_cast<unsigned>(n), hub_off);
   grad_t_kernel<Real><<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(lam_f, n, static_cast<Real>(1.0 - alpha), grad_t);

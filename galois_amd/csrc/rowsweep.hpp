@@ -2,30 +2,43 @@
 // value(e, col[e]) for every row, the pull step of PageRank (over the
 // transpose) and of its adjoint (over the CSR).
 //
-// Layout.  The edge array is cut into windows of kWin edges; each row is
-// assigned to the window that holds its first edge, so a window owns a run of
-// consecutive rows and at most 2*kWin contiguous edges.  Rows longer than
-// kWin are cut into pieces of kPiece edges.  One launch covers every window
-// and every piece (items interleaved over the grid); a second small launch
-// combines the piece partials of the long rows in piece order.
+// Layout (merge path).  A sequential CSR walk visits n row ends and m edges
+// in one interleaved order; that sequence is cut into tiles of kTile items,
+// so every tile holds exactly kTile row ends + edges whatever the degree mix
+// (empty rows, hubs and everything between cost the same per item).  The
+// plan stores each tile's first row and first edge.
 //
-// Window work.  Lane t of a 256-lane workgroup owns 16 consecutive edge
-// slots (16-byte column loads), evaluates their values into registers and
-// sums them row segment by row segment.  A row inside one lane's slots is
-// finalised by that lane; a row that crosses lanes is finalised by the lane
-// holding its first edge, which adds the continuation partials left in LDS by
-// the later lanes, in lane order.  Sums are compensated (Neumaier) in FP32,
-// plain in FP64; every reduction has a fixed order, so results are
-// bit-reproducible run to run.
+// Tile work (one 256-lane workgroup).
+//  1. Stage: the tile's row ends go to LDS; its columns are read with 16-byte
+//     loads and every edge value is evaluated at once by all lanes (up to
+//     kUnits gathers in flight per lane) into LDS.
+//  2. Walk: lane t takes merge-path items [t*kUnits, (t+1)*kUnits), finds its
+//     starting (row, edge) by binary search in LDS and walks them in order,
+//     adding edge values and emitting a row sum at every row end.
+//  3. Carries: the partial of the row left open at the end of each lane is
+//     combined by a segmented scan over lanes (fixed tree order); the lane
+//     that emits a row adds the carries of the earlier lanes of that row.
+//  4. Finalise: the tile's row sums are written to LDS and finalised by
+//     consecutive lanes, so the per-row loads and stores are coalesced.
+// A row that began in an earlier tile is not finalised in the tile where it
+// ends: that tile stores its partial (head) and every tile stores the partial
+// of the row open at its end (tail); a second small launch adds the tails of
+// the earlier tiles to the head, in tile order, and finalises those rows.
+// Sums are compensated (Neumaier) in FP32 and plain in FP64; every reduction
+// has a fixed order, so results are bit-reproducible run to run.
 //
 // Src  : __device__ Real operator()(int64_t e, int32_t c) const  (value of
 //        edge slot e whose column id is c)
 // Fin  : __device__ void load()  (read per-sweep scalars, once per thread)
-//        __device__ void operator()(uint32_t r, Real sum, double& a0,
-//        double& a1) const  (finalise row r; two FP64 per-workgroup
-//        accumulators, e.g. residual and dangling mass)
+//        struct Pre; __device__ Pre pre(uint32_t r) const  (per-row operands,
+//        loaded early so their latency overlaps the tile's gathers)
+//        __device__ void operator()(uint32_t r, Real sum, const Pre&,
+//        double& a0, double& a1) const  (finalise row r; two FP64
+//        per-workgroup accumulators, e.g. residual and dangling mass)
 #pragma once
 
+#include <algorithm>
+#include <climits>
 #include <cmath>
 
 #include "gx_device.hpp"
@@ -35,13 +48,13 @@ namespace gx {
 namespace rs {
 
 constexpr int kBlock = 256;
-constexpr int kWin = 2040;       // edges per window (rows up to kWin stay in windows)
-constexpr int kPiece = 4096;     // edges per long-row piece
-constexpr int kPerLane = 16;     // edge slots per lane
-constexpr int kRowCap = 1024;    // rows per window held in LDS
-constexpr int kSweepGrid = 256 * 8;
-constexpr int kLongGrid = 2048;
-static_assert(kPerLane * kBlock >= 2 * kWin + 3, "window must fit the lanes' slots");
+constexpr int kUnits = 8;                  // merge-path items per lane
+constexpr int kTile = kBlock * kUnits;     // items (row ends + edges) per tile
+constexpr int kWaves = kBlock / kWave;
+constexpr int kSweepGrid = 256 * 8;        // persistent workgroups of the tile launch
+constexpr int kFixGrid = 256;              // workgroups of the spanning-row launch
+constexpr int kParts = kSweepGrid + kFixGrid;
+constexpr int kChunks = (kTile / 4 + 1 + kBlock - 1) / kBlock;  // 16-byte column loads per lane
 
 template <class Real>
 struct Acc;
@@ -74,17 +87,6 @@ struct Acc<double> {
   __device__ __forceinline__ double comp() const { return 0.0; }
 };
 
-template <class Real>
-__device__ __forceinline__ Acc<Real> wave_reduce(Acc<Real> a) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const Real os = __shfl_xor(a.s, off);
-    const Real oc = __shfl_xor(a.comp(), off);
-    a.merge(os, oc);
-  }
-  return a;
-}
-
 // Workgroup sum of one double per lane in a fixed order (valid in lane 0).
 __device__ __forceinline__ double block_sum(double v, double* lds) {
   v = wave_sum(v);
@@ -107,176 +109,266 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 struct RowPlan {
   const int32_t* rowoff;  // n+1
   const int32_t* col;     // len
-  int64_t len;            // number of edges
-  int32_t* win_rows;      // nwin+1 first rows of windows (last = n)
-  int64_t nwin;
-  uint32_t* long_rows;    // rows with more than kWin edges
-  uint32_t* piece_first;  // nlong+1 exclusive prefix of pieces per long row
-  uint32_t* piece_row;    // owning long-row index of each piece
-  int64_t nlong, npieces;
+  int64_t n;              // rows
+  int64_t len;            // edges
+  int32_t* tile_row;      // ntiles+1: row of each tile's first item
+  int32_t* tile_edge;     // ntiles+1: edge of each tile's first item
+  int64_t ntiles;
+  bool vec;               // col is 16-byte aligned
 };
 
 // Builds (or returns the cached) plan of a row-offset array; defined in
 // rowsweep.hip.
 RowPlan build_plan(gx_graph* g, const int32_t* rowoff, const int32_t* col, int64_t n, int64_t m, PullPlan& cache);
 
+// LDS index with one pad word per 8: lanes walk items 8 apart, and the pad
+// spreads them over the banks.
+__device__ __forceinline__ int pad8(int j) { return j + (j >> 3); }
+
 template <class Real>
-struct SweepShared {
-  int32_t off[kRowCap + 1];
-  Real cs[kBlock];
-  Real cc[kBlock];
+struct TileShared {
+  Real val[kTile + kTile / 8];               // edge values (padded index)
+  alignas(16) char rows[kTile * sizeof(Real)];  // relative row ends, then row sums
+  Real ws[kWaves], wc[kWaves];               // per-wave tail segment of the carries
+  int wkey[kWaves], wfirst[kWaves];          // its key; key of the wave's lane 0
+  Real ts[kWaves], tc[kWaves];               // tile tail partial per wave
 };
 
-template <class Real, class Src, class Fin>
-__device__ __forceinline__ void sweep_window(int64_t k, const RowPlan& P, const Src& src, const Fin& fin,
-                                             SweepShared<Real>& sh, double& a0, double& a1) {
+// Inclusive scan of (s, c) over the lanes of a wavefront, restarted wherever
+// `key` changes (equal keys are contiguous).  Fixed tree order.
+template <class Real>
+__device__ __forceinline__ Acc<Real> wave_seg_scan(int key, Acc<Real> a) {
+  const int lane = static_cast<int>(lane_id());
+#pragma unroll
+  for (int off = 1; off < kWave; off <<= 1) {
+    const Real os = __shfl_up(a.s, off);
+    const Real oc = __shfl_up(a.comp(), off);
+    const int ok = __shfl_up(key, off);
+    if (lane >= off && ok == key) a.merge(os, oc);
+  }
+  return a;
+}
+
+struct TileCoord {
+  int32_t r0, r1, e0, e1;
+  bool defer0;  // row r0 began in an earlier tile and ends in this one
+};
+
+__device__ __forceinline__ TileCoord tile_coord(const RowPlan& P, int64_t T) {
+  TileCoord c;
+  c.r0 = P.tile_row[T];
+  c.r1 = P.tile_row[T + 1];
+  c.e0 = P.tile_edge[T];
+  c.e1 = P.tile_edge[T + 1];
+  c.defer0 = c.r1 > c.r0 && P.rowoff[c.r0] < c.e0;
+  return c;
+}
+
+// Columns of a tile, 16-byte loads from the 4-aligned edge below e0.
+__device__ __forceinline__ void tile_cols(const RowPlan& P, const TileCoord& tc, int32_t (&cc)[kChunks][4]) {
   const int tid = threadIdx.x;
-  const int32_t* __restrict__ rowoff = P.rowoff;
-  const int32_t r0 = P.win_rows[k], r1 = P.win_rows[k + 1];
-  if (r0 == r1) return;
-  int32_t rl = r1;
-  if (rowoff[r1] - rowoff[r1 - 1] > kWin) rl = r1 - 1;  // long last row: piece items
-  const int32_t nrows = rl - r0;
-  if (nrows == 0) return;
-  const int32_t e0 = rowoff[r0];
-  const int32_t nnz = rowoff[rl] - e0;
-  if (nrows > kRowCap) {
-    // Rare (long runs of empty rows): a lane per row.
-    for (int32_t r = r0 + tid; r < rl; r += kBlock) {
-      Acc<Real> a;
-      for (int32_t i = rowoff[r], end = rowoff[r + 1]; i < end; ++i) a.add(src(i, P.col[i]));
-      fin(r, a.value(), a0, a1);
+  const int32_t A = tc.e0 & ~3;
+  const int nch = (tc.e1 - A + 3) >> 2;
+#pragma unroll
+  for (int k = 0; k < kChunks; ++k) {
+    const int ch = tid + k * kBlock;
+    const int64_t p = static_cast<int64_t>(A) + 4 * ch;
+    if (P.vec && ch < nch && p + 4 <= P.len) {
+      const v4i c = ld_stream(reinterpret_cast<const v4i*>(P.col + p));
+      cc[k][0] = c.x;
+      cc[k][1] = c.y;
+      cc[k][2] = c.z;
+      cc[k][3] = c.w;
+    } else {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cc[k][j] = (ch < nch && p + j < P.len) ? P.col[p + j] : 0;
     }
-    return;
   }
-  for (int32_t i = tid; i <= nrows; i += kBlock) sh.off[i] = ld_stream(&rowoff[r0 + i]) - e0;
-  __syncthreads();
-  for (int32_t i = tid; i < nrows; i += kBlock)
-    if (sh.off[i] == sh.off[i + 1]) fin(r0 + i, Real(0), a0, a1);
-  // This lane's slots, relative to e0: [lo, hi) within [0, nnz).
+}
+
+// One tile (see the file comment).  `cc` holds the tile's columns; the next
+// tile's coordinates and columns are loaded into `ncoord`/`ncc` while this
+// one is processed.
+template <class Real, class Src, class Fin>
+__device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan& P, const Src& src, const Fin& fin,
+                                           TileShared<Real>& sh, Real* __restrict__ carry, const TileCoord& tc,
+                                           const int32_t (&cc)[kChunks][4], TileCoord& ncoord,
+                                           int32_t (&ncc)[kChunks][4], double& a0, double& a1) {
+  const int tid = threadIdx.x;
+  const int lane = static_cast<int>(lane_id());
+  const int wid = tid / kWave;
+  const int32_t r0 = tc.r0, e0 = tc.e0, e1 = tc.e1;
+  const int nre = tc.r1 - r0;  // row ends in this tile
+  const int ned = e1 - e0;     // edges in this tile
+  const int L = nre + ned;
+  const bool defer0 = tc.defer0;
+  int32_t* s_end = reinterpret_cast<int32_t*>(sh.rows);
+  Real* s_sum = reinterpret_cast<Real*>(sh.rows);
   const int32_t A = e0 & ~3;
-  const int32_t base_rel = A - e0 + tid * kPerLane;  // may be negative for lane 0
-  const int32_t lo = max(base_rel, 0), hi = min(base_rel + kPerLane, nnz);
-  Real v[kPerLane];
-  if (lo < hi) {
-    const int64_t abs0 = static_cast<int64_t>(A) + tid * kPerLane;
-    if (abs0 + kPerLane <= P.len) {
-      const v4i* p = reinterpret_cast<const v4i*>(P.col + abs0);
+
+  // 1. Issue every load of the tile at once: gathers, row ends, the first
+  // finalisation operands, and the next tile's coordinates and columns.
+  Real v[kChunks][4];
 #pragma unroll
-      for (int q = 0; q < kPerLane / 4; ++q) {
-        const v4i c = ld_stream(p + q);
-        const int32_t cc[4] = {c.x, c.y, c.z, c.w};
+  for (int k = 0; k < kChunks; ++k)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const int32_t i = base_rel + 4 * q + j;
-          v[4 * q + j] = (i >= lo && i < hi) ? src(abs0 + 4 * q + j, cc[j]) : Real(0);
+    for (int j = 0; j < 4; ++j) {
+      const int64_t q = static_cast<int64_t>(A) + 4 * (tid + k * kBlock) + j;
+      v[k][j] = (q >= e0 && q < e1) ? src(q, cc[k][j]) : Real(0);
+    }
+  int32_t rend0 = 0;
+  if (tid < nre) rend0 = ld_stream(&P.rowoff[r0 + 1 + tid]);
+  typename Fin::Pre pre0{};
+  if (tid < nre) pre0 = fin.pre(static_cast<uint32_t>(r0 + tid));
+  if (Tn < P.ntiles) {
+    ncoord = tile_coord(P, Tn);
+    tile_cols(P, ncoord, ncc);
+  }
+  if (tid < nre) s_end[tid] = rend0 - e0;
+  for (int j = tid + kBlock; j < nre; j += kBlock) s_end[j] = ld_stream(&P.rowoff[r0 + 1 + j]) - e0;
+#pragma unroll
+  for (int k = 0; k < kChunks; ++k)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t q = static_cast<int64_t>(A) + 4 * (tid + k * kBlock) + j;
+      if (q >= e0 && q < e1) sh.val[pad8(static_cast<int>(q - e0))] = v[k][j];
+    }
+  __syncthreads();
+
+  // 2. Walk this lane's merge-path items.
+  const int d = tid * kUnits;
+  int rr = 0, ee = 0;
+  if (d < L) {
+    int lo = max(0, d - ned), hi = min(d, nre);  // rows ended before item d
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (s_end[mid - 1] + mid <= d) lo = mid; else hi = mid - 1;
+    }
+    rr = lo;
+    ee = d - lo;
+  }
+  Acc<Real> acc, first;
+  Real em[kUnits];
+  uint32_t emask = 0;
+  int rstart = 0;
+#pragma unroll
+  for (int u = 0; u < kUnits; ++u) {
+    em[u] = Real(0);
+    if (d + u < L) {
+      if (rr < nre && ee >= s_end[rr]) {
+        if (emask == 0) {
+          first = acc;
+          rstart = rr;
         }
-      }
-    } else {
-#pragma unroll
-      for (int q = 0; q < kPerLane; ++q) {
-        const int32_t i = base_rel + q;
-        v[q] = (i >= lo && i < hi) ? src(abs0 + q, P.col[abs0 + q]) : Real(0);
-      }
-    }
-  }
-  // Segmented pass (no barrier inside: lanes of a wavefront may diverge).
-  bool owner_open = false;
-  int ri = 0;
-  Acc<Real> head;  // first segment when it continues a row begun in an earlier lane
-  Acc<Real> own;   // segment of the row this lane opened and that continues
-  if (lo < hi) {
-    int a = 0, b = nrows - 1;  // last row with offset <= lo
-    while (a < b) {
-      const int mid = (a + b + 1) >> 1;
-      if (sh.off[mid] <= lo) a = mid; else b = mid - 1;
-    }
-    ri = a;
-    int32_t rend = sh.off[ri + 1];
-    bool started_here = sh.off[ri] >= lo;
-    Acc<Real> acc;
-#pragma unroll
-    for (int q = 0; q < kPerLane; ++q) {
-      const int32_t i = base_rel + q;
-      if (i < lo || i >= hi) continue;
-      while (i >= rend) {
-        if (started_here)
-          fin(r0 + ri, acc.value(), a0, a1);
-        else
-          head = acc;
+        em[u] = acc.value();
+        emask |= 1u << u;
         acc = Acc<Real>();
-        ++ri;
-        while (sh.off[ri + 1] == sh.off[ri]) ++ri;  // skip empty rows
-        rend = sh.off[ri + 1];
-        started_here = true;
+        ++rr;
+      } else {
+        acc.add(sh.val[pad8(ee)]);
+        ++ee;
       }
-      acc.add(v[q]);
-    }
-    if (!started_here) {
-      head = acc;
-    } else if (rend <= hi) {
-      fin(r0 + ri, acc.value(), a0, a1);
-    } else {
-      owner_open = true;
-      own = acc;
     }
   }
-  sh.cs[tid] = head.s;
-  sh.cc[tid] = head.comp();
-  __syncthreads();
-  if (owner_open) {
-    const int32_t rend = sh.off[ri + 1];
-    const int last_lane = (rend - 1 - (A - e0)) / kPerLane;
-    for (int t2 = tid + 1; t2 <= last_lane; ++t2) own.merge(sh.cs[t2], sh.cc[t2]);
-    fin(r0 + ri, own.value(), a0, a1);
-  }
-  __syncthreads();
-}
 
-template <class Real, class Src>
-__device__ __forceinline__ void sweep_piece(int64_t p, const RowPlan& P, const Src& src, Real* __restrict__ piece_s,
-                                            Real* __restrict__ piece_c, SweepShared<Real>& sh) {
-  const unsigned lane = lane_id(), wid = threadIdx.x / kWave;
-  const uint32_t li = P.piece_row[p];
-  const uint32_t r = P.long_rows[li];
-  const int64_t local = p - P.piece_first[li];
-  const int64_t b = P.rowoff[r] + local * kPiece;
-  const int64_t e = min(b + static_cast<int64_t>(kPiece), static_cast<int64_t>(P.rowoff[r + 1]));
-  Acc<Real> a;
-  for (int64_t i = b + threadIdx.x; i < e; i += kBlock) a.add(src(i, ld_stream(&P.col[i])));
-  a = wave_reduce(a);
+  // 3. Carries: key = the row left open by this lane.
+  const int key = (d < L) ? rr : INT_MAX;
+  const Acc<Real> seg = wave_seg_scan(key, acc);
+  const Real prev_s = __shfl_up(seg.s, 1);
+  const Real prev_c = __shfl_up(seg.comp(), 1);
+  const int prev_key = __shfl_up(key, 1);
+  const int key0 = __shfl(key, 0);
+  if (lane == kWave - 1) {
+    sh.ws[wid] = seg.s;
+    sh.wc[wid] = seg.comp();
+    sh.wkey[wid] = key;
+  }
+  if (lane == 0) sh.wfirst[wid] = key;
+  // Tile tail: the carries of the row open at the tile's end (local nre).
+  Acc<Real> tail;
+  if (key == nre) tail = acc;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) {
+    const Real os = __shfl_xor(tail.s, off);
+    const Real oc = __shfl_xor(tail.comp(), off);
+    tail.merge(os, oc);
+  }
   if (lane == 0) {
-    sh.cs[wid] = a.s;
-    sh.cc[wid] = a.comp();
+    sh.ts[wid] = tail.s;
+    sh.tc[wid] = tail.comp();
+  }
+  __syncthreads();  // walk done: s_end may now be overwritten by row sums
+
+  if (emask) {
+    Acc<Real> f = first;
+    if (lane > 0 && prev_key == rstart) f.merge(prev_s, prev_c);
+    if (key0 == rstart || lane == 0) {
+      // The run of this row reaches lane 0: add earlier wavefronts whose
+      // trailing segment belongs to it.
+      for (int w = wid - 1; w >= 0 && sh.wkey[w] == rstart; --w) {
+        f.merge(sh.ws[w], sh.wc[w]);
+        if (sh.wfirst[w] != rstart) break;
+      }
+    }
+    const Real fval = f.value();
+    int k = 0;
+#pragma unroll
+    for (int u = 0; u < kUnits; ++u) {
+      if ((emask >> u) & 1u) {
+        const int r = rstart + k;
+        const Real val = (k == 0) ? fval : em[u];
+        if (r == 0 && defer0)
+          carry[T] = val;  // head of a row that began in an earlier tile
+        else
+          s_sum[r] = val;
+        ++k;
+      }
+    }
+  }
+  if (tid == 0) {
+    Acc<Real> t;
+    for (int w = 0; w < kWaves; ++w) t.merge(sh.ts[w], sh.tc[w]);
+    carry[P.ntiles + T] = t.s;
+    carry[2 * P.ntiles + T] = t.comp();
   }
   __syncthreads();
-  if (threadIdx.x == 0) {
-    Acc<Real> t;
-    for (int w = 0; w < kBlock / kWave; ++w) t.merge(sh.cs[w], sh.cc[w]);
-    piece_s[p] = t.s;
-    piece_c[p] = t.comp();
-  }
+
+  // 4. Finalise the rows that end here (coalesced).
+  if (tid < nre && (tid > 0 || !defer0)) fin(static_cast<uint32_t>(r0 + tid), s_sum[tid], pre0, a0, a1);
+  for (int j = tid + kBlock; j < nre; j += kBlock)
+    fin(static_cast<uint32_t>(r0 + j), s_sum[j], fin.pre(static_cast<uint32_t>(r0 + j)), a0, a1);
   __syncthreads();
 }
 
-// One launch: every window and every long-row piece.  Workgroup partials of
-// the two accumulators go to part0/part1[blockIdx.x].  Returns at once when
-// *skip is non-zero (a converged iteration loop).
+// One launch over every tile (persistent workgroups; tile T runs on
+// workgroup T mod gridDim.x).  Workgroup partials of the two accumulators
+// go to part0/part1[blockIdx.x].  Returns at once when *skip is non-zero (a
+// converged iteration loop).
 template <class Real, class Src, class Fin>
 __global__ void __launch_bounds__(kBlock) sweep_kernel(RowPlan P, Src src, Fin fin, const unsigned* __restrict__ skip,
                                                        double* __restrict__ part0, double* __restrict__ part1,
-                                                       Real* __restrict__ piece_s, Real* __restrict__ piece_c) {
+                                                       Real* __restrict__ carry) {
   if (skip && *skip) return;
   fin.load();
-  __shared__ SweepShared<Real> sh;
+  __shared__ TileShared<Real> sh;
   __shared__ double red[kBlock / kWave];
   double a0 = 0, a1 = 0;
-  const int64_t items = P.nwin + P.npieces;
-  for (int64_t k = blockIdx.x; k < items; k += gridDim.x) {
-    if (k < P.nwin)
-      sweep_window<Real>(k, P, src, fin, sh, a0, a1);
-    else
-      sweep_piece<Real>(k - P.nwin, P, src, piece_s, piece_c, sh);
+  int64_t T = blockIdx.x;
+  TileCoord tc{};
+  int32_t cc[kChunks][4];
+  if (T < P.ntiles) {
+    tc = tile_coord(P, T);
+    tile_cols(P, tc, cc);
+  }
+  for (; T < P.ntiles; T += gridDim.x) {
+    TileCoord nc{};
+    int32_t ncc[kChunks][4];
+    sweep_tile<Real>(T, T + gridDim.x, P, src, fin, sh, carry, tc, cc, nc, ncc, a0, a1);
+    tc = nc;
+#pragma unroll
+    for (int k = 0; k < kChunks; ++k)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) cc[k][j] = ncc[k][j];
   }
   a0 = block_sum(a0, red);
   a1 = block_sum(a1, red);
@@ -286,22 +378,26 @@ __global__ void __launch_bounds__(kBlock) sweep_kernel(RowPlan P, Src src, Fin f
   }
 }
 
-// Long rows: combine the piece partials in order and finalise.
+// Rows that span tiles: head of the tile where the row ends plus the tails
+// of the earlier tiles, latest first.
 template <class Real, class Fin>
-__global__ void __launch_bounds__(kBlock) long_finalize_kernel(RowPlan P, Fin fin, const unsigned* __restrict__ skip,
-                                                               const Real* __restrict__ piece_s,
-                                                               const Real* __restrict__ piece_c,
+__global__ void __launch_bounds__(kBlock) span_finalize_kernel(RowPlan P, Fin fin, const unsigned* __restrict__ skip,
+                                                               const Real* __restrict__ carry,
                                                                double* __restrict__ part0,
                                                                double* __restrict__ part1) {
   if (skip && *skip) return;
   fin.load();
   __shared__ double red[kBlock / kWave];
   double a0 = 0, a1 = 0;
-  for (int64_t i = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x; i < P.nlong;
-       i += static_cast<int64_t>(gridDim.x) * kBlock) {
+  for (int64_t T = blockIdx.x * static_cast<int64_t>(kBlock) + threadIdx.x; T < P.ntiles;
+       T += static_cast<int64_t>(gridDim.x) * kBlock) {
+    const int32_t r0 = P.tile_row[T];
+    if (r0 >= P.tile_row[T + 1] || P.rowoff[r0] >= P.tile_edge[T]) continue;
     Acc<Real> a;
-    for (uint32_t p = P.piece_first[i]; p < P.piece_first[i + 1]; ++p) a.merge(piece_s[p], piece_c[p]);
-    fin(P.long_rows[i], a.value(), a0, a1);
+    a.merge(carry[T], Real(0));
+    for (int64_t t2 = T - 1; t2 >= 0 && P.tile_row[t2 + 1] == r0; --t2)
+      a.merge(carry[P.ntiles + t2], carry[2 * P.ntiles + t2]);
+    fin(static_cast<uint32_t>(r0), a.value(), fin.pre(static_cast<uint32_t>(r0)), a0, a1);
   }
   a0 = block_sum(a0, red);
   a1 = block_sum(a1, red);
@@ -311,21 +407,27 @@ __global__ void __launch_bounds__(kBlock) long_finalize_kernel(RowPlan P, Fin fi
   }
 }
 
-// Host: one full sweep (window/piece launch + long-row launch) on stream s.
-// Partials: part0/part1 hold kSweepGrid + kLongGrid entries; returns the
-// number of partials written.  `timer` brackets the main launch.
+// Host: one full sweep (tile launch + spanning-row launch) on stream s.
+// part0/part1 hold kParts entries each and carry 3 * P.ntiles; returns the
+// number of partials written (at most kParts, contiguous from 0).  `timer` brackets the tile launch.
 template <class Real, class Src, class Fin>
 int launch_sweep(const RowPlan& P, const Src& src, const Fin& fin, const unsigned* skip, double* part0,
-                 double* part1, Real* piece_s, Real* piece_c, hipStream_t s, KernelTimer* timer) {
+                 double* part1, Real* carry, hipStream_t s, KernelTimer* timer) {
+  // Persistent grid: every workgroup resident at once (occupancy x CUs).
+  static const int grid = [] {
+    int per_cu = 0, dev = 0, cus = 0;
+    GX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel<Real, Src, Fin>, kBlock, 0));
+    GX_HIP(hipGetDevice(&dev));
+    GX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return std::max(1, std::min(kSweepGrid, per_cu * cus));
+  }();
   if (timer) timer->begin();
-  sweep_kernel<Real, Src, Fin><<<kSweepGrid, kBlock, 0, s>>>(P, src, fin, skip, part0, part1, piece_s, piece_c);
+  sweep_kernel<Real, Src, Fin><<<grid, kBlock, 0, s>>>(P, src, fin, skip, part0, part1, carry);
   if (timer) timer->end();
   GX_LAUNCHED("sweep_kernel");
-  if (P.nlong == 0) return kSweepGrid;
-  long_finalize_kernel<Real, Fin><<<kLongGrid, kBlock, 0, s>>>(P, fin, skip, piece_s, piece_c, part0 + kSweepGrid,
-                                                              part1 + kSweepGrid);
-  GX_LAUNCHED("long_finalize_kernel");
-  return kSweepGrid + kLongGrid;
+  span_finalize_kernel<Real, Fin><<<kFixGrid, kBlock, 0, s>>>(P, fin, skip, carry, part0 + grid, part1 + grid);
+  GX_LAUNCHED("span_finalize_kernel");
+  return grid + kFixGrid;
 }
 
 }  // namespace rs

@@ -37,7 +37,6 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr uint32_t kInf = GX_DIST_INFINITY;
-constexpr uint32_t kHub = 1024;
 constexpr int kGrid = 256 * 8;
 constexpr int kStepsPerGraph = 16;
 
@@ -78,6 +77,7 @@ struct Bufs {
   uint32_t shift;
   unsigned cap;
   uint32_t* hub_off;
+  uint32_t hub;  // frontier.hub_degree
 };
 
 __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
@@ -197,7 +197,7 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
   lq_init(qn);
   lq_init(qf);
   RelaxOp op = make_relax(B, &qn, &qf);
-  expand_queue<kBlock>(B.row, B.filt, S->filt_count, kHub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
+  expand_queue<kBlock>(B.row, B.filt, S->filt_count, B.hub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
   op.flush(true);
 }
 
@@ -358,6 +358,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.shift = shift;
   B.cap = cap;
   B.hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
+  B.hub = hub_degree();
   SsspState* host = static_cast<SsspState*>(g->host_pinned(sizeof(SsspState)));
 
   RegionTimer region(s);
@@ -372,7 +373,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
                         (const void*)B.farq[1], (const void*)B.hubq, (const void*)B.stamp, (const void*)B.farmark,
                         (const void*)B.done, (const void*)B.filt, (const void*)B.S, (const void*)B.hub_off})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
-  key = mix_key(mix_key(key, shift), cap);
+  key = mix_key(mix_key(mix_key(key, shift), cap), B.hub);
   uint32_t replays = 0;
   for (;;) {
     ktimer.begin();

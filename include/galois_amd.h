@@ -95,6 +95,21 @@ gx_status gx_init(int device);
  * events and report it in gx_stats.main_kernel_ms.  Off by default. */
 void gx_set_profiling(int on);
 
+/* Tuning knobs (process-wide; no counterpart in the reference, whose
+ * equivalents are compile-time constants such as CHUNK_SIZE/EDGE_TILE_SIZE,
+ * bfs.cpp:84-86, ConnectedComponents.cpp:125).  Set by the autotuner
+ * (galois_amd/autotune.py) or by hand; -1 restores the built-in default.
+ *   "frontier.hub_degree"  frontier vertices with more edges than this are
+ *                          expanded by the whole grid (BFS, SSSP, CC, k-core,
+ *                          BC, push PageRank); default 1024, range [64, 2^30]
+ *   "pr.hot_sources"       pull PageRank: in-edges from the first this-many
+ *                          sources of the degree order are summed in a pass
+ *                          of their own (their ranks stay L2-resident); 0 =
+ *                          one pass; default: 2 MB worth of ranks
+ * Unknown keys and out-of-range values are GX_ERR_INVALID. */
+gx_status gx_tune_set(const char* key, int64_t value);
+gx_status gx_tune_get(const char* key, int64_t* value);
+
 /* ---------------------------------------------------------- graph lifecycle
  * Replaces CSRGraph::copy_to_gpu (libgpu/src/csr_graph.cu:274-293) and the
  * skelapp ownership of hg/gg (libgpu/src/skelapp/skel.cu:45-75). */

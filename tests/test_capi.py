@@ -125,3 +125,19 @@ def test_no_device_reported_on_cpu_host():
         pytest.skip("GPU present")
     from galois_amd import _lib
     assert _lib.lib().gx_init(0) in (_lib.GX_ERR_NO_DEVICE, _lib.GX_ERR_HIP)
+
+
+def test_tuning_knobs():
+    import galois_amd as gx
+    from galois_amd._lib import GxError
+    assert gx.tune_get("pr.hot_sources") == -1
+    gx.tune_set("frontier.hub_degree", 4096)
+    assert gx.tune_get("frontier.hub_degree") == 4096
+    gx.tune_set("frontier.hub_degree", -1)
+    assert gx.tune_get("frontier.hub_degree") == -1
+    with pytest.raises(GxError):
+        gx.tune_set("frontier.hub_degree", 3)  # below the range
+    with pytest.raises(GxError):
+        gx.tune_set("no.such.knob", 1)
+    with pytest.raises(GxError):
+        gx.tune_get("no.such.knob")

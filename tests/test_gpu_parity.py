@@ -256,6 +256,44 @@ def test_rmat16_pagerank(gx, oracle, rmat16):
     assert err.sum() < 1e-6 and st32.iterations == it
 
 
+@pytest.mark.parametrize("hot", [0, 700, 20000])
+def test_pagerank_hot_cold_split(gx, oracle, rmat16, hot):
+    """The two-pass pull (sources below `hot` in degree order summed first)
+    gives the single-pass result: FP64 to 1e-11 relative, FP32 to 1e-6 L1."""
+    G, row, col, w = rmat16
+    say(f"rmat16 pagerank hot={hot}")
+    exp, it, _ = oracle.pagerank(row, col, tol=1e-9, max_iter=100, threads=8)
+    gx.tune_set("pr.hot_sources", hot)
+    try:
+        x64, st = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp64")
+        x32, st32 = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
+    finally:
+        gx.tune_set("pr.hot_sources", -1)
+    assert st.iterations == it
+    np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-11, atol=1e-17)
+    assert np.abs(x32.cpu().numpy().astype(np.float64) - exp).sum() < 1e-6
+
+
+@pytest.mark.parametrize("hub", [64, 100000])
+def test_hub_degree_knob(gx, oracle, rmat16, hub):
+    """Results do not depend on the grid-wide hub threshold."""
+    G, row, col, w = rmat16
+    s = int(np.argmax(np.diff(row)))
+    say(f"rmat16 hub_degree={hub}")
+    gx.tune_set("frontier.hub_degree", hub)
+    try:
+        lv, _ = gx.bfs(G, s)
+        d, _ = gx.sssp(G, s, 6)
+        lab, _ = gx.connected_components(G)
+        deg, _, _, _ = gx.kcore(G, 10)
+    finally:
+        gx.tune_set("frontier.hub_degree", -1)
+    np.testing.assert_array_equal(u32(lv), oracle.bfs(row, col, s, threads=8))
+    np.testing.assert_array_equal(u32(d), oracle.sssp(row, col, w, s, 6, threads=8))
+    np.testing.assert_array_equal(u32(lab), oracle.cc(row, col, threads=8))
+    np.testing.assert_array_equal(u32(deg), oracle.kcore(row, col, 10, threads=8))
+
+
 def test_grid_parity(gx, oracle):
     say("grid 256 parity")
     G = gx.Graph.grid(256)
