@@ -166,6 +166,87 @@ def kernel_table(gx, quick: bool) -> dict:
             out[f"bc_speedup_batched_vs_plain_{gname}"] = round(per_src_plain * nsrc / stb.elapsed_ms, 2)
             out[f"bc_max_rel_diff_plain_vs_batched_{gname}"] = rel
         del G
+    # Config 4b: one batch of 64 uneven graphs (RMAT scales 12-18, edge
+    # factors 8-32 from the seed), 4 seeded sources per graph, batched BC over
+    # their disjoint union vs one source per pass; checked on a subset
+    # against the oracle (FP64 Brandes).
+    rng = np.random.default_rng(SEED)
+    ngraphs = 16 if quick else 64
+    specs = [(int(rng.integers(12, 19)), int(rng.integers(8, 33))) for _ in range(ngraphs)]
+    if quick:
+        specs = [(min(sc, 14), ef) for sc, ef in specs]
+    parts = [gx.Graph.rmat(sc, ef, seed=SEED + 1000 + i, symmetric=True) for i, (sc, ef) in enumerate(specs)]
+    U, offs = gx.Graph.union(parts)
+    srcs = []
+    for i, P in enumerate(parts):
+        prow, _, _ = P.to_host()
+        srcs.extend((gx.pick_sources(prow, 4, seed=SEED + i) + offs[i]).tolist())
+    del parts
+    srcs = np.array(srcs, np.uint32)
+    gx.betweenness_centrality_batched(U, srcs[:64])
+    bat, stb = gx.betweenness_centrality_batched(U, srcs)
+    nplain = 16
+    pl, stp = gx.betweenness_centrality(U, srcs[:nplain])
+    per_src_plain = stp.elapsed_ms / nplain
+    sub, _ = gx.betweenness_centrality_batched(U, srcs[:8])
+    urow, ucol, _ = U.to_host()
+    import ctypes as _C
+    lib = _C.CDLL(str(ROOT / "oracle" / "liboracle.so"))
+    exp = np.empty(U.n, np.float64)
+    lib.oracle_bc.argtypes = [_C.c_int64, _C.c_void_p, _C.c_void_p, _C.c_void_p, _C.c_uint32, _C.c_void_p, _C.c_int]
+    assert lib.oracle_bc(U.n, urow.ctypes.data, ucol.ctypes.data, srcs[:8].ctypes.data, 8, exp.ctypes.data,
+                         int(os.environ.get("OMP_NUM_THREADS", "8"))) == 0
+    got = sub.cpu().numpy()
+    out["bc_batched_uneven64"] = {
+        "graphs": ngraphs, "sum_n": int(U.n), "sum_m": int(U.m), "sources": int(len(srcs)),
+        "ms": round(stb.elapsed_ms, 3), "TEPS_G": round(len(srcs) * U.m / ngraphs / stb.elapsed_ms / 1e6, 3),
+        "plain_ms_per_source": round(per_src_plain, 3),
+        "speedup_batched_vs_plain": round(per_src_plain * len(srcs) / stb.elapsed_ms, 2),
+        "max_rel_err_vs_oracle_8_sources": float(np.max(np.abs(got - exp) / np.maximum(np.abs(exp), 1.0)))}
+    del U, bat, pl, sub
+    # Config 5: differentiable PageRank on directed weighted RMAT-22 (FP32
+    # lognormal weights, row-normalised), loss sum(c * x), c ~ N(0,1).
+    scale5 = 16 if quick else 22
+    import torch
+    G = gx.Graph.rmat(scale5, 16, symmetric=False, weights="lognormal", seed=SEED + 5)
+    n, m = G.n, G.m
+    gen = torch.Generator(device="cuda").manual_seed(SEED)
+    c = torch.randn(n, generator=gen, device="cuda", dtype=torch.float32)
+    t = torch.full((n,), 1.0 / n, device="cuda", dtype=torch.float32)
+    res = {}
+    for prec in ("fp32", "fp64"):
+        gx.pagerank_weighted(G, None, t, tol=1e-10, max_iter=200, precision=prec)
+        x, stf = gx.pagerank_weighted(G, None, t, tol=1e-10, max_iter=200, precision=prec)
+        gw, gt, sta = gx.pagerank_adjoint(G, None, x, c, tol=1e-10, max_iter=200, precision=prec)
+        res[prec] = (x, gw, gt, stf, sta)
+    _, gw32, gt32, stf32, sta32 = res["fp32"]
+    x64, gw64, gt64, _, _ = res["fp64"]
+    a64, b64 = gw64.cpu().numpy(), gt64.cpu().numpy()
+    rel_w = float(np.linalg.norm(gw32.cpu().numpy() - a64) / np.linalg.norm(a64))
+    rel_t = float(np.linalg.norm(gt32.cpu().numpy() - b64) / np.linalg.norm(b64))
+    # Central differences (FP64, eps 1e-4) on the 16 edges of largest gradient.
+    _, _, w_host = G.to_host()
+    top = np.argsort(-np.abs(a64))[:16]
+    wt = torch.from_numpy(w_host.astype(np.float32)).cuda()
+    c64 = c.double()
+    fd = []
+    for e in top:
+        vals, ws = [], []
+        for sgn in (1.0, -1.0):
+            w2 = wt.clone()
+            w2[int(e)] += sgn * 1e-4
+            ws.append(float(w2[int(e)]))  # the float32 weight actually used
+            xe, _ = gx.pagerank_weighted(G, w2, t, tol=1e-14, max_iter=400, precision="fp64")
+            vals.append(float((xe * c64).sum()))
+        fd.append((vals[0] - vals[1]) / (ws[0] - ws[1]))
+    fd = np.array(fd)
+    out["pagerank_adjoint_rmat22"] = {
+        "n": n, "m": m, "forward_fp32_ms": round(stf32.elapsed_ms, 3), "adjoint_fp32_ms": round(sta32.elapsed_ms, 3),
+        "adjoint_over_forward": round(sta32.elapsed_ms / stf32.elapsed_ms, 3),
+        "forward_iterations": stf32.iterations, "adjoint_iterations": sta32.iterations,
+        "grad_w_rel_err_fp32_vs_fp64": rel_w, "grad_t_rel_err_fp32_vs_fp64": rel_t,
+        "fd_max_rel_err_16_edges": float(np.max(np.abs(fd - a64[top]) / np.abs(a64[top])))}
+    del G
     return out
 
 
@@ -178,6 +259,9 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="small graphs for the kernel table")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel table")
+    ap.add_argument("--pmc-pass", action="store_true",
+                    help="workload for rocprofv3 --pmc runs: 3 pull sweeps on the headline graph, then a 1 GiB "
+                         "device copy (the calibration reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -197,6 +281,17 @@ def main() -> int:
 
     def step():
         return gx.pagerank(G, alpha=0.85, tol=1e-9, max_iter=100, precision="fp32")
+
+    if args.pmc_pass:
+        gx.pagerank(G, alpha=0.85, tol=0.0, max_iter=1, precision="fp32")  # plans and split built
+        _, st = gx.pagerank(G, alpha=0.85, tol=0.0, max_iter=3, precision="fp32")
+        del G
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        sync()
+        log(f"pmc pass: {st.iterations} sweeps, then a 1 GiB copy")
+        return 0
 
     for _ in range(args.warmup):
         step()

@@ -67,7 +67,8 @@ APPS = PKG / "apps"
 BIN = PKG / "bin"
 # Lonestar app entry points: source stem -> executable name of the reference.
 APP_NAMES = {"bfs": "bfs", "sssp": "sssp", "connected_components": "connected-components", "kcore": "k-core",
-             "pagerank_pull": "pagerank-pull", "betweennesscentrality": "betweennesscentrality"}
+             "pagerank_pull": "pagerank-pull", "betweennesscentrality": "betweennesscentrality",
+             "graph_convert": "graph-convert"}
 
 
 def _build_app(src: Path) -> Path:

@@ -55,6 +55,10 @@ struct BfsState {
   unsigned long long scout;  // degree sum of the vertices discovered this level
   unsigned long long edges;  // degree sum of every discovered vertex
   unsigned long long items;
+  // Output arrays of the current call, read by the captured level kernels
+  // (so the captured graph does not depend on the caller's buffers).
+  uint32_t* level_out;
+  uint32_t* parent_out;
 };
 
 struct Bufs {
@@ -80,8 +84,10 @@ __global__ void bfs_init_kernel(uint32_t* __restrict__ level, uint32_t* __restri
   }
 }
 
-__global__ void bfs_reset_kernel(BfsState* S, uint32_t* q0, uint32_t src) {
+__global__ void bfs_reset_kernel(BfsState* S, uint32_t* q0, uint32_t src, uint32_t* level, uint32_t* parent) {
   if (threadIdx.x == 0) {
+    S->level_out = level;
+    S->parent_out = parent;
     S->count[0] = 1;
     S->count[1] = 0;
     S->hub_count = 0;
@@ -154,6 +160,8 @@ __device__ __forceinline__ void td_finish(TdOp& op, double* /*unused*/) {
 __global__ void __launch_bounds__(kBlock) td_expand_kernel(Bufs B) {
   BfsState* S = B.S;
   if (!S->active) return;
+  B.level = S->level_out;
+  B.parent = S->parent_out;
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
   TdOp op{B, &sq, S->level, 0ull};
@@ -164,6 +172,8 @@ __global__ void __launch_bounds__(kBlock) td_expand_kernel(Bufs B) {
 __global__ void __launch_bounds__(kBlock) td_hubs_kernel(Bufs B) {
   BfsState* S = B.S;
   if (!S->active || S->hub_count == 0) return;
+  B.level = S->level_out;
+  B.parent = S->parent_out;
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
   TdOp op{B, &sq, S->level, 0ull};
@@ -290,10 +300,10 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   RegionTimer region(s);
   KernelTimer ktimer(s);
   bfs_init_kernel<<<sweep_grid(n), kBlock, 0, s>>>(level, nullptr, n, src);
-  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src);
+  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src, level, nullptr);
   GX_LAUNCHED("bfs init");
   uint64_t key = 0;
-  for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.level, (const void*)B.q[0],
+  for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.q[0],
                         (const void*)B.q[1], (const void*)B.hubq, (const void*)B.S, (const void*)B.hub_off})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
   key = mix_key(mix_key(key, cap), B.hub);
@@ -363,7 +373,7 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   KernelTimer ktimer(s);
   uint32_t launches = 2;
   bfs_init_kernel<<<sweep_grid(n), kBlock, 0, s>>>(level, parent, n, src);
-  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src);
+  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src, level, parent);
   GX_LAUNCHED("bfs init");
   int32_t rsrc[2];
   read_bytes(g, rsrc, g->row + src, sizeof(rsrc));

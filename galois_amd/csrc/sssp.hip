@@ -59,6 +59,8 @@ struct SsspState {
   unsigned steps;
   unsigned long long edges;
   unsigned long long items;
+  // Output of the current call, read by the captured step kernels.
+  uint32_t* dist_out;
 };
 
 struct Bufs {
@@ -93,8 +95,9 @@ __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restri
   }
 }
 
-__global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src) {
+__global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, uint32_t* dist) {
   if (threadIdx.x == 0) {
+    S->dist_out = dist;
     S->near_count[0] = 1;
     S->near_count[1] = 0;
     S->far_count[0] = S->far_count[1] = 0;
@@ -191,6 +194,7 @@ __device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn
 
 // Relax step, part 2: expand the filtered items.
 __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
+  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kRelax) return;
   __shared__ LdsQueue<kQCap> qn, qf;
@@ -202,6 +206,7 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
 }
 
 __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
+  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kRelax || S->hub_count == 0) return;
   __shared__ LdsQueue<kQCap> qn, qf;
@@ -213,6 +218,7 @@ __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
 }
 
 __global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
+  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kRelax) return;
   __shared__ LdsQueue<kQCap> qo;
@@ -249,6 +255,7 @@ __global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
 
 // Advance step, part 1: smallest bucket among far items above the current.
 __global__ void __launch_bounds__(kBlock) far_min_kernel(Bufs B) {
+  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kAdvance) return;
   const unsigned fs = S->far_sel;
@@ -267,6 +274,7 @@ __global__ void __launch_bounds__(kBlock) far_min_kernel(Bufs B) {
 // Advance step, part 2: far items of the new bucket go to the near queue,
 // later ones are kept (one copy per vertex), settled ones are dropped.
 __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
+  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kAdvance) return;
   const uint32_t next = S->min_bucket;
@@ -365,10 +373,10 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   KernelTimer ktimer(s);
   sssp_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(dist, B.stamp, B.farmark, B.done, n, src);
   GX_LAUNCHED("sssp_init_kernel");
-  sssp_reset_kernel<<<1, 64, 0, s>>>(B.S, B.nearq[0], src);
+  sssp_reset_kernel<<<1, 64, 0, s>>>(B.S, B.nearq[0], src, dist);
   GX_LAUNCHED("sssp_reset_kernel");
   uint64_t key = 0;
-  for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w, (const void*)B.dist,
+  for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w,
                         (const void*)B.nearq[0], (const void*)B.nearq[1], (const void*)B.farq[0],
                         (const void*)B.farq[1], (const void*)B.hubq, (const void*)B.stamp, (const void*)B.farmark,
                         (const void*)B.done, (const void*)B.filt, (const void*)B.S, (const void*)B.hub_off})

@@ -523,3 +523,21 @@ def test_batched_bc_over_union_of_uneven_graphs(gx, oracle):
         exp[offs[i]:offs[i + 1]] = oracle.bc(row, col, s)
     bc, st = gx.betweenness_centrality_batched(U, np.array(srcs, np.uint32))
     _bc_close(bc.cpu().numpy(), exp)
+
+
+def test_autotuner_picks_and_caches(gx, tmp_path):
+    from galois_amd import autotune
+    say("autotune rmat14")
+    G = gx.Graph.rmat(14, 16, symmetric=True)
+    cache = tmp_path / "tune.json"
+    try:
+        c1 = autotune.tune(G, "bfs", cache=cache, source=0)
+        assert c1["frontier.hub_degree"] in autotune.CANDIDATES["frontier.hub_degree"]
+        assert gx.tune_get("frontier.hub_degree") == c1["frontier.hub_degree"]
+        c2 = autotune.tune(G, "pagerank", cache=cache, iterations=2)
+        assert c2["pr.hot_sources"] in autotune.CANDIDATES["pr.hot_sources"]
+        autotune.reset()
+        assert autotune.tune(G, "bfs", cache=cache, source=0) == c1  # from the cache
+        assert len(__import__("json").loads(cache.read_text())) == 2
+    finally:
+        autotune.reset()
