@@ -259,6 +259,10 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="small graphs for the kernel table")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-kernels", action="store_true", help="skip the per-kernel table")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="set a tuning knob before running (gx_tune_set), e.g. pr.hot_sources=0")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run galois_amd.autotune on the headline graph before the timed steps")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="workload for rocprofv3 --pmc runs: 3 pull sweeps on the headline graph, then a 1 GiB "
                          "device copy (the calibration reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
@@ -273,6 +277,9 @@ def main() -> int:
     from galois_amd import _lib
     _lib.check(_lib.lib().gx_init(0))
     torch.cuda.set_device(0)
+    for kv in args.tune:
+        key, val = kv.split("=", 1)
+        gx.tune_set(key, int(val))
 
     log(f"generating directed RMAT-{args.scale}")
     G = gx.Graph.rmat(args.scale, 16, symmetric=False, seed=SEED)
@@ -293,6 +300,12 @@ def main() -> int:
         log(f"pmc pass: {st.iterations} sweeps, then a 1 GiB copy")
         return 0
 
+    tuned = {}
+    if args.autotune:
+        from galois_amd import autotune
+        t0 = time.perf_counter()
+        tuned = autotune.tune(G, "pagerank", cache=None, iterations=3)
+        log(f"autotune: {tuned} in {time.perf_counter() - t0:.1f} s")
     for _ in range(args.warmup):
         step()
     sync()
@@ -360,7 +373,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"pagerank_pull_fp32_kahan_rmat{args.scale}_directed", "n": n, "m": m,
-                   "edge_factor": 16, "alpha": 0.85, "tol": 1e-9, "max_iter": 100, "seed": SEED},
+                   "edge_factor": 16, "alpha": 0.85, "tol": 1e-9, "max_iter": 100, "seed": SEED,
+                   "knobs": {k: gx.tune_get(k) for k in ("pr.hot_sources", "frontier.hub_degree")},
+                   "autotuned": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "kernels": kernels,

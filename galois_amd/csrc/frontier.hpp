@@ -21,8 +21,10 @@
 
 namespace gx {
 
-// Identity frontier: vertex i of [0, count).  Otherwise q[i].
-template <int BLOCK, class Op>
+// Identity frontier: vertex i of [0, count).  Otherwise q[i].  With ADMIT,
+// op.admit(v) is asked once per frontier vertex and a vertex it rejects
+// contributes no edges (a fused filter).
+template <int BLOCK, class Op, bool ADMIT = false>
 __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, const uint32_t* __restrict__ q,
                                              unsigned qcount, uint32_t hub_thresh, uint32_t* __restrict__ hubq,
                                              unsigned* hub_count, unsigned cap, unsigned* overflow, Op& op) {
@@ -38,11 +40,15 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
     bool hub = false;
     if (i < qcount) {
       v = q ? q[i] : i;
-      start = row[v];
-      deg = static_cast<uint32_t>(row[v + 1] - start);
-      if (deg > hub_thresh) {
-        hub = true;
-        deg = 0;
+      bool in = true;
+      if constexpr (ADMIT) in = op.admit(v);
+      if (in) {
+        start = row[v];
+        deg = static_cast<uint32_t>(row[v + 1] - start);
+        if (deg > hub_thresh) {
+          hub = true;
+          deg = 0;
+        }
       }
     }
     if (hubq) wave_append<uint32_t>(hub, v, hubq, hub_count, cap, overflow);
@@ -77,15 +83,14 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
 // ceil(deg(hubq[h']) / BLOCK), off[hubs] = total steps, with hubs =
 // min(*hub_count, cap) (the count runs past cap when the hub queue
 // overflowed; the caller reports that).  One workgroup; `off` holds cap + 1.
-template <int BLOCK>
-__global__ void __launch_bounds__(1024) hub_offsets_kernel(const int32_t* __restrict__ row,
-                                                           const uint32_t* __restrict__ hubq,
-                                                           const unsigned* __restrict__ hub_count, unsigned cap,
-                                                           uint32_t* __restrict__ off) {
-  __shared__ uint32_t lds[1024 / kWave + 1];
-  const unsigned hubs = min(*hub_count, cap);
+// NT = threads of the calling workgroup (all of them call it).
+template <int BLOCK, int NT>
+__device__ __forceinline__ void hub_offsets_block(const int32_t* __restrict__ row, const uint32_t* __restrict__ hubq,
+                                                  unsigned hub_count, unsigned cap, uint32_t* __restrict__ off) {
+  __shared__ uint32_t lds[NT / kWave + 1];
+  const unsigned hubs = min(hub_count, cap);
   uint32_t carry = 0;
-  for (unsigned base = 0; base < hubs; base += 1024) {
+  for (unsigned base = 0; base < hubs; base += NT) {
     const unsigned i = base + threadIdx.x;
     uint32_t c = 0;
     if (i < hubs) {
@@ -93,11 +98,19 @@ __global__ void __launch_bounds__(1024) hub_offsets_kernel(const int32_t* __rest
       c = static_cast<uint32_t>((row[v + 1] - row[v] + BLOCK - 1) / BLOCK);
     }
     uint32_t total;
-    const uint32_t ex = block_exclusive_scan<1024>(c, lds, &total);
+    const uint32_t ex = block_exclusive_scan<NT>(c, lds, &total);
     if (i < hubs) off[i] = carry + ex;
     carry += total;
   }
   if (threadIdx.x == 0) off[hubs] = carry;
+}
+
+template <int BLOCK>
+__global__ void __launch_bounds__(1024) hub_offsets_kernel(const int32_t* __restrict__ row,
+                                                           const uint32_t* __restrict__ hubq,
+                                                           const unsigned* __restrict__ hub_count, unsigned cap,
+                                                           uint32_t* __restrict__ off) {
+  hub_offsets_block<BLOCK, 1024>(row, hubq, *hub_count, cap, off);
 }
 
 // Edges of the hub vertices hubq[0..hubs), cut into BLOCK-edge steps that

@@ -267,12 +267,11 @@ static void build_split(gx_graph* g, int64_t hot) {
   D.hot_edges = ma;
 }
 
-// Sources gathered in the hot pass (knob "pr.hot_sources"; default: the top
-// ids of the degree order whose y fills 2 MB, half of one XCD's 4 MB L2).
-// 0 = single pass.
+// Sources gathered in the hot pass (knob "pr.hot_sources"; default 0 = one
+// pass, which measured faster on RMAT-26; the autotuner may pick a split).
 static int64_t hot_sources(int64_t n, size_t real_bytes) {
-  int64_t h = static_cast<int64_t>((size_t(2) << 20) / real_bytes);
-  h = tune_value("pr.hot_sources", h);
+  (void)real_bytes;
+  const int64_t h = tune_value("pr.hot_sources", 0);
   return (h <= 0 || h >= n) ? 0 : h;
 }
 

@@ -43,7 +43,6 @@ RowPlan build_plan(gx_graph* g, const int32_t* rowoff, const int32_t* col, int64
     P.tile_row = cache.buf.as<int32_t>();
     P.tile_edge = P.tile_row + (ntiles + 1);
     P.ntiles = ntiles;
-    P.vec = (reinterpret_cast<uintptr_t>(col) & 15) == 0;
     return P;
   };
   if (cache.ready) return view();

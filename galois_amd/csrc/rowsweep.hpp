@@ -9,9 +9,10 @@
 // plan stores each tile's first row and first edge.
 //
 // Tile work (one 256-lane workgroup).
-//  1. Stage: the tile's row ends go to LDS; its columns are read with 16-byte
-//     loads and every edge value is evaluated at once by all lanes (up to
-//     kUnits gathers in flight per lane) into LDS.
+//  1. Stage: the tile's row ends go to LDS; its columns (read one tile
+//     ahead, consecutive lanes on consecutive edges) are turned into edge
+//     values by all lanes at once (kItems gathers in flight per lane) and
+//     stored to LDS.
 //  2. Walk: lane t takes merge-path items [t*kUnits, (t+1)*kUnits), finds its
 //     starting (row, edge) by binary search in LDS and walks them in order,
 //     adding edge values and emitting a row sum at every row end.
@@ -54,7 +55,7 @@ constexpr int kWaves = kBlock / kWave;
 constexpr int kSweepGrid = 256 * 8;        // persistent workgroups of the tile launch
 constexpr int kFixGrid = 256;              // workgroups of the spanning-row launch
 constexpr int kParts = kSweepGrid + kFixGrid;
-constexpr int kChunks = (kTile / 4 + 1 + kBlock - 1) / kBlock;  // 16-byte column loads per lane
+constexpr int kItems = kTile / kBlock;    // edges per lane at most (a tile has <= kTile edges)
 
 template <class Real>
 struct Acc;
@@ -114,7 +115,6 @@ struct RowPlan {
   int32_t* tile_row;      // ntiles+1: row of each tile's first item
   int32_t* tile_edge;     // ntiles+1: edge of each tile's first item
   int64_t ntiles;
-  bool vec;               // col is 16-byte aligned
 };
 
 // Builds (or returns the cached) plan of a row-offset array; defined in
@@ -164,25 +164,17 @@ __device__ __forceinline__ TileCoord tile_coord(const RowPlan& P, int64_t T) {
   return c;
 }
 
-// Columns of a tile, 16-byte loads from the 4-aligned edge below e0.
-__device__ __forceinline__ void tile_cols(const RowPlan& P, const TileCoord& tc, int32_t (&cc)[kChunks][4]) {
+// Columns of a tile: edge e0 + tid + i*kBlock in cc[i].  Consecutive lanes
+// hold consecutive edges, so the gathers of one wavefront instruction hit
+// runs of neighbouring sources (rows in degree order have ascending, and for
+// heavy rows dense, source ids) that the texture unit coalesces into few
+// cache lines.
+__device__ __forceinline__ void tile_cols(const RowPlan& P, const TileCoord& tc, int32_t (&cc)[kItems]) {
   const int tid = threadIdx.x;
-  const int32_t A = tc.e0 & ~3;
-  const int nch = (tc.e1 - A + 3) >> 2;
 #pragma unroll
-  for (int k = 0; k < kChunks; ++k) {
-    const int ch = tid + k * kBlock;
-    const int64_t p = static_cast<int64_t>(A) + 4 * ch;
-    if (P.vec && ch < nch && p + 4 <= P.len) {
-      const v4i c = ld_stream(reinterpret_cast<const v4i*>(P.col + p));
-      cc[k][0] = c.x;
-      cc[k][1] = c.y;
-      cc[k][2] = c.z;
-      cc[k][3] = c.w;
-    } else {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) cc[k][j] = (ch < nch && p + j < P.len) ? P.col[p + j] : 0;
-    }
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t q = static_cast<int64_t>(tc.e0) + tid + i * kBlock;
+    cc[i] = q < tc.e1 ? ld_stream(&P.col[q]) : 0;
   }
 }
 
@@ -192,8 +184,8 @@ __device__ __forceinline__ void tile_cols(const RowPlan& P, const TileCoord& tc,
 template <class Real, class Src, class Fin>
 __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan& P, const Src& src, const Fin& fin,
                                            TileShared<Real>& sh, Real* __restrict__ carry, const TileCoord& tc,
-                                           const int32_t (&cc)[kChunks][4], TileCoord& ncoord,
-                                           int32_t (&ncc)[kChunks][4], double& a0, double& a1) {
+                                           const int32_t (&cc)[kItems], TileCoord& ncoord,
+                                           int32_t (&ncc)[kItems], double& a0, double& a1) {
   const int tid = threadIdx.x;
   const int lane = static_cast<int>(lane_id());
   const int wid = tid / kWave;
@@ -204,18 +196,15 @@ __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan&
   const bool defer0 = tc.defer0;
   int32_t* s_end = reinterpret_cast<int32_t*>(sh.rows);
   Real* s_sum = reinterpret_cast<Real*>(sh.rows);
-  const int32_t A = e0 & ~3;
 
   // 1. Issue every load of the tile at once: gathers, row ends, the first
   // finalisation operands, and the next tile's coordinates and columns.
-  Real v[kChunks][4];
+  Real v[kItems];
 #pragma unroll
-  for (int k = 0; k < kChunks; ++k)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t q = static_cast<int64_t>(A) + 4 * (tid + k * kBlock) + j;
-      v[k][j] = (q >= e0 && q < e1) ? src(q, cc[k][j]) : Real(0);
-    }
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t q = static_cast<int64_t>(e0) + tid + i * kBlock;
+    v[i] = q < e1 ? src(q, cc[i]) : Real(0);
+  }
   int32_t rend0 = 0;
   if (tid < nre) rend0 = ld_stream(&P.rowoff[r0 + 1 + tid]);
   typename Fin::Pre pre0{};
@@ -227,12 +216,10 @@ __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan&
   if (tid < nre) s_end[tid] = rend0 - e0;
   for (int j = tid + kBlock; j < nre; j += kBlock) s_end[j] = ld_stream(&P.rowoff[r0 + 1 + j]) - e0;
 #pragma unroll
-  for (int k = 0; k < kChunks; ++k)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t q = static_cast<int64_t>(A) + 4 * (tid + k * kBlock) + j;
-      if (q >= e0 && q < e1) sh.val[pad8(static_cast<int>(q - e0))] = v[k][j];
-    }
+  for (int i = 0; i < kItems; ++i) {
+    const int j = tid + i * kBlock;
+    if (j < ned) sh.val[pad8(j)] = v[i];
+  }
   __syncthreads();
 
   // 2. Walk this lane's merge-path items.
@@ -355,20 +342,18 @@ __global__ void __launch_bounds__(kBlock) sweep_kernel(RowPlan P, Src src, Fin f
   double a0 = 0, a1 = 0;
   int64_t T = blockIdx.x;
   TileCoord tc{};
-  int32_t cc[kChunks][4];
+  int32_t cc[kItems];
   if (T < P.ntiles) {
     tc = tile_coord(P, T);
     tile_cols(P, tc, cc);
   }
   for (; T < P.ntiles; T += gridDim.x) {
     TileCoord nc{};
-    int32_t ncc[kChunks][4];
+    int32_t ncc[kItems];
     sweep_tile<Real>(T, T + gridDim.x, P, src, fin, sh, carry, tc, cc, nc, ncc, a0, a1);
     tc = nc;
 #pragma unroll
-    for (int k = 0; k < kChunks; ++k)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) cc[k][j] = ncc[k][j];
+    for (int i = 0; i < kItems; ++i) cc[i] = ncc[i];
   }
   a0 = block_sum(a0, red);
   a1 = block_sum(a1, red);

@@ -105,7 +105,7 @@ void gx_set_profiling(int on);
  *   "pr.hot_sources"       pull PageRank: in-edges from the first this-many
  *                          sources of the degree order are summed in a pass
  *                          of their own (their ranks stay L2-resident); 0 =
- *                          one pass; default: 2 MB worth of ranks
+ *                          one pass (the default)
  * Unknown keys and out-of-range values are GX_ERR_INVALID. */
 gx_status gx_tune_set(const char* key, int64_t value);
 gx_status gx_tune_get(const char* key, int64_t* value);
