@@ -46,25 +46,30 @@ __global__ void kcore_reset_kernel(KcoreCounters* c) {
   }
 }
 
-// degree[v] = out-degree; vertices below k form the first dead frontier.
+constexpr int kQCap = 2048;
+
+// degree[v] = out-degree; vertices below k form the first dead frontier
+// (staged in LDS, one global append per workgroup flush).
 __global__ void __launch_bounds__(kBlock) kcore_init_kernel(const int32_t* __restrict__ row,
                                                             uint32_t* __restrict__ degree, int64_t n, uint32_t k,
                                                             uint32_t* __restrict__ q, KcoreCounters* c,
                                                             unsigned cap) {
-  int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
-  int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
-  for (; v - threadIdx.x % kWave < n; v += stride) {
+  __shared__ LdsQueue<kQCap> sq;
+  lq_init(sq);
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x); b < n; b += stride) {
+    const int64_t v = b + threadIdx.x;
     bool dead = false;
     if (v < n) {
       const uint32_t d = static_cast<uint32_t>(row[v + 1] - row[v]);
       degree[v] = d;
       dead = d < k;
     }
-    wave_append<uint32_t>(dead, static_cast<uint32_t>(v), q, &c->count[0], cap, &c->overflow);
+    lq_push(sq, dead, static_cast<uint32_t>(v));
+    lq_flush(sq, q, &c->count[0], cap, &c->overflow, false);
   }
+  lq_flush(sq, q, &c->count[0], cap, &c->overflow, true);
 }
-
-constexpr int kQCap = 2048;
 
 struct PeelOp {
   const int32_t* col;

@@ -20,11 +20,9 @@
 //    near frontier drains, one pass finds the smallest live far bucket and a
 //    second pass moves that bucket into the near frontier and keeps the rest.
 //  - The loop is driven on the device: every step is the same fixed kernel
-//    sequence (begin, relax, hubs, far-min, far-split) whose kernels read the
-//    step's mode from a device state word and return at once when it is not
-//    theirs.  The stale-item filter is fused into the relax expansion, the
-//    last relax workgroup lays out the hub steps, and each begin closes the
-//    previous step, so a step costs five launches.  Steps are captured once into a HIP graph
+//    sequence (begin, filter, relax, hubs, far-min, far-split, end) whose
+//    kernels read the step's mode from a device state word and return at
+//    once when it is not theirs.  Steps are captured once into a HIP graph
 //    (kStepsPerGraph steps) and replayed until the state says done, so the
 //    host synchronises once per graph replay instead of once per round.
 #include <algorithm>
@@ -47,6 +45,7 @@ enum Mode : unsigned { kIdle = 0, kRelax = 1, kAdvance = 2 };
 struct SsspState {
   unsigned near_count[2];  // near queues (double buffered by q)
   unsigned far_count[2];   // far piles (double buffered by far_sel)
+  unsigned filt_count;     // filtered near items of the current round
   unsigned hub_count;
   unsigned overflow;
   unsigned min_bucket;
@@ -58,7 +57,6 @@ struct SsspState {
   unsigned round;
   unsigned phase;
   unsigned steps;
-  unsigned relax_done;      // relax workgroups finished this step
   unsigned long long edges;
   unsigned long long items;
   // Output of the current call, read by the captured step kernels.
@@ -75,6 +73,7 @@ struct Bufs {
   uint32_t* done;
   uint32_t* nearq[2];
   uint32_t* farq[2];
+  uint32_t* filt;
   uint32_t* hubq;
   SsspState* S;
   uint32_t shift;
@@ -102,6 +101,7 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
     S->near_count[0] = 1;
     S->near_count[1] = 0;
     S->far_count[0] = S->far_count[1] = 0;
+    S->filt_count = 0;
     S->hub_count = 0;
     S->overflow = 0;
     S->min_bucket = 0xffffffffu;
@@ -119,36 +119,16 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
   }
 }
 
-// Closes the previous step (queue flip, or the new bucket after an advance);
-// a no-op when no step is open.
-__device__ __forceinline__ void step_close(SsspState* S) {
-  if (S->mode == kRelax) {
-    S->q ^= 1;
-  } else if (S->mode == kAdvance) {
-    if (S->min_bucket == 0xffffffffu) {
-      S->done = 1;  // every far item is settled
-    } else {
-      S->bucket = S->min_bucket;
-      S->far_count[S->far_sel] = 0;
-      S->far_sel ^= 1;
-      S->q ^= 1;
-    }
-  }
-  S->mode = kIdle;
-}
-
-// Closes the previous step, then chooses this step's mode and clears the
-// counters the step appends to.
+// Chooses the step's mode and clears the counters the step appends to.
 __global__ void step_begin_kernel(SsspState* S) {
   if (threadIdx.x != 0) return;
-  step_close(S);
   if (S->done || S->overflow) {
     S->mode = kIdle;
     return;
   }
   S->round += 1;
+  S->filt_count = 0;
   S->hub_count = 0;
-  S->relax_done = 0;
   S->near_count[S->q ^ 1] = 0;
   if (S->near_count[S->q] > 0) {
     S->mode = kRelax;
@@ -164,9 +144,9 @@ __global__ void step_begin_kernel(SsspState* S) {
   S->steps += 1;
 }
 
-__global__ void step_end_kernel(SsspState* S) {
-  if (threadIdx.x == 0) step_close(S);
-}
+// Relax step, part 1: drop near items whose distance has not changed since
+// their last expansion; mark the rest expanded at their current distance.
+__global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B);
 
 constexpr int kQCap = 2048;
 
@@ -179,17 +159,6 @@ struct RelaxOp {
   uint32_t* farq;
   unsigned* far_count;
   uint32_t bucket, round, phase;
-  unsigned long long edges, items;  // this lane's expanded degree sum and items
-  // Near items whose distance has not changed since their last expansion are
-  // dropped; the rest are marked expanded at their current distance.
-  __device__ __forceinline__ bool admit(uint32_t u) {
-    const uint32_t d = B.dist[u];
-    if (B.done[u] == d) return false;
-    B.done[u] = d;
-    edges += static_cast<unsigned long long>(B.row[u + 1] - B.row[u]);
-    items += 1;
-    return true;
-  }
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     bool to_near = false, to_far = false;
     uint32_t v = 0;
@@ -219,40 +188,21 @@ struct RelaxOp {
 __device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn, LdsQueue<kQCap>* qf) {
   SsspState* S = B.S;
   const unsigned q = S->q, fs = S->far_sel;
-  return RelaxOp{B,        qn,       qf,       B.nearq[q ^ 1], &S->near_count[q ^ 1], B.farq[fs], &S->far_count[fs],
-                 S->bucket, S->round, S->phase, 0ull,           0ull};
+  return RelaxOp{B, qn, qf, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.farq[fs], &S->far_count[fs], S->bucket,
+                 S->round, S->phase};
 }
 
-// Relax step: expand the near items that pass the filter; the last
-// workgroup to finish lays out the hub steps for hubs_kernel.
+// Relax step, part 2: expand the filtered items.
 __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
   B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kRelax) return;
   __shared__ LdsQueue<kQCap> qn, qf;
-  __shared__ unsigned last;
   lq_init(qn);
   lq_init(qf);
   RelaxOp op = make_relax(B, &qn, &qf);
-  const unsigned q = S->q;
-  expand_queue<kBlock, RelaxOp, true>(B.row, B.nearq[q], S->near_count[q], B.hub, B.hubq, &S->hub_count, B.cap,
-                                      &S->overflow, op);
+  expand_queue<kBlock>(B.row, B.filt, S->filt_count, B.hub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
   op.flush(true);
-  const unsigned long long edges = wave_sum(op.edges), items = wave_sum(op.items);
-  if (lane_id() == 0 && items) {
-    atomicAdd(&S->edges, edges);
-    atomicAdd(&S->items, items);
-  }
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) last = atomicAdd(&S->relax_done, 1u) == gridDim.x - 1;
-  __syncthreads();
-  if (last) {
-    __threadfence();
-    hub_offsets_block<kBlock, kBlock>(B.row, B.hubq, __hip_atomic_load(&S->hub_count, __ATOMIC_RELAXED,
-                                                                       __HIP_MEMORY_SCOPE_AGENT),
-                                      B.cap, B.hub_off);
-  }
 }
 
 __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
@@ -265,6 +215,42 @@ __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
   RelaxOp op = make_relax(B, &qn, &qf);
   expand_hubs<kBlock>(B.row, B.hubq, S->hub_count, B.cap, B.hub_off, op);
   op.flush(true);
+}
+
+__global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
+  B.dist = B.S->dist_out;
+  SsspState* S = B.S;
+  if (S->mode != kRelax) return;
+  __shared__ LdsQueue<kQCap> qo;
+  lq_init(qo);
+  const unsigned q = S->q;
+  const unsigned cnt = S->near_count[q];
+  const uint32_t* in = B.nearq[q];
+  unsigned long long edges = 0, items = 0;
+  for (unsigned b = blockIdx.x * kBlock; b < cnt; b += gridDim.x * kBlock) {
+    const unsigned i = b + threadIdx.x;
+    bool keep = false;
+    uint32_t u = 0;
+    if (i < cnt) {
+      u = in[i];
+      const uint32_t d = B.dist[u];
+      keep = B.done[u] != d;
+      if (keep) {
+        B.done[u] = d;
+        edges += static_cast<unsigned long long>(B.row[u + 1] - B.row[u]);
+        items += 1;
+      }
+    }
+    lq_push(qo, keep, u);
+    lq_flush(qo, B.filt, &S->filt_count, B.cap, &S->overflow, false);
+  }
+  lq_flush(qo, B.filt, &S->filt_count, B.cap, &S->overflow, true);
+  edges = wave_sum(edges);
+  items = wave_sum(items);
+  if (lane_id() == 0 && items) {
+    atomicAdd(&S->edges, edges);
+    atomicAdd(&S->items, items);
+  }
 }
 
 // Advance step, part 1: smallest bucket among far items above the current.
@@ -321,12 +307,32 @@ __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
   lq_flush(qk, B.farq[fs ^ 1], &S->far_count[fs ^ 1], B.cap, &S->overflow, true);
 }
 
+__global__ void step_end_kernel(SsspState* S) {
+  if (threadIdx.x != 0) return;
+  if (S->mode == kRelax) {
+    S->q ^= 1;
+  } else if (S->mode == kAdvance) {
+    if (S->min_bucket == 0xffffffffu) {
+      S->done = 1;  // every far item is settled
+    } else {
+      S->bucket = S->min_bucket;
+      S->far_count[S->far_sel] = 0;
+      S->far_sel ^= 1;
+      S->q ^= 1;
+    }
+  }
+  S->mode = kIdle;
+}
+
 void enqueue_step(const Bufs& B, hipStream_t s) {
   step_begin_kernel<<<1, 64, 0, s>>>(B.S);
+  filter_kernel<<<kGrid, kBlock, 0, s>>>(B);
   relax_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off);
   hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
   far_min_kernel<<<kGrid, kBlock, 0, s>>>(B);
   far_split_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  step_end_kernel<<<1, 64, 0, s>>>(B.S);
   GX_LAUNCHED("sssp step");
 }
 
@@ -355,6 +361,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.stamp = g->scratch<uint32_t>(WS_V1, cap);
   B.farmark = g->scratch<uint32_t>(WS_V2, cap);
   B.done = g->scratch<uint32_t>(WS_V3, cap);
+  B.filt = g->scratch<uint32_t>(WS_V4, cap);
   B.S = g->scratch<SsspState>(WS_COUNTERS, 1);
   B.shift = shift;
   B.cap = cap;
@@ -372,7 +379,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w,
                         (const void*)B.nearq[0], (const void*)B.nearq[1], (const void*)B.farq[0],
                         (const void*)B.farq[1], (const void*)B.hubq, (const void*)B.stamp, (const void*)B.farmark,
-                        (const void*)B.done, (const void*)B.S, (const void*)B.hub_off})
+                        (const void*)B.done, (const void*)B.filt, (const void*)B.S, (const void*)B.hub_off})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
   key = mix_key(mix_key(mix_key(key, shift), cap), B.hub);
   uint32_t replays = 0;
@@ -380,7 +387,6 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
     ktimer.begin();
     g->replay(g->sssp_batch, key, [&](hipStream_t cs) {
       for (int i = 0; i < kStepsPerGraph; ++i) enqueue_step(B, cs);
-      step_end_kernel<<<1, 64, 0, cs>>>(B.S);
     });
     ktimer.end();
     ++replays;
@@ -393,7 +399,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = host->steps;
-    st->launches = 2 + replays * (kStepsPerGraph * 5 + 1);
+    st->launches = 2 + replays * kStepsPerGraph * 8;
     st->edges_visited = host->edges;
     st->items = host->items;
     st->elapsed_ms = ms;
