@@ -65,6 +65,8 @@ template <class Real>
 struct PrSrc {
   const Real* __restrict__ y;
   __device__ __forceinline__ Real operator()(int64_t, int32_t c) const { return y[c]; }
+  __device__ __forceinline__ Real load(int32_t c) const { return y[c]; }
+  __device__ __forceinline__ Real apply(int64_t, Real v) const { return v; }
 };
 
 // x'[r] = base + alpha*sum; residual |x'-x| and dangling mass in FP64.
@@ -329,12 +331,16 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
       PrFin<Real> fin{state, x[cur], x[nxt], y[nxt], outdeg, part, Real(0), Real(0)};
       ktimer.begin();
       if (hot > 0) {
-        rs::launch_sweep<Real>(PA, src, PartFin<Real>{part}, &state->converged, partsA, partsA + rs::kParts, carryA,
-                               s, nullptr);
+        rs::launch_sweep<Real, rs::HotCache<Real>::kHot>(PA, src, PartFin<Real>{part}, &state->converged, partsA,
+                                                         partsA + rs::kParts, carryA, s, nullptr);
         launches += 2;
       }
-      const int np = rs::launch_sweep<Real>(P, src, fin, &state->converged, part_resid, part_dang, carry, s,
-                                           nullptr);
+      // Degree order puts the most-gathered sources first: cache them in LDS
+      // (only the single pass / hot pass; the cold pass has no such prefix).
+      const int np = hot > 0 ? rs::launch_sweep<Real>(P, src, fin, &state->converged, part_resid, part_dang,
+                                                      carry, s, nullptr)
+                             : rs::launch_sweep<Real, rs::HotCache<Real>::kHot>(
+                                   P, src, fin, &state->converged, part_resid, part_dang, carry, s, nullptr);
       ktimer.end();
       pr_reduce_kernel<<<1, kBlock, 0, s>>>(state, part_resid, part_dang, np, static_cast<unsigned>(nxt));
       GX_LAUNCHED("pr_reduce_kernel");

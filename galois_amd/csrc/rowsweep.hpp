@@ -8,7 +8,7 @@
 // (empty rows, hubs and everything between cost the same per item).  The
 // plan stores each tile's first row and first edge.
 //
-// Tile work (one 256-lane workgroup).
+// Tile work (one 512-lane workgroup, two resident per CU).
 //  1. Stage: the tile's row ends go to LDS; its columns (read one tile
 //     ahead, consecutive lanes on consecutive edges) are turned into edge
 //     values by all lanes at once (kItems gathers in flight per lane) and
@@ -48,14 +48,21 @@
 namespace gx {
 namespace rs {
 
-constexpr int kBlock = 256;
-constexpr int kUnits = 8;                  // merge-path items per lane
-constexpr int kTile = kBlock * kUnits;     // items (row ends + edges) per tile
-constexpr int kWaves = kBlock / kWave;
-constexpr int kSweepGrid = 256 * 8;        // persistent workgroups of the tile launch
-constexpr int kFixGrid = 256;              // workgroups of the spanning-row launch
+constexpr int kBlock = 256;                 // generic kernels (reductions, spanning rows)
+constexpr int kSweepBlock = 512;            // tile workgroup: 8 wavefronts, two per CU with the LDS cache
+constexpr int kUnits = 8;                   // merge-path items per lane
+constexpr int kTile = kSweepBlock * kUnits; // items (row ends + edges) per tile
+constexpr int kWaves = kSweepBlock / kWave;
+constexpr int kSweepGrid = 256 * 8;         // upper bound of the persistent tile launch
+constexpr int kFixGrid = 256;               // workgroups of the spanning-row launch
 constexpr int kParts = kSweepGrid + kFixGrid;
-constexpr int kItems = kTile / kBlock;    // edges per lane at most (a tile has <= kTile edges)
+constexpr int kItems = kTile / kSweepBlock; // edges per lane at most (a tile has <= kTile edges)
+// LDS cache of the gathered vector for sources [0, kHot) (the hottest ones
+// in degree order), sized to what the tile leaves of the 160 KB of LDS.
+template <class Real>
+struct HotCache {
+  static constexpr int kHot = sizeof(Real) == 4 ? 11264 : 1024;
+};
 
 template <class Real>
 struct Acc;
@@ -88,7 +95,9 @@ struct Acc<double> {
   __device__ __forceinline__ double comp() const { return 0.0; }
 };
 
-// Workgroup sum of one double per lane in a fixed order (valid in lane 0).
+// Workgroup sum of one double per lane in a fixed order (valid in lane 0);
+// NT = threads of the workgroup, lds holds NT / 64 doubles.
+template <int NT = kBlock>
 __device__ __forceinline__ double block_sum(double v, double* lds) {
   v = wave_sum(v);
   const unsigned w = threadIdx.x / kWave;
@@ -96,7 +105,7 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   __syncthreads();
   double r = 0;
   if (threadIdx.x == 0)
-    for (int i = 0; i < kBlock / kWave; ++i) r += lds[i];
+    for (int i = 0; i < NT / kWave; ++i) r += lds[i];
   __syncthreads();
   return r;
 }
@@ -121,13 +130,13 @@ struct RowPlan {
 // rowsweep.hip.
 RowPlan build_plan(gx_graph* g, const int32_t* rowoff, const int32_t* col, int64_t n, int64_t m, PullPlan& cache);
 
-// LDS index with one pad word per 8: lanes walk items 8 apart, and the pad
-// spreads them over the banks.
-__device__ __forceinline__ int pad8(int j) { return j + (j >> 3); }
+// LDS index with one pad word per kUnits: lanes walk items kUnits apart, and
+// the pad spreads them over the banks.
+__device__ __forceinline__ int pad8(int j) { return j + j / kUnits; }
 
 template <class Real>
 struct TileShared {
-  Real val[kTile + kTile / 8];               // edge values (padded index)
+  Real val[kTile + kTile / kUnits];          // edge values (padded index)
   alignas(16) char rows[kTile * sizeof(Real)];  // relative row ends, then row sums
   Real ws[kWaves], wc[kWaves];               // per-wave tail segment of the carries
   int wkey[kWaves], wfirst[kWaves];          // its key; key of the wave's lane 0
@@ -164,7 +173,7 @@ __device__ __forceinline__ TileCoord tile_coord(const RowPlan& P, int64_t T) {
   return c;
 }
 
-// Columns of a tile: edge e0 + tid + i*kBlock in cc[i].  Consecutive lanes
+// Columns of a tile: edge e0 + tid + i*kSweepBlock in cc[i].  Consecutive lanes
 // hold consecutive edges, so the gathers of one wavefront instruction hit
 // runs of neighbouring sources (rows in degree order have ascending, and for
 // heavy rows dense, source ids) that the texture unit coalesces into few
@@ -173,7 +182,7 @@ __device__ __forceinline__ void tile_cols(const RowPlan& P, const TileCoord& tc,
   const int tid = threadIdx.x;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const int64_t q = static_cast<int64_t>(tc.e0) + tid + i * kBlock;
+    const int64_t q = static_cast<int64_t>(tc.e0) + tid + i * kSweepBlock;
     cc[i] = q < tc.e1 ? ld_stream(&P.col[q]) : 0;
   }
 }
@@ -181,9 +190,10 @@ __device__ __forceinline__ void tile_cols(const RowPlan& P, const TileCoord& tc,
 // One tile (see the file comment).  `cc` holds the tile's columns; the next
 // tile's coordinates and columns are loaded into `ncoord`/`ncc` while this
 // one is processed.
-template <class Real, class Src, class Fin>
+template <class Real, int HOT, class Src, class Fin>
 __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan& P, const Src& src, const Fin& fin,
-                                           TileShared<Real>& sh, Real* __restrict__ carry, const TileCoord& tc,
+                                           TileShared<Real>& sh, const Real* hot, Real* __restrict__ carry,
+                                           const TileCoord& tc,
                                            const int32_t (&cc)[kItems], TileCoord& ncoord,
                                            int32_t (&ncc)[kItems], double& a0, double& a1) {
   const int tid = threadIdx.x;
@@ -202,8 +212,11 @@ __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan&
   Real v[kItems];
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const int64_t q = static_cast<int64_t>(e0) + tid + i * kBlock;
-    v[i] = q < e1 ? src(q, cc[i]) : Real(0);
+    const int64_t q = static_cast<int64_t>(e0) + tid + i * kSweepBlock;
+    if constexpr (HOT > 0)
+      v[i] = q < e1 ? src.apply(q, cc[i] < HOT ? hot[cc[i]] : src.load(cc[i])) : Real(0);
+    else
+      v[i] = q < e1 ? src(q, cc[i]) : Real(0);
   }
   int32_t rend0 = 0;
   if (tid < nre) rend0 = ld_stream(&P.rowoff[r0 + 1 + tid]);
@@ -214,10 +227,10 @@ __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan&
     tile_cols(P, ncoord, ncc);
   }
   if (tid < nre) s_end[tid] = rend0 - e0;
-  for (int j = tid + kBlock; j < nre; j += kBlock) s_end[j] = ld_stream(&P.rowoff[r0 + 1 + j]) - e0;
+  for (int j = tid + kSweepBlock; j < nre; j += kSweepBlock) s_end[j] = ld_stream(&P.rowoff[r0 + 1 + j]) - e0;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
-    const int j = tid + i * kBlock;
+    const int j = tid + i * kSweepBlock;
     if (j < ned) sh.val[pad8(j)] = v[i];
   }
   __syncthreads();
@@ -322,7 +335,7 @@ __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan&
 
   // 4. Finalise the rows that end here (coalesced).
   if (tid < nre && (tid > 0 || !defer0)) fin(static_cast<uint32_t>(r0 + tid), s_sum[tid], pre0, a0, a1);
-  for (int j = tid + kBlock; j < nre; j += kBlock)
+  for (int j = tid + kSweepBlock; j < nre; j += kSweepBlock)
     fin(static_cast<uint32_t>(r0 + j), s_sum[j], fin.pre(static_cast<uint32_t>(r0 + j)), a0, a1);
   __syncthreads();
 }
@@ -330,15 +343,24 @@ __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan&
 // One launch over every tile (persistent workgroups; tile T runs on
 // workgroup T mod gridDim.x).  Workgroup partials of the two accumulators
 // go to part0/part1[blockIdx.x].  Returns at once when *skip is non-zero (a
-// converged iteration loop).
-template <class Real, class Src, class Fin>
-__global__ void __launch_bounds__(kBlock) sweep_kernel(RowPlan P, Src src, Fin fin, const unsigned* __restrict__ skip,
-                                                       double* __restrict__ part0, double* __restrict__ part1,
-                                                       Real* __restrict__ carry) {
+// converged iteration loop).  With HOT > 0 the gathered vector of sources
+// [0, HOT) is first copied to LDS (src.load(c) is the gathered value,
+// src.apply(e, value) the edge's term).
+template <class Real, int HOT, class Src, class Fin>
+__global__ void __launch_bounds__(kSweepBlock) sweep_kernel(RowPlan P, Src src, Fin fin,
+                                                            const unsigned* __restrict__ skip,
+                                                            double* __restrict__ part0, double* __restrict__ part1,
+                                                            Real* __restrict__ carry) {
   if (skip && *skip) return;
   fin.load();
   __shared__ TileShared<Real> sh;
-  __shared__ double red[kBlock / kWave];
+  __shared__ double red[kWaves];
+  __shared__ Real hot[HOT > 0 ? HOT : 1];
+  if constexpr (HOT > 0) {
+    const int64_t nh = P.n < HOT ? P.n : HOT;
+    for (int64_t i = threadIdx.x; i < nh; i += kSweepBlock) hot[i] = src.load(static_cast<int32_t>(i));
+    __syncthreads();
+  }
   double a0 = 0, a1 = 0;
   int64_t T = blockIdx.x;
   TileCoord tc{};
@@ -350,13 +372,13 @@ __global__ void __launch_bounds__(kBlock) sweep_kernel(RowPlan P, Src src, Fin f
   for (; T < P.ntiles; T += gridDim.x) {
     TileCoord nc{};
     int32_t ncc[kItems];
-    sweep_tile<Real>(T, T + gridDim.x, P, src, fin, sh, carry, tc, cc, nc, ncc, a0, a1);
+    sweep_tile<Real, HOT>(T, T + gridDim.x, P, src, fin, sh, hot, carry, tc, cc, nc, ncc, a0, a1);
     tc = nc;
 #pragma unroll
     for (int i = 0; i < kItems; ++i) cc[i] = ncc[i];
   }
-  a0 = block_sum(a0, red);
-  a1 = block_sum(a1, red);
+  a0 = block_sum<kSweepBlock>(a0, red);
+  a1 = block_sum<kSweepBlock>(a1, red);
   if (threadIdx.x == 0) {
     part0[blockIdx.x] = a0;
     part1[blockIdx.x] = a1;
@@ -394,20 +416,24 @@ __global__ void __launch_bounds__(kBlock) span_finalize_kernel(RowPlan P, Fin fi
 
 // Host: one full sweep (tile launch + spanning-row launch) on stream s.
 // part0/part1 hold kParts entries each and carry 3 * P.ntiles; returns the
-// number of partials written (at most kParts, contiguous from 0).  `timer` brackets the tile launch.
-template <class Real, class Src, class Fin>
+// number of partials written (at most kParts, contiguous from 0).  `timer`
+// brackets the tile launch.  HOT > 0 caches the gathered values of sources
+// [0, HOT) in LDS (only worth it when the lowest ids are the most gathered,
+// as in the degree-ordered PageRank pull).
+template <class Real, int HOT = 0, class Src, class Fin>
 int launch_sweep(const RowPlan& P, const Src& src, const Fin& fin, const unsigned* skip, double* part0,
                  double* part1, Real* carry, hipStream_t s, KernelTimer* timer) {
   // Persistent grid: every workgroup resident at once (occupancy x CUs).
   static const int grid = [] {
     int per_cu = 0, dev = 0, cus = 0;
-    GX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel<Real, Src, Fin>, kBlock, 0));
+    GX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sweep_kernel<Real, HOT, Src, Fin>, kSweepBlock,
+                                                        0));
     GX_HIP(hipGetDevice(&dev));
     GX_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
     return std::max(1, std::min(kSweepGrid, per_cu * cus));
   }();
   if (timer) timer->begin();
-  sweep_kernel<Real, Src, Fin><<<grid, kBlock, 0, s>>>(P, src, fin, skip, part0, part1, carry);
+  sweep_kernel<Real, HOT, Src, Fin><<<grid, kSweepBlock, 0, s>>>(P, src, fin, skip, part0, part1, carry);
   if (timer) timer->end();
   GX_LAUNCHED("sweep_kernel");
   span_finalize_kernel<Real, Fin><<<kFixGrid, kBlock, 0, s>>>(P, fin, skip, carry, part0 + grid, part1 + grid);

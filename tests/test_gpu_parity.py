@@ -253,7 +253,9 @@ def test_rmat16_pagerank(gx, oracle, rmat16):
     np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-11, atol=1e-17)
     x32, st32 = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
     err = np.abs(x32.cpu().numpy().astype(np.float64) - exp)
-    assert err.sum() < 1e-6 and st32.iterations == it
+    # FP32 stops when its own residual (rounding-order dependent) drops below
+    # tol: within a couple of iterations of the FP64 count.
+    assert err.sum() < 1e-6 and abs(st32.iterations - it) <= 2
 
 
 @pytest.mark.parametrize("hot", [0, 700, 20000])
