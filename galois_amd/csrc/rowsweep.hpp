@@ -9,10 +9,11 @@
 // plan stores each tile's first row and first edge.
 //
 // Tile work (one 512-lane workgroup, two resident per CU).
-//  1. Stage: the tile's row ends go to LDS; its columns (read one tile
-//     ahead, consecutive lanes on consecutive edges) are turned into edge
-//     values by all lanes at once (kItems gathers in flight per lane) and
-//     stored to LDS.
+//  1. Stage: a software pipeline keeps the next tile's gathers, row ends and
+//     finalisation operands, and the columns of the tile after it, in
+//     flight while a tile is processed (consecutive lanes on consecutive
+//     edges; values of the hottest sources come from an LDS cache).  A
+//     tile's values and row ends are then stored to LDS.
 //  2. Walk: lane t takes merge-path items [t*kUnits, (t+1)*kUnits), finds its
 //     starting (row, edge) by binary search in LDS and walks them in order,
 //     adding edge values and emitting a row sum at every row end.
@@ -187,18 +188,57 @@ __device__ __forceinline__ void tile_cols(const RowPlan& P, const TileCoord& tc,
   }
 }
 
-// One tile (see the file comment).  `cc` holds the tile's columns; the next
-// tile's coordinates and columns are loaded into `ncoord`/`ncc` while this
-// one is processed.
+// The loads of one tile, issued ahead of the tile's processing: columns
+// (two tiles ahead), then gathered values, the lane's first row end and the
+// finalisation operands of the lane's first row (one tile ahead).
+template <class Real, class Fin>
+struct Stage {
+  TileCoord tc;
+  bool valid;
+  int32_t cc[kItems];
+  Real v[kItems];
+  int32_t rend;
+  typename Fin::Pre pre;
+};
+
+template <class Real, class Fin>
+__device__ __forceinline__ void stage_cols(const RowPlan& P, int64_t T, Stage<Real, Fin>& st) {
+  st.valid = T < P.ntiles;
+  if (st.valid) {
+    st.tc = tile_coord(P, T);
+    tile_cols(P, st.tc, st.cc);
+  }
+}
+
 template <class Real, int HOT, class Src, class Fin>
-__device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan& P, const Src& src, const Fin& fin,
-                                           TileShared<Real>& sh, const Real* hot, Real* __restrict__ carry,
-                                           const TileCoord& tc,
-                                           const int32_t (&cc)[kItems], TileCoord& ncoord,
-                                           int32_t (&ncc)[kItems], double& a0, double& a1) {
+__device__ __forceinline__ void stage_gather(const RowPlan& P, const Src& src, const Fin& fin, const Real* hot,
+                                             Stage<Real, Fin>& st) {
+  if (!st.valid) return;
+  const int tid = threadIdx.x;
+  const int32_t e0 = st.tc.e0, e1 = st.tc.e1;
+#pragma unroll
+  for (int i = 0; i < kItems; ++i) {
+    const int64_t q = static_cast<int64_t>(e0) + tid + i * kSweepBlock;
+    if constexpr (HOT > 0)
+      st.v[i] = q < e1 ? src.apply(q, st.cc[i] < HOT ? hot[st.cc[i]] : src.load(st.cc[i])) : Real(0);
+    else
+      st.v[i] = q < e1 ? src(q, st.cc[i]) : Real(0);
+  }
+  const int nre = st.tc.r1 - st.tc.r0;
+  st.rend = tid < nre ? ld_stream(&P.rowoff[st.tc.r0 + 1 + tid]) : 0;
+  st.pre = tid < nre ? fin.pre(static_cast<uint32_t>(st.tc.r0 + tid)) : typename Fin::Pre{};
+}
+
+// One tile (see the file comment); its loads were issued by stage_gather
+// one tile earlier.
+template <class Real, class Src, class Fin>
+__device__ __forceinline__ void sweep_tile(int64_t T, const RowPlan& P, const Fin& fin, TileShared<Real>& sh,
+                                           Real* __restrict__ carry, const Stage<Real, Fin>& st, double& a0,
+                                           double& a1) {
   const int tid = threadIdx.x;
   const int lane = static_cast<int>(lane_id());
   const int wid = tid / kWave;
+  const TileCoord& tc = st.tc;
   const int32_t r0 = tc.r0, e0 = tc.e0, e1 = tc.e1;
   const int nre = tc.r1 - r0;  // row ends in this tile
   const int ned = e1 - e0;     // edges in this tile
@@ -206,32 +246,15 @@ __device__ __forceinline__ void sweep_tile(int64_t T, int64_t Tn, const RowPlan&
   const bool defer0 = tc.defer0;
   int32_t* s_end = reinterpret_cast<int32_t*>(sh.rows);
   Real* s_sum = reinterpret_cast<Real*>(sh.rows);
+  const typename Fin::Pre& pre0 = st.pre;
 
-  // 1. Issue every load of the tile at once: gathers, row ends, the first
-  // finalisation operands, and the next tile's coordinates and columns.
-  Real v[kItems];
-#pragma unroll
-  for (int i = 0; i < kItems; ++i) {
-    const int64_t q = static_cast<int64_t>(e0) + tid + i * kSweepBlock;
-    if constexpr (HOT > 0)
-      v[i] = q < e1 ? src.apply(q, cc[i] < HOT ? hot[cc[i]] : src.load(cc[i])) : Real(0);
-    else
-      v[i] = q < e1 ? src(q, cc[i]) : Real(0);
-  }
-  int32_t rend0 = 0;
-  if (tid < nre) rend0 = ld_stream(&P.rowoff[r0 + 1 + tid]);
-  typename Fin::Pre pre0{};
-  if (tid < nre) pre0 = fin.pre(static_cast<uint32_t>(r0 + tid));
-  if (Tn < P.ntiles) {
-    ncoord = tile_coord(P, Tn);
-    tile_cols(P, ncoord, ncc);
-  }
-  if (tid < nre) s_end[tid] = rend0 - e0;
+  // 1. Values and row ends to LDS.
+  if (tid < nre) s_end[tid] = st.rend - e0;
   for (int j = tid + kSweepBlock; j < nre; j += kSweepBlock) s_end[j] = ld_stream(&P.rowoff[r0 + 1 + j]) - e0;
 #pragma unroll
   for (int i = 0; i < kItems; ++i) {
     const int j = tid + i * kSweepBlock;
-    if (j < ned) sh.val[pad8(j)] = v[i];
+    if (j < ned) sh.val[pad8(j)] = st.v[i];
   }
   __syncthreads();
 
@@ -362,20 +385,21 @@ __global__ void __launch_bounds__(kSweepBlock) sweep_kernel(RowPlan P, Src src, 
     __syncthreads();
   }
   double a0 = 0, a1 = 0;
+  // Software pipeline: tile T is processed while the gathers of tile
+  // T + grid and the columns of tile T + 2*grid are in flight.
+  const int64_t G = gridDim.x;
   int64_t T = blockIdx.x;
-  TileCoord tc{};
-  int32_t cc[kItems];
-  if (T < P.ntiles) {
-    tc = tile_coord(P, T);
-    tile_cols(P, tc, cc);
-  }
-  for (; T < P.ntiles; T += gridDim.x) {
-    TileCoord nc{};
-    int32_t ncc[kItems];
-    sweep_tile<Real, HOT>(T, T + gridDim.x, P, src, fin, sh, hot, carry, tc, cc, nc, ncc, a0, a1);
-    tc = nc;
-#pragma unroll
-    for (int i = 0; i < kItems; ++i) cc[i] = ncc[i];
+  Stage<Real, Fin> cur, nxt;
+  stage_cols(P, T, cur);
+  stage_gather<Real, HOT>(P, src, fin, hot, cur);
+  stage_cols(P, T + G, nxt);
+  for (; T < P.ntiles; T += G) {
+    stage_gather<Real, HOT>(P, src, fin, hot, nxt);
+    Stage<Real, Fin> after;
+    stage_cols(P, T + 2 * G, after);
+    sweep_tile<Real, Src>(T, P, fin, sh, carry, cur, a0, a1);
+    cur = nxt;
+    nxt = after;
   }
   a0 = block_sum<kSweepBlock>(a0, red);
   a1 = block_sum<kSweepBlock>(a1, red);
