@@ -14,6 +14,7 @@ knobs respond to (hub count, gathered working set).
 Knobs and candidates:
   frontier.hub_degree  degree above which a frontier vertex is expanded by the
                        whole grid (BFS, SSSP, CC, k-core, BC)
+  pr.blocking          FP32 PageRank: propagation-blocked sweep (1) or merge-path pull (0)
   pr.hot_sources       sources of the first PageRank pull pass (0 = one pass)
 """
 from __future__ import annotations
@@ -27,6 +28,7 @@ import numpy as np
 
 CANDIDATES: dict[str, list[int]] = {
     "frontier.hub_degree": [256, 512, 1024, 2048, 8192, 65536],
+    "pr.blocking": [1, 0],
     "pr.hot_sources": [0, 1 << 17, 1 << 18, 1 << 19, 1 << 20, 1 << 21],
 }
 
@@ -37,7 +39,7 @@ KNOBS: dict[str, list[str]] = {
     "cc": ["frontier.hub_degree"],
     "kcore": ["frontier.hub_degree"],
     "bc": ["frontier.hub_degree"],
-    "pagerank": ["pr.hot_sources"],
+    "pagerank": ["pr.blocking", "pr.hot_sources"],
 }
 
 DEFAULT_CACHE = Path(os.environ.get("XDG_CACHE_HOME", Path.home() / ".cache")) / "galois_amd" / "tune.json"

@@ -115,6 +115,19 @@ struct PullPlan {
 // gathered most often get the lowest ids, so the hot part of the gathered
 // vector is a few contiguous megabytes.  trow/tcol are the transpose in new
 // ids (tcol padded by kEdgePad), outdeg is in new order.
+// Edge layout of the propagation-blocked PageRank sweep (pr_blocking.hip),
+// built once per graph from the degree-ordered transpose.  Edges are grouped
+// into runs by (source chunk, destination bin), chunk-major; src16/dst16 are
+// the in-chunk source and in-bin destination of every edge, vals the per-edge
+// contributions of the current iteration.  pieces/jobs/tasks/multi are the
+// work tables of the two phases.
+struct PbLayout {
+  bool ready = false;
+  int64_t nbins = 0, npieces = 0, ntasks = 0, nmulti = 0;
+  int64_t njobs[3] = {0, 0, 0};  // long, mid, short jobs
+  DevBuf src16, dst16, vals, pieces, jobs, tasks, multi, gacc;
+};
+
 struct DegreeOrder {
   bool ready = false;
   DevBuf new_of_old, old_of_new, trow, tcol, outdeg;
@@ -123,6 +136,7 @@ struct DegreeOrder {
   int64_t hot = -1, hot_edges = 0;
   DevBuf rowA, colA, rowB, colB;
   PullPlan planA, planB;
+  PbLayout pb;
 };
 
 // A fixed kernel sequence captured once into an executable HIP graph and

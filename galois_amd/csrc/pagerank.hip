@@ -33,10 +33,12 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cmath>
+#include <type_traits>
 
 #include "frontier.hpp"
 #include "gx_device.hpp"
 #include "gx_internal.hpp"
+#include "pr_blocking.hpp"
 #include "rowsweep.hpp"
 
 using namespace gx;
@@ -113,6 +115,34 @@ struct PartFin {
   __device__ __forceinline__ Pre pre(uint32_t) const { return Pre{}; }
   __device__ __forceinline__ void operator()(uint32_t r, Real sum, const Pre&, double&, double&) const {
     __builtin_nontemporal_store(sum, &part[r]);
+  }
+};
+
+// Row finalisation of the propagation-blocked FP32 sweep: the row sum
+// arrives exact (fixed point -> FP64); x' = base + alpha*sum is rounded to
+// FP32 once.
+struct PbFin {
+  const PrState* state;
+  const float* __restrict__ x_old;
+  float* __restrict__ x_new;
+  float* __restrict__ y_new;
+  const uint32_t* __restrict__ outdeg;
+  double base, alpha;
+  __device__ __forceinline__ void load() {
+    base = state->base;
+    alpha = state->alpha;
+  }
+  __device__ __forceinline__ void operator()(int64_t r, double sum, double& resid, double& dang) const {
+    const float xn = static_cast<float>(base + alpha * sum);
+    resid += fabs(static_cast<double>(xn) - static_cast<double>(rs::ld_stream(&x_old[r])));
+    __builtin_nontemporal_store(xn, &x_new[r]);
+    const uint32_t d = rs::ld_stream(&outdeg[r]);
+    if (d) {
+      y_new[r] = xn / static_cast<float>(d);
+    } else {
+      y_new[r] = 0.0f;
+      dang += static_cast<double>(xn);
+    }
   }
 };
 
@@ -277,6 +307,12 @@ static int64_t hot_sources(int64_t n, size_t real_bytes) {
   return (h <= 0 || h >= n) ? 0 : h;
 }
 
+// FP32 iterations use the propagation-blocked sweep unless the knob
+// "pr.blocking" is 0 or the graph is outside its table limits.
+static bool use_blocking(const gx_graph* g, size_t real_bytes) {
+  return real_bytes == sizeof(float) && tune_value("pr.blocking", 1) != 0 && pb::supported(g->n, g->m);
+}
+
 // Normalised pull PageRank over the degree-ordered transpose.
 template <class Real>
 static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, Real* pr, gx_stats* st) {
@@ -284,15 +320,19 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
   hipStream_t s = g->stream;
   build_degree_order(g);
   DegreeOrder& D = g->degree_order;
-  const int64_t hot = hot_sources(n, sizeof(Real));
+  const bool blocked = use_blocking(g, sizeof(Real));
+  const int64_t hot = blocked ? 0 : hot_sources(n, sizeof(Real));
   rs::RowPlan P{}, PA{};
-  if (hot > 0) {
+  if (blocked) {
+    pb::build_layout(g);
+  } else if (hot > 0) {
     build_split(g, hot);
     PA = rs::build_plan(g, D.rowA.as<int32_t>(), D.colA.as<int32_t>(), n, D.hot_edges, D.planA);
     P = rs::build_plan(g, D.rowB.as<int32_t>(), D.colB.as<int32_t>(), n, g->m - D.hot_edges, D.planB);
   } else {
     P = rs::build_plan(g, D.trow.as<int32_t>(), D.tcol.as<int32_t>(), n, g->m, g->pull_plan);
   }
+  const PbLayout& L = D.pb;
   Real* part = hot > 0 ? g->scratch<Real>(WS_V3, n) : nullptr;
   Real* carryA = hot > 0 ? g->scratch<Real>(WS_V4, 3 * PA.ntiles) : nullptr;
   double* partsA = hot > 0 ? g->scratch<double>(WS_V5, 2 * rs::kParts) : nullptr;
@@ -301,11 +341,11 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
   Real* yb = g->scratch<Real>(WS_V1, 2 * n);
   Real* x[2] = {xb, xb + n};
   Real* y[2] = {yb, yb + n};
-  const int nparts = rs::kParts;
+  const int nparts = blocked ? static_cast<int>(std::max<int64_t>(L.ntasks + L.nmulti, rs::kParts)) : rs::kParts;
   double* parts = g->scratch<double>(WS_REDUCE, 2 * nparts);
   double* part_resid = parts;
   double* part_dang = parts + nparts;
-  Real* carry = g->scratch<Real>(WS_V2, 3 * P.ntiles);
+  Real* carry = blocked ? nullptr : g->scratch<Real>(WS_V2, 3 * P.ntiles);
   PrState* state = g->scratch<PrState>(WS_COUNTERS, 1);
   PrState* host = static_cast<PrState*>(g->host_pinned(sizeof(PrState)));
 
@@ -327,6 +367,36 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
     const uint32_t end = std::min(max_iter, it + batch);
     for (; it < end; ++it) {
       const int nxt = cur ^ 1;
+      if constexpr (std::is_same<Real, float>::value) {
+        if (blocked) {
+          const pb::Job* jobs = L.jobs.as<pb::Job>();
+          ktimer.begin();
+          pb::gather_kernel<float><<<static_cast<unsigned>(L.npieces), pb::kBlock, 0, s>>>(
+              L.pieces.as<pb::Piece>(), L.src16.as<uint16_t>(), y[cur], n, L.vals.as<float>(), &state->converged);
+          GX_LAUNCHED("pb::gather_kernel");
+          PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, 0.0, 0.0};
+          pb::accum_kernel<PbFin><<<static_cast<unsigned>(L.ntasks), pb::kBlock, 0, s>>>(
+              L.tasks.as<pb::Task>(), jobs, jobs + L.njobs[0], jobs + L.njobs[0] + L.njobs[1], L.vals.as<float>(),
+              L.dst16.as<uint16_t>(), L.gacc.as<unsigned long long>(), n, fin, part_resid, part_dang,
+              &state->converged);
+          GX_LAUNCHED("pb::accum_kernel");
+          launches += 2;
+          if (L.nmulti > 0) {
+            pb::multi_kernel<PbFin><<<static_cast<unsigned>(L.nmulti), pb::kBlock, 0, s>>>(
+                L.multi.as<pb::Multi>(), L.gacc.as<unsigned long long>(), n, fin, part_resid + L.ntasks,
+                part_dang + L.ntasks, &state->converged);
+            GX_LAUNCHED("pb::multi_kernel");
+            launches += 1;
+          }
+          ktimer.end();
+          pr_reduce_kernel<<<1, kBlock, 0, s>>>(state, part_resid, part_dang,
+                                                static_cast<int>(L.ntasks + L.nmulti), static_cast<unsigned>(nxt));
+          GX_LAUNCHED("pr_reduce_kernel");
+          launches += 1;
+          cur = nxt;
+          continue;
+        }
+      }
       PrSrc<Real> src{y[cur]};
       PrFin<Real> fin{state, x[cur], x[nxt], y[nxt], outdeg, part, Real(0), Real(0)};
       ktimer.begin();

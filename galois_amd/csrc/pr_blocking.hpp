@@ -1,0 +1,266 @@
+// Propagation-blocked PageRank sweep (gfx950).
+//
+// The pull sweep of rowsweep.hpp gathers y[u] for every in-edge: ~1e9
+// 4-byte random reads per RMAT-26 iteration, each of which moves a whole
+// cache line, so the sweep is bound by wasted line traffic (about ten times
+// the algorithmic bytes).  Propagation blocking (Beamer, Asanovic, Patterson,
+// "Reducing PageRank Communication via Propagation Blocking", IPDPS 2017)
+// replaces the random reads with two streaming phases over a static edge
+// layout built once per graph:
+//
+//   layout  edges grouped into runs by (source chunk, destination bin),
+//           chunk-major, kSpan vertices per chunk and per bin; per edge the
+//           in-chunk source (src16) and in-bin destination (dst16).
+//   phase 1 (gather_kernel) one workgroup per piece of a chunk: the chunk's
+//           kSpan y values are staged in LDS and vals[p] = y[src16[p]] is
+//           streamed out in layout order (2-byte read, 4-byte write per edge).
+//   phase 2 (accum_kernel) one workgroup per task (a bin, or a slice of a
+//           heavy bin): the bin's runs are read from every chunk and
+//           vals[p] is added into an LDS accumulator per destination
+//           (4 + 2 bytes read per edge), then the bin's rows are finalised.
+//
+// Row sums are exact: every contribution (0 <= y <= 1 for the normalised
+// iteration) is rounded to a 64-bit fixed-point number with 2^-62 resolution
+// and added with integer atomics, so the sum does not depend on the order of
+// the additions and the result is deterministic.  Slices of a heavy bin add
+// their LDS sums into a per-bin global accumulator that multi_kernel
+// finalises (and clears for the next iteration).
+//
+// Phase-2 work is described by jobs (start, length) of at most kJob entries
+// in three size classes, so that runs of every length keep the lanes busy:
+// long jobs (65..512 entries) take a wavefront each, mid jobs (9..64) eight
+// lanes each, short jobs (1..8) one lane each.
+#pragma once
+
+#include <cstdint>
+
+#include "gx_device.hpp"
+#include "gx_internal.hpp"
+#include "rowsweep.hpp"
+
+namespace gx {
+namespace pb {
+
+constexpr int kShift = 14;           // log2 of the vertices per chunk and per bin
+constexpr int kSpan = 1 << kShift;   // 16K: 64 KB of FP32 y, 128 KB of 64-bit sums
+constexpr int kBlock = 1024;
+constexpr int kWaves = kBlock / kWave;
+constexpr int kJob = 512;            // entries per long job: 8 per lane
+constexpr int kMid = 64;             // mid jobs: 8 lanes x 8 entries
+constexpr int kShort = 8;            // short jobs: 1 lane x 8 entries
+constexpr int64_t kTaskEntries = int64_t(1) << 19;   // bins above this are sliced
+constexpr int64_t kPieceEntries = int64_t(1) << 18;  // phase-1 piece length
+constexpr int64_t kMaxKeys = int64_t(1) << 28;       // chunk x bin table limit
+constexpr uint32_t kSingle = 0xffffffffu;            // task finalises its bin itself
+
+struct Piece {
+  uint32_t chunk, p0, p1, pad;
+};
+struct Job {
+  uint32_t start, len;
+};
+struct Task {
+  uint32_t bin, slot;  // slot: global accumulator of a sliced bin, or kSingle
+  uint32_t job0[3], job1[3];  // [begin, end) in the long / mid / short job lists
+};
+struct Multi {
+  uint32_t bin, slot;
+};
+
+// Whether the layout can be built for this graph (nonempty, table size).
+inline bool supported(int64_t n, int64_t m) {
+  const int64_t nb = (n + kSpan - 1) / kSpan;
+  return n > 0 && m > 0 && m < (int64_t(1) << 31) && nb * nb <= kMaxKeys;
+}
+
+// y value -> fixed point with 2^-62 resolution, rounded to nearest.  Valid
+// for 0 <= v < 2; the normalised iteration keeps every x (and so every y and
+// every row sum) within [0, 1].
+__device__ __forceinline__ unsigned long long to_fixed(float v) {
+  const uint32_t b = __float_as_uint(v);
+  const int e = static_cast<int>(b >> 23);  // biased exponent (v >= 0)
+  if (e == 0) return 0;                     // zero / subnormal: below 2^-63
+  const unsigned long long mant = (b & 0x7fffffu) | 0x800000u;
+  const int sh = e - 88;                    // v * 2^62 = mant * 2^(e - 127 - 23 + 62)
+  if (sh >= 0) return mant << sh;
+  if (sh <= -25) return 0;
+  const int r = -sh;
+  return (mant + (1ull << (r - 1))) >> r;
+}
+
+__device__ __forceinline__ double from_fixed(unsigned long long s) {
+  return static_cast<double>(s) * 0x1p-62;
+}
+
+// Phase 1: vals[p] = y[chunk base + src16[p]] over one piece.
+template <class Real>
+__global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict__ pieces,
+                                                        const uint16_t* __restrict__ src16,
+                                                        const Real* __restrict__ y, int64_t n,
+                                                        Real* __restrict__ vals, const unsigned* stop) {
+  if (*stop) return;
+  __shared__ Real ys[kSpan];
+  const Piece P = pieces[blockIdx.x];
+  const int64_t base = static_cast<int64_t>(P.chunk) << kShift;
+  const int cnt = static_cast<int>(n - base < kSpan ? n - base : kSpan);
+  for (int i = threadIdx.x; i < cnt; i += kBlock) ys[i] = rs::ld_stream(&y[base + i]);
+  __syncthreads();
+  // Body in 4-entry groups: one 8-byte index load and one 16-byte store per
+  // lane, contiguous across the wavefront; the unaligned head and tail of the
+  // piece entry by entry.
+  const int64_t a4 = (static_cast<int64_t>(P.p0) + 3) & ~int64_t(3);
+  const int64_t b4 = static_cast<int64_t>(P.p1) & ~int64_t(3);
+  if (a4 >= b4) {
+    for (int64_t q = static_cast<int64_t>(P.p0) + threadIdx.x; q < P.p1; q += kBlock)
+      __builtin_nontemporal_store(ys[src16[q]], &vals[q]);
+    return;
+  }
+  if (threadIdx.x < a4 - P.p0) {
+    const int64_t q = P.p0 + threadIdx.x;
+    vals[q] = ys[src16[q]];
+  }
+  if (threadIdx.x < P.p1 - b4) {
+    const int64_t q = b4 + threadIdx.x;
+    vals[q] = ys[src16[q]];
+  }
+  typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+  typedef Real r4 __attribute__((ext_vector_type(4)));
+  const u32x2* s4 = reinterpret_cast<const u32x2*>(src16 + a4);
+  r4* v4 = reinterpret_cast<r4*>(vals + a4);
+  const int64_t groups = (b4 - a4) >> 2;
+  constexpr int kU = 4;
+  for (int64_t gi = threadIdx.x; gi < groups; gi += kU * kBlock) {
+    u32x2 w[kU];
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int64_t q = gi + k * kBlock;
+      if (q < groups) w[k] = __builtin_nontemporal_load(&s4[q]);
+    }
+#pragma unroll
+    for (int k = 0; k < kU; ++k) {
+      const int64_t q = gi + k * kBlock;
+      if (q < groups) {
+        r4 o;
+        o.x = ys[w[k].x & 0xffffu];
+        o.y = ys[w[k].x >> 16];
+        o.z = ys[w[k].y & 0xffffu];
+        o.w = ys[w[k].y >> 16];
+        __builtin_nontemporal_store(o, &v4[q]);
+      }
+    }
+  }
+}
+
+// Adds the entries of one job class into the LDS sums.  Lanes are grouped
+// LPJ to a job; lane `sub` of a group takes entries sub + LPJ*k, k < 8.  A
+// wavefront loads the entries of kU job groups before it adds any of them,
+// so that kU x 8 loads per lane are in flight.
+template <int LPJ>
+__device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_t j0, uint32_t j1,
+                                           const float* __restrict__ vals, const uint16_t* __restrict__ dst16,
+                                           unsigned long long* acc) {
+  constexpr int kGroups = kWave / LPJ;
+  constexpr int kU = 4;
+  const unsigned lane = lane_id(), wave = threadIdx.x / kWave;
+  const unsigned grp = lane / LPJ, sub = lane % LPJ;
+  for (uint32_t jb = j0 + wave * kGroups * kU; jb < j1; jb += kWaves * kGroups * kU) {
+    Job J[kU];
+    float v[kU][8];
+    uint32_t d[kU][8];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const uint32_t j = jb + u * kGroups + grp;
+      J[u] = Job{0, 0};
+      if (j < j1) J[u] = jobs[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t i = sub + LPJ * k;
+        v[u][k] = 0.0f;
+        d[u][k] = 0;
+        if (i < J[u].len) {
+          v[u][k] = rs::ld_stream(&vals[J[u].start + i]);
+          d[u][k] = dst16[J[u].start + i];
+        }
+      }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+#pragma unroll
+      for (int k = 0; k < 8; ++k)
+        if (sub + LPJ * k < J[u].len) atomicAdd(&acc[d[u][k]], to_fixed(v[u][k]));
+  }
+}
+
+// Phase 2: one task.  Fin::operator()(row, sum, resid, dang) finalises a row.
+template <class Fin>
+__global__ void __launch_bounds__(kBlock) accum_kernel(const Task* __restrict__ tasks, const Job* __restrict__ jl,
+                                                       const Job* __restrict__ jm, const Job* __restrict__ js,
+                                                       const float* __restrict__ vals,
+                                                       const uint16_t* __restrict__ dst16,
+                                                       unsigned long long* __restrict__ gacc, int64_t n, Fin fin,
+                                                       double* __restrict__ part_resid,
+                                                       double* __restrict__ part_dang, const unsigned* stop) {
+  if (*stop) return;
+  __shared__ unsigned long long acc[kSpan];
+  __shared__ double red[kWaves];
+  const Task T = tasks[blockIdx.x];
+  for (int i = threadIdx.x; i < kSpan; i += kBlock) acc[i] = 0;
+  __syncthreads();
+  accum_jobs<kWave>(jl, T.job0[0], T.job1[0], vals, dst16, acc);
+  accum_jobs<8>(jm, T.job0[1], T.job1[1], vals, dst16, acc);
+  accum_jobs<1>(js, T.job0[2], T.job1[2], vals, dst16, acc);
+  __syncthreads();
+  const int64_t r0 = static_cast<int64_t>(T.bin) << kShift;
+  const int rows = static_cast<int>(n - r0 < kSpan ? n - r0 : kSpan);
+  double resid = 0, dang = 0;
+  if (T.slot == kSingle) {
+    fin.load();
+    for (int i = threadIdx.x; i < rows; i += kBlock) fin(r0 + i, from_fixed(acc[i]), resid, dang);
+  } else {
+    unsigned long long* g = gacc + static_cast<size_t>(T.slot) * kSpan;
+    for (int i = threadIdx.x; i < rows; i += kBlock)
+      if (acc[i]) atomicAdd(&g[i], acc[i]);
+  }
+  resid = rs::block_sum<kBlock>(resid, red);
+  dang = rs::block_sum<kBlock>(dang, red);
+  if (threadIdx.x == 0) {
+    part_resid[blockIdx.x] = resid;
+    part_dang[blockIdx.x] = dang;
+  }
+}
+
+// Finalises the sliced bins from their global accumulators and clears them.
+template <class Fin>
+__global__ void __launch_bounds__(kBlock) multi_kernel(const Multi* __restrict__ multi,
+                                                       unsigned long long* __restrict__ gacc, int64_t n, Fin fin,
+                                                       double* __restrict__ part_resid,
+                                                       double* __restrict__ part_dang, const unsigned* stop) {
+  if (*stop) return;
+  __shared__ double red[kWaves];
+  const Multi M = multi[blockIdx.x];
+  const int64_t r0 = static_cast<int64_t>(M.bin) << kShift;
+  const int rows = static_cast<int>(n - r0 < kSpan ? n - r0 : kSpan);
+  unsigned long long* g = gacc + static_cast<size_t>(M.slot) * kSpan;
+  fin.load();
+  double resid = 0, dang = 0;
+  for (int i = threadIdx.x; i < rows; i += kBlock) {
+    const unsigned long long s = g[i];
+    g[i] = 0;
+    fin(r0 + i, from_fixed(s), resid, dang);
+  }
+  resid = rs::block_sum<kBlock>(resid, red);
+  dang = rs::block_sum<kBlock>(dang, red);
+  if (threadIdx.x == 0) {
+    part_resid[blockIdx.x] = resid;
+    part_dang[blockIdx.x] = dang;
+  }
+}
+
+// Builds g->degree_order.pb from the CSR and the degree order (which must
+// exist).  Requires supported(n, m).
+void build_layout(gx_graph* g);
+
+}  // namespace pb
+}  // namespace gx

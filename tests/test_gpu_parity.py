@@ -266,14 +266,47 @@ def test_pagerank_hot_cold_split(gx, oracle, rmat16, hot):
     say(f"rmat16 pagerank hot={hot}")
     exp, it, _ = oracle.pagerank(row, col, tol=1e-9, max_iter=100, threads=8)
     gx.tune_set("pr.hot_sources", hot)
+    gx.tune_set("pr.blocking", 0)  # FP32 through the merge-path pull as well
     try:
         x64, st = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp64")
         x32, st32 = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
     finally:
         gx.tune_set("pr.hot_sources", -1)
+        gx.tune_set("pr.blocking", -1)
     assert st.iterations == it
     np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-11, atol=1e-17)
     assert np.abs(x32.cpu().numpy().astype(np.float64) - exp).sum() < 1e-6
+
+
+@pytest.mark.parametrize("scale", [16, 20])
+def test_pagerank_blocked_vs_pull(gx, oracle, scale):
+    """FP32 propagation-blocked sweep (exact fixed-point row sums) and the
+    FP32 merge-path pull (Kahan row sums) against the FP64 oracle: L1 < 1e-6;
+    FP32 stops on its own residual, which reaches 1e-9 no earlier than FP64's
+    (minus rounding luck) and often later; the blocked result is
+    bit-identical across runs.  At scale 20 the heaviest destination bins
+    exceed one task and are summed in slices (one extra launch per
+    iteration)."""
+    say(f"rmat{scale} pagerank blocked vs pull")
+    G = gx.Graph.rmat(scale, 16, symmetric=False)
+    row, col, _ = G.to_host()
+    exp, it, _ = oracle.pagerank(row, col, tol=1e-9, max_iter=100, threads=8)
+    xb, stb = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
+    xb2, _ = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
+    gx.tune_set("pr.blocking", 0)
+    try:
+        xp, stp = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
+    finally:
+        gx.tune_set("pr.blocking", -1)
+    for x, st in ((xb, stb), (xp, stp)):
+        err = np.abs(x.cpu().numpy().astype(np.float64) - exp)
+        assert err.sum() < 1e-6 and it - 2 <= st.iterations < 100, (err.sum(), st.iterations, it)
+    np.testing.assert_array_equal(xb.cpu().numpy(), xb2.cpu().numpy())
+    if scale == 20:
+        # init (3) + unpermute (1) + per enqueued iteration (batches of 8):
+        # gather, accumulate, sliced bins, reduce
+        enq = min(100, -(-stb.iterations // 8) * 8)
+        assert stb.launches == 4 + 4 * enq, (stb.launches, stb.iterations)
 
 
 @pytest.mark.parametrize("hub", [64, 100000])
