@@ -126,8 +126,19 @@ __global__ void level_end_kernel(BfsState* S) {
   S->active = 0;
 }
 
+// Plain pointers only: the output arrays come from the device state and the
+// next queue is picked by the state's index, so the kernel-argument Bufs is
+// never copied or modified (a modified copy, with its runtime-indexed queue
+// array, would be kept in scratch memory).
 struct TdOp {
-  Bufs B;
+  const int32_t* __restrict__ row;
+  const int32_t* __restrict__ col;
+  uint32_t* level;
+  uint32_t* parent;
+  uint32_t* qout;
+  unsigned* qcount;
+  unsigned cap;
+  BfsState* S;
   LdsQueue<kQCap>* sq;
   uint32_t nl;
   unsigned long long scout;  // this lane's degree sum of discovered vertices
@@ -135,50 +146,48 @@ struct TdOp {
     bool won = false;
     uint32_t v = 0;
     if (valid) {
-      v = static_cast<uint32_t>(B.col[e]);
-      if (B.level[v] == kInf && atomicCAS(&B.level[v], kInf, nl) == kInf) {
+      v = static_cast<uint32_t>(col[e]);
+      if (level[v] == kInf && atomicCAS(&level[v], kInf, nl) == kInf) {
         won = true;
-        if (B.parent) B.parent[v] = u;
-        scout += static_cast<unsigned long long>(B.row[v + 1] - B.row[v]);
+        if (parent) parent[v] = u;
+        scout += static_cast<unsigned long long>(row[v + 1] - row[v]);
       }
     }
     lq_push(*sq, won, v);
   }
-  __device__ __forceinline__ void after_step() {
-    BfsState* S = B.S;
-    lq_flush(*sq, B.q[S->q ^ 1], &S->count[S->q ^ 1], B.cap, &S->overflow, false);
-  }
+  __device__ __forceinline__ void after_step() { lq_flush(*sq, qout, qcount, cap, &S->overflow, false); }
 };
 
-__device__ __forceinline__ void td_finish(TdOp& op, double* /*unused*/) {
-  BfsState* S = op.B.S;
-  lq_flush(*op.sq, op.B.q[S->q ^ 1], &S->count[S->q ^ 1], op.B.cap, &S->overflow, true);
+__device__ __forceinline__ TdOp make_td(const Bufs& B, LdsQueue<kQCap>* sq) {
+  BfsState* S = B.S;
+  const unsigned nq = S->q ^ 1;
+  return TdOp{B.row, B.col, S->level_out, S->parent_out, B.q[nq], &S->count[nq], B.cap, S, sq, S->level, 0ull};
+}
+
+__device__ __forceinline__ void td_finish(TdOp& op) {
+  lq_flush(*op.sq, op.qout, op.qcount, op.cap, &op.S->overflow, true);
   unsigned long long sc = wave_sum(op.scout);
-  if (lane_id() == 0 && sc) atomicAdd(&S->scout, sc);
+  if (lane_id() == 0 && sc) atomicAdd(&op.S->scout, sc);
 }
 
 __global__ void __launch_bounds__(kBlock) td_expand_kernel(Bufs B) {
   BfsState* S = B.S;
   if (!S->active) return;
-  B.level = S->level_out;
-  B.parent = S->parent_out;
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
-  TdOp op{B, &sq, S->level, 0ull};
+  TdOp op = make_td(B, &sq);
   expand_queue<kBlock>(B.row, B.q[S->q], S->count[S->q], B.hub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
-  td_finish(op, nullptr);
+  td_finish(op);
 }
 
 __global__ void __launch_bounds__(kBlock) td_hubs_kernel(Bufs B) {
   BfsState* S = B.S;
   if (!S->active || S->hub_count == 0) return;
-  B.level = S->level_out;
-  B.parent = S->parent_out;
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
-  TdOp op{B, &sq, S->level, 0ull};
+  TdOp op = make_td(B, &sq);
   expand_hubs<kBlock>(B.row, B.hubq, S->hub_count, B.cap, B.hub_off, op);
-  td_finish(op, nullptr);
+  td_finish(op);
 }
 
 void enqueue_level(const Bufs& B, hipStream_t s) {

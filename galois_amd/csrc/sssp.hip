@@ -150,8 +150,12 @@ __global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B);
 
 constexpr int kQCap = 2048;
 
+// Holds the output pointer read from the device state: the kernel-argument
+// Bufs stays unmodified (a modified copy of it, with its runtime-indexed
+// queue arrays, would be kept in scratch memory).
 struct RelaxOp {
   Bufs B;
+  uint32_t* dist;
   LdsQueue<kQCap>* qn;  // staged near pushes
   LdsQueue<kQCap>* qf;  // staged far pushes
   uint32_t* nearq_out;
@@ -164,9 +168,9 @@ struct RelaxOp {
     uint32_t v = 0;
     if (valid) {
       v = static_cast<uint32_t>(B.col[e]);
-      const uint32_t nd = B.dist[u] + B.w[e];
-      if (nd < B.dist[v]) {
-        const uint32_t old = atomicMin(&B.dist[v], nd);
+      const uint32_t nd = dist[u] + B.w[e];
+      if (nd < dist[v]) {
+        const uint32_t old = atomicMin(&dist[v], nd);
         if (nd < old) {
           if ((nd >> B.shift) <= bucket)
             to_near = atomicExch(&B.stamp[v], round) != round;
@@ -188,13 +192,12 @@ struct RelaxOp {
 __device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn, LdsQueue<kQCap>* qf) {
   SsspState* S = B.S;
   const unsigned q = S->q, fs = S->far_sel;
-  return RelaxOp{B, qn, qf, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.farq[fs], &S->far_count[fs], S->bucket,
-                 S->round, S->phase};
+  return RelaxOp{B, S->dist_out, qn, qf, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.farq[fs], &S->far_count[fs],
+                 S->bucket, S->round, S->phase};
 }
 
 // Relax step, part 2: expand the filtered items.
 __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
-  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kRelax) return;
   __shared__ LdsQueue<kQCap> qn, qf;
@@ -206,7 +209,6 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
 }
 
 __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
-  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kRelax || S->hub_count == 0) return;
   __shared__ LdsQueue<kQCap> qn, qf;
@@ -218,9 +220,9 @@ __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
 }
 
 __global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
-  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kRelax) return;
+  const uint32_t* __restrict__ dist = S->dist_out;
   __shared__ LdsQueue<kQCap> qo;
   lq_init(qo);
   const unsigned q = S->q;
@@ -233,7 +235,7 @@ __global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
     uint32_t u = 0;
     if (i < cnt) {
       u = in[i];
-      const uint32_t d = B.dist[u];
+      const uint32_t d = dist[u];
       keep = B.done[u] != d;
       if (keep) {
         B.done[u] = d;
@@ -255,16 +257,16 @@ __global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
 
 // Advance step, part 1: smallest bucket among far items above the current.
 __global__ void __launch_bounds__(kBlock) far_min_kernel(Bufs B) {
-  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kAdvance) return;
+  const uint32_t* __restrict__ dist = S->dist_out;
   const unsigned fs = S->far_sel;
   const unsigned cnt = S->far_count[fs];
   const uint32_t* farq = B.farq[fs];
   const uint32_t bucket = S->bucket;
   uint32_t best = 0xffffffffu;
   for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock) {
-    const uint32_t b = B.dist[farq[i]] >> B.shift;
+    const uint32_t b = dist[farq[i]] >> B.shift;
     if (b > bucket && b < best) best = b;
   }
   best = wave_min(best);
@@ -274,9 +276,9 @@ __global__ void __launch_bounds__(kBlock) far_min_kernel(Bufs B) {
 // Advance step, part 2: far items of the new bucket go to the near queue,
 // later ones are kept (one copy per vertex), settled ones are dropped.
 __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
-  B.dist = B.S->dist_out;
   SsspState* S = B.S;
   if (S->mode != kAdvance) return;
+  const uint32_t* __restrict__ dist = S->dist_out;
   const uint32_t next = S->min_bucket;
   if (next == 0xffffffffu) return;
   __shared__ LdsQueue<kQCap> qn, qk;
@@ -292,7 +294,7 @@ __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
     uint32_t v = 0;
     if (i < cnt) {
       v = farq[i];
-      const uint32_t b = B.dist[v] >> B.shift;
+      const uint32_t b = dist[v] >> B.shift;
       if (b == next)
         to_near = atomicExch(&B.stamp[v], round) != round;
       else if (b > next)
