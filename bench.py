@@ -50,11 +50,11 @@ def sync():
     torch.cuda.synchronize()
 
 
-def measured_traffic(scale: int):
-    """HBM bytes per pull sweep from the committed rocprofv3 PMC summary
-    (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy kernel), or
-    None when no summary for this graph is present."""
-    f = ROOT / "profiles" / "pagerank_pull_rmat26_traffic.json"
+def measured_traffic(scale: int, sweep: str):
+    """HBM bytes per PageRank sweep from the committed rocprofv3 PMC summary
+    of that sweep (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy
+    kernel), or None when no summary for this graph and sweep is present."""
+    f = ROOT / "profiles" / f"pagerank_{sweep}_rmat26_traffic.json"
     if scale != 26 or not f.exists():
         return None
     try:
@@ -290,7 +290,7 @@ def main() -> int:
         return gx.pagerank(G, alpha=0.85, tol=1e-9, max_iter=100, precision="fp32")
 
     if args.pmc_pass:
-        gx.pagerank(G, alpha=0.85, tol=0.0, max_iter=1, precision="fp32")  # plans and split built
+        gx.pagerank(G, alpha=0.85, tol=0.0, max_iter=1, precision="fp32")  # sweep tables built
         _, st = gx.pagerank(G, alpha=0.85, tol=0.0, max_iter=3, precision="fp32")
         del G
         a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -328,18 +328,29 @@ def main() -> int:
     ms_per_step = dt * 1e3 / args.steps
     log(f"pagerank fp32: {ms_per_step:.1f} ms/step, {iters / args.steps:.0f} iterations, {value:.2f} GTEPS")
 
-    # Roofline of the dominant kernel, the pull sweep (hot-source pass + cold
-    # pass + spanning-row completion, bracketed together per iteration).
-    # Algorithmic bytes of one sweep: 4m column ids + 4(n+1) row offsets +
-    # 4n y (read once) + 4n x_old + 4n out-degrees + 8n (x', y' written).
+    # Roofline of the dominant kernel: one FP32 sweep, bracketed by HIP events
+    # per iteration on the graph's stream.
+    #  blocked (default): phase 1 reads 2m in-chunk source ids and 4n y, writes
+    #    4m contributions; phase 2 reads the 4m contributions and 2m in-bin
+    #    destination ids, reads x_old and out-degrees (8n), writes x' and y'
+    #    (8n): 12m + 20n bytes.
+    #  pull: 4m column ids + 4(n+1) row offsets + 4n y (read once) + 4n x_old +
+    #    4n out-degrees + 8n (x', y' written).
+    blocked = gx.tune_get("pr.blocking") != 0
+    sweep = "blocked" if blocked else "pull"
     sweep_ms = sweep_ms_total / max(sweeps, 1)
-    algo_bytes = 4 * m + 4 * (n + 1) + 20 * n
+    if blocked:
+        algo_bytes = 12 * m + 20 * n
+        kname = ("propagation-blocked sweep: pb::gather_kernel<float> + pb::accum_kernel<PbFin> "
+                 "+ pb::multi_kernel<PbFin> (sliced bins)")
+    else:
+        algo_bytes = 4 * m + 4 * (n + 1) + 20 * n
+        kname = ("pull sweep: rs::sweep_kernel<float,HOT,PrSrc,PrFin> + rs::span_finalize_kernel")
     achieved = algo_bytes / (sweep_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": measured_traffic(args.scale),
-                "kernel": "pull sweep: rs::sweep_kernel<float,PrSrc,PartFin> + rs::sweep_kernel<float,PrSrc,PrFin>"
-                          " + rs::span_finalize_kernel (x2)",
-                "bytes_per_launch": algo_bytes, "avg_launch_ms": round(sweep_ms, 4), "launches": sweeps}
+                "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": measured_traffic(args.scale, sweep),
+                "kernel": kname, "bytes_per_launch": algo_bytes, "avg_launch_ms": round(sweep_ms, 4),
+                "launches": sweeps}
 
     # FP64 run and the FP32-vs-FP64 error.
     x64, st64 = gx.pagerank(G, alpha=0.85, tol=1e-9, max_iter=100, precision="fp64")
@@ -372,9 +383,10 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"pagerank_pull_fp32_kahan_rmat{args.scale}_directed", "n": n, "m": m,
+        "config": {"workload": f"pagerank_fp32_rmat{args.scale}_directed", "n": n, "m": m,
                    "edge_factor": 16, "alpha": 0.85, "tol": 1e-9, "max_iter": 100, "seed": SEED,
-                   "knobs": {k: gx.tune_get(k) for k in ("pr.hot_sources", "frontier.hub_degree")},
+                   "sweep": "propagation_blocked" if blocked else "merge_path_pull",
+                   "knobs": {k: gx.tune_get(k) for k in ("pr.blocking", "pr.hot_sources", "frontier.hub_degree")},
                    "autotuned": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,
