@@ -14,6 +14,12 @@
 //   D = sum of x over dangling vertices; stop when sum|x'-x| < tol (FP64) or
 //   after max_iter iterations.
 //
+// FP32 iterations run the propagation-blocked sweep of pr_blocking.hpp by
+// default (knob "pr.blocking"): per-edge contributions streamed through a
+// static (source chunk, destination bin) layout and exact fixed-point row
+// sums, instead of a random gather per edge.  FP64 iterations and
+// pr.blocking = 0 use the merge-path pull sweep below.
+//
 // Design of the pull sweep (gfx950, HBM/latency bound, no MFMA)
 //  - The pull runs over the transpose in degree order (vertices relabelled
 //    by out-degree, descending), so the most-gathered y entries share cache
@@ -119,21 +125,21 @@ struct PartFin {
 };
 
 // Row finalisation of the propagation-blocked FP32 sweep: the row sum
-// arrives exact (fixed point -> FP64); x' = base + alpha*sum is rounded to
-// FP32 once.
+// arrives exact (64-bit fixed point) and is rounded to FP32 once; x' =
+// base + alpha*sum in FP32 as in PrFin; residual and dangling mass in FP64.
 struct PbFin {
   const PrState* state;
   const float* __restrict__ x_old;
   float* __restrict__ x_new;
   float* __restrict__ y_new;
   const uint32_t* __restrict__ outdeg;
-  double base, alpha;
+  float base, alpha;
   __device__ __forceinline__ void load() {
-    base = state->base;
-    alpha = state->alpha;
+    base = static_cast<float>(state->base);
+    alpha = static_cast<float>(state->alpha);
   }
-  __device__ __forceinline__ void operator()(int64_t r, double sum, double& resid, double& dang) const {
-    const float xn = static_cast<float>(base + alpha * sum);
+  __device__ __forceinline__ void operator()(int64_t r, unsigned long long fixed, double& resid, double& dang) const {
+    const float xn = base + alpha * pb::fixed_to_float(fixed);
     resid += fabs(static_cast<double>(xn) - static_cast<double>(rs::ld_stream(&x_old[r])));
     __builtin_nontemporal_store(xn, &x_new[r]);
     const uint32_t d = rs::ld_stream(&outdeg[r]);
@@ -374,7 +380,7 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
           pb::gather_kernel<float><<<static_cast<unsigned>(L.npieces), pb::kBlock, 0, s>>>(
               L.pieces.as<pb::Piece>(), L.src16.as<uint16_t>(), y[cur], n, L.vals.as<float>(), &state->converged);
           GX_LAUNCHED("pb::gather_kernel");
-          PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, 0.0, 0.0};
+          PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, 0.0f, 0.0f};
           pb::accum_kernel<PbFin><<<static_cast<unsigned>(L.ntasks), pb::kBlock, 0, s>>>(
               L.tasks.as<pb::Task>(), jobs, jobs + L.njobs[0], jobs + L.njobs[0] + L.njobs[1], L.vals.as<float>(),
               L.dst16.as<uint16_t>(), L.gacc.as<unsigned long long>(), n, fin, part_resid, part_dang,

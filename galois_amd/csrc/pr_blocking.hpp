@@ -88,8 +88,9 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
   return (mant + (1ull << (r - 1))) >> r;
 }
 
-__device__ __forceinline__ double from_fixed(unsigned long long s) {
-  return static_cast<double>(s) * 0x1p-62;
+// Exact fixed-point sum -> FP32, rounded once.
+__device__ __forceinline__ float fixed_to_float(unsigned long long s) {
+  return __ull2float_rn(s) * 0x1p-62f;
 }
 
 // Phase 1: vals[p] = y[chunk base + src16[p]] over one piece.
@@ -193,7 +194,8 @@ __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_
   }
 }
 
-// Phase 2: one task.  Fin::operator()(row, sum, resid, dang) finalises a row.
+// Phase 2: one task.  Fin::operator()(row, fixed-point sum, resid, dang)
+// finalises a row.
 template <class Fin>
 __global__ void __launch_bounds__(kBlock) accum_kernel(const Task* __restrict__ tasks, const Job* __restrict__ jl,
                                                        const Job* __restrict__ jm, const Job* __restrict__ js,
@@ -217,7 +219,7 @@ __global__ void __launch_bounds__(kBlock) accum_kernel(const Task* __restrict__ 
   double resid = 0, dang = 0;
   if (T.slot == kSingle) {
     fin.load();
-    for (int i = threadIdx.x; i < rows; i += kBlock) fin(r0 + i, from_fixed(acc[i]), resid, dang);
+    for (int i = threadIdx.x; i < rows; i += kBlock) fin(r0 + i, acc[i], resid, dang);
   } else {
     unsigned long long* g = gacc + static_cast<size_t>(T.slot) * kSpan;
     for (int i = threadIdx.x; i < rows; i += kBlock)
@@ -248,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) multi_kernel(const Multi* __restrict__
   for (int i = threadIdx.x; i < rows; i += kBlock) {
     const unsigned long long s = g[i];
     g[i] = 0;
-    fin(r0 + i, from_fixed(s), resid, dang);
+    fin(r0 + i, s, resid, dang);
   }
   resid = rs::block_sum<kBlock>(resid, red);
   dang = rs::block_sum<kBlock>(dang, red);
