@@ -77,6 +77,27 @@ struct PrSrc {
   __device__ __forceinline__ Real apply(int64_t, Real v) const { return v; }
 };
 
+// FP32 update x' = base + alpha*(s_hi + s_lo) with base and alpha carried as
+// unevaluated FP32 pairs (hi + lo): an error-free product (FMA) and sum
+// (TwoSum) leave x' nearly correctly rounded, so its FP32 storage is the only
+// rounding that matters.  Rounding alpha = 0.85 itself to FP32 (relative
+// 2.8e-8) would move the fixed point by about alpha/(1-alpha) times that,
+// ~2e-7 in L1 on RMAT-26.
+struct F32Pair {
+  float hi, lo;
+};
+__device__ __forceinline__ F32Pair split_f64(double v) {
+  const float h = static_cast<float>(v);
+  return F32Pair{h, static_cast<float>(v - static_cast<double>(h))};
+}
+__device__ __forceinline__ float pr_update_f32(F32Pair b, F32Pair a, float sh, float sl) {
+  const float p = __fmul_rn(a.hi, sh);
+  const float pe = __fmaf_rn(a.hi, sh, -p);
+  float t, te;
+  two_sum(p, b.hi, t, te);
+  return t + (((pe + te) + b.lo) + (a.hi * sl + a.lo * sh));
+}
+
 // x'[r] = base + alpha*sum; residual |x'-x| and dangling mass in FP64.
 template <class Real>
 struct PrFin {
@@ -87,9 +108,12 @@ struct PrFin {
   const uint32_t* __restrict__ outdeg;
   const Real* __restrict__ part;  // row sums of the hot pass (null: single pass)
   Real base, alpha;
+  F32Pair bp, ap;  // FP32: base and alpha as pairs
   __device__ __forceinline__ void load() {
     base = static_cast<Real>(state->base);
     alpha = static_cast<Real>(state->alpha);
+    bp = split_f64(state->base);
+    ap = split_f64(state->alpha);
   }
   struct Pre {
     Real xo, pa;
@@ -99,7 +123,14 @@ struct PrFin {
     return Pre{rs::ld_stream(&x_old[r]), part ? rs::ld_stream(&part[r]) : Real(0), rs::ld_stream(&outdeg[r])};
   }
   __device__ __forceinline__ void operator()(uint32_t r, Real sum, const Pre& p, double& resid, double& dang) const {
-    const Real xn = base + alpha * (p.pa + sum);
+    Real xn;
+    if constexpr (std::is_same<Real, float>::value) {
+      float sh, sl;
+      two_sum(p.pa, sum, sh, sl);
+      xn = pr_update_f32(bp, ap, sh, sl);
+    } else {
+      xn = base + alpha * (p.pa + sum);
+    }
     resid += fabs(static_cast<double>(xn) - static_cast<double>(p.xo));
     __builtin_nontemporal_store(xn, &x_new[r]);
     const uint32_t d = p.d;
@@ -125,21 +156,23 @@ struct PartFin {
 };
 
 // Row finalisation of the propagation-blocked FP32 sweep: the row sum
-// arrives exact (64-bit fixed point) and is rounded to FP32 once; x' =
-// base + alpha*sum in FP32 as in PrFin; residual and dangling mass in FP64.
+// arrives exact (64-bit fixed point) as an FP32 pair; x' = base + alpha*sum
+// by the compensated FP32 update; residual and dangling mass in FP64.
 struct PbFin {
   const PrState* state;
   const float* __restrict__ x_old;
   float* __restrict__ x_new;
   float* __restrict__ y_new;
   const uint32_t* __restrict__ outdeg;
-  float base, alpha;
+  F32Pair base, alpha;
   __device__ __forceinline__ void load() {
-    base = static_cast<float>(state->base);
-    alpha = static_cast<float>(state->alpha);
+    base = split_f64(state->base);
+    alpha = split_f64(state->alpha);
   }
   __device__ __forceinline__ void operator()(int64_t r, unsigned long long fixed, double& resid, double& dang) const {
-    const float xn = base + alpha * pb::fixed_to_float(fixed);
+    float sh, sl;
+    pb::fixed_to_float2(fixed, sh, sl);
+    const float xn = pr_update_f32(base, alpha, sh, sl);
     resid += fabs(static_cast<double>(xn) - static_cast<double>(rs::ld_stream(&x_old[r])));
     __builtin_nontemporal_store(xn, &x_new[r]);
     const uint32_t d = rs::ld_stream(&outdeg[r]);
@@ -380,7 +413,7 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
           pb::gather_kernel<float><<<static_cast<unsigned>(L.npieces), pb::kBlock, 0, s>>>(
               L.pieces.as<pb::Piece>(), L.src16.as<uint16_t>(), y[cur], n, L.vals.as<float>(), &state->converged);
           GX_LAUNCHED("pb::gather_kernel");
-          PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, 0.0f, 0.0f};
+          PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, {}, {}};
           pb::accum_kernel<PbFin><<<static_cast<unsigned>(L.ntasks), pb::kBlock, 0, s>>>(
               L.tasks.as<pb::Task>(), jobs, jobs + L.njobs[0], jobs + L.njobs[0] + L.njobs[1], L.vals.as<float>(),
               L.dst16.as<uint16_t>(), L.gacc.as<unsigned long long>(), n, fin, part_resid, part_dang,

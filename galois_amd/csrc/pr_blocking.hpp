@@ -88,9 +88,12 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
   return (mant + (1ull << (r - 1))) >> r;
 }
 
-// Exact fixed-point sum -> FP32, rounded once.
-__device__ __forceinline__ float fixed_to_float(unsigned long long s) {
-  return __ull2float_rn(s) * 0x1p-62f;
+// Exact fixed-point sum -> unevaluated FP32 pair hi + lo (about 48 bits).
+__device__ __forceinline__ void fixed_to_float2(unsigned long long s, float& hi, float& lo) {
+  const float h = __ull2float_rn(s);
+  const long long r = static_cast<long long>(s - __float2ull_rn(h));  // exact, may be negative
+  hi = h * 0x1p-62f;
+  lo = __ll2float_rn(r) * 0x1p-62f;
 }
 
 // Phase 1: vals[p] = y[chunk base + src16[p]] over one piece.
