@@ -247,6 +247,46 @@ def kernel_table(gx, quick: bool) -> dict:
         "grad_w_rel_err_fp32_vs_fp64": rel_w, "grad_t_rel_err_fp32_vs_fp64": rel_t,
         "fd_max_rel_err_16_edges": float(np.max(np.abs(fd - a64[top]) / np.abs(a64[top])))}
     del G
+    if not quick:
+        out.update(rmat26_table(gx))
+    return out
+
+
+def rmat26_table(gx) -> dict:
+    """BFS (top-down and direction-optimising, from the max-degree vertex),
+    SSSP (delta 64 from vertex 0), CC and k-core (k=10) on the symmetrised
+    RMAT-26 graph with uniform weights 1..255 (2.1e9 directed edges): the
+    scale the plain kernels are targeted at.  Parity is tested at smaller
+    scales; here only times, GTEPS (m / time, the Graph500 convention for the
+    direction-optimising BFS) and the algorithmic-byte share of HBM peak."""
+    out = {}
+    G = gx.Graph.rmat(26, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
+    row = G.row_host()
+    n, m = G.n, G.m
+    deg = np.diff(row).astype(np.int64)
+    del row
+    src = int(np.argmax(deg))
+    runs = {"bfs": lambda: gx.bfs(G, src), "bfs_do": lambda: gx.bfs_do(G, src), "sssp": lambda: gx.sssp(G, 0, 6),
+            "cc": lambda: gx.connected_components(G), "kcore10": lambda: gx.kcore(G, 10)}
+    for kname, fn in runs.items():
+        fn()
+        res = min((fn() for _ in range(2)), key=lambda r: r[-1].elapsed_ms)
+        st = res[-1]
+        if kname.startswith("bfs"):
+            lv = res[0].cpu().numpy().view(np.uint32)
+            reached = lv != gx.GX_DIST_INFINITY
+            byt = 4 * (n + 1) + 4 * int(deg[reached].sum()) + 4 * n + 4 * int(reached.sum())
+        elif kname == "sssp":
+            d = res[0].cpu().numpy().view(np.uint32)
+            byt = 8 * int(deg[d != gx.GX_DIST_INFINITY].sum()) + 12 * n
+        elif kname == "cc":
+            byt = 4 * m + 12 * n
+        else:
+            alive = res[1].cpu().numpy()
+            byt = 4 * (n + 1) + 4 * int(deg[alive == 0].sum()) + 8 * n
+        out[f"{kname}_rmat26"] = {"ms": round(st.elapsed_ms, 4), "GTEPS": round(m / st.elapsed_ms / 1e6, 3),
+                                  "pct_hbm": pct(byt, st.elapsed_ms), "rounds": st.iterations, "n": n, "m": m}
+    del G
     return out
 
 

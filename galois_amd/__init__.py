@@ -167,6 +167,12 @@ class Graph:
                                       None if w is None else w.ctypes.data))
         return row, col[:self.m], (None if w is None else w[:self.m])
 
+    def row_host(self) -> np.ndarray:
+        """Row offsets int32[n+1] only (the edge arrays are not copied)."""
+        row = np.empty(self.n + 1, dtype=np.int32)
+        check(lib().gx_graph_download(self._h, row.ctypes.data, None, None))
+        return row
+
     def transpose_host(self):
         """(trow, tcol, perm) of the device-built transpose as numpy arrays."""
         trow = np.empty(self.n + 1, dtype=np.int32)

@@ -382,7 +382,7 @@ __global__ void relabel_keys_kernel(const int32_t* __restrict__ col, const uint3
 // Sort `cnt` edge keys (u << bits | v), drop duplicates and self-loops, and
 // make a library-owned CSR of n vertices on g.
 static void csr_from_keys(gx_graph* g, DevBuf& keys, int64_t cnt, int bits, int64_t n, hipStream_t s) {
-  GX_REQUIRE(cnt < (int64_t(1) << 32), "too many generated edges for 32-bit compaction");
+  GX_REQUIRE(cnt <= (int64_t(1) << 32), "too many generated edges for 32-bit compaction");
   DevBuf sorted;
   sorted.ensure(std::max<int64_t>(cnt, 1) * sizeof(uint64_t));
   size_t tmp = 0;
