@@ -309,6 +309,48 @@ def test_pagerank_blocked_vs_pull(gx, oracle, scale):
         assert stb.launches == 4 + 4 * enq, (stb.launches, stb.iterations)
 
 
+def test_pagerank_blocked_shapes(gx, oracle):
+    """Propagation-blocked sweep on shapes its tables must split: a source
+    with 600K out-edges (one chunk in several phase-1 pieces), a destination
+    with 600K in-edges (its bin sliced over several tasks plus the global
+    accumulator), n one past a 16K chunk boundary (a one-vertex chunk and
+    bin), a band of vertices with neither in- nor out-edges (after the degree
+    order, empty chunks at the end), dangling vertices.  FP32 vs the FP64 oracle to 1e-6 in L1, and
+    deterministic."""
+    say("pagerank blocked shapes")
+    rng = np.random.default_rng(11)
+    n = 5 * 16384 + 16385  # 98,305 vertices: last chunk holds one vertex
+    k = 600000
+    hub_out, hub_in = 70000, 3
+    srcs = [np.full(k, hub_out), rng.integers(0, n, k), rng.integers(0, 16384, 200000)]
+    dsts = [rng.integers(0, n, k), np.full(k, hub_in), rng.integers(0, n, 200000)]
+    src = np.concatenate(srcs).astype(np.int64)
+    dst = np.concatenate(dsts).astype(np.int64)
+    quiet = (src >= 32768) & (src < 49152) | (dst >= 32768) & (dst < 49152)  # chunk and bin 2 empty
+    src, dst = src[~quiet], dst[~quiet]
+    src = np.append(src, n - 1)
+    dst = np.append(dst, 0)  # the one-vertex chunk has an out-edge
+    keys = np.unique(src * n + dst)
+    s, d = keys // n, keys % n
+    keep = s != d
+    s, d = s[keep], d[keep]
+    row = np.zeros(n + 1, np.int64)
+    np.add.at(row, s + 1, 1)
+    row = np.cumsum(row).astype(np.int32)
+    col = d.astype(np.int32)
+    G = gx.Graph.from_csr(row, col, None, symmetric=False)
+    exp, it, _ = oracle.pagerank(row, col, tol=1e-10, max_iter=200, threads=8)
+    x1, st1 = gx.pagerank(G, tol=1e-10, max_iter=200, precision="fp32")
+    x2, _ = gx.pagerank(G, tol=1e-10, max_iter=200, precision="fp32")
+    a = x1.cpu().numpy()
+    assert np.abs(a.astype(np.float64) - exp).sum() < 1e-6
+    np.testing.assert_array_equal(a, x2.cpu().numpy())
+    # tol 1e-10 is below FP32 resolution here: most x values stop changing and
+    # FP32's own residual can reach tol before FP64's does; accuracy is the
+    # check above.
+    assert st1.iterations <= 200 and (st1.residual < 1e-10 or st1.iterations == 200)
+
+
 @pytest.mark.parametrize("hub", [64, 100000])
 def test_hub_degree_knob(gx, oracle, rmat16, hub):
     """Results do not depend on the grid-wide hub threshold."""
