@@ -109,6 +109,28 @@ SIGNATURES: dict[str, tuple] = {
     "gx_pagerank_adjoint": (INT, [P, P, P, P, D, D, U32, INT, P, P, PST]),
     "gx_bc": (INT, [P, P, U32, P, PST]),
     "gx_bc_batched": (INT, [P, P, U32, P, PST]),
+    # D-IrGL personality partition
+    "gx_part_create": (INT, [INT, PG]),
+    "gx_part_free": (INT, [P]),
+    "gx_part_load": (INT, [P, U64, U64, P, P, P, U32, U32, U32]),
+    "gx_part_range": (INT, [P, INT, C.POINTER(U32), C.POINTER(U32)]),
+    "gx_part_get": (INT, [P, INT, U32, C.POINTER(U32)]),
+    "gx_part_set": (INT, [P, INT, U32, U32]),
+    "gx_part_add": (INT, [P, INT, U32, U32]),
+    "gx_part_min": (INT, [P, INT, U32, U32, C.POINTER(INT)]),
+    "gx_part_fill": (INT, [P, INT, U32, U32, U32]),
+    "gx_part_read": (INT, [P, INT, U32, U32, P]),
+    "gx_part_bitset_get": (INT, [P, INT, P]),
+    "gx_part_bitset_reset": (INT, [P, INT, U32, U32]),
+    "gx_part_bfs_init": (INT, [P, U32, U32, U32, U64]),
+    "gx_part_bfs_first": (INT, [P, U32, U32]),
+    "gx_part_bfs": (INT, [P, U32, U32, U32, C.POINTER(U32), C.POINTER(U32)]),
+    "gx_part_bfs_check": (INT, [P, U32, U32, U32, C.POINTER(U64), C.POINTER(U32)]),
+    "gx_part_kcore_init1": (INT, [P, U32, U32]),
+    "gx_part_kcore_init2": (INT, [P, U32, U32]),
+    "gx_part_kcore_step1": (INT, [P, U32, U32, U32, C.POINTER(U32)]),
+    "gx_part_kcore_step2": (INT, [P, U32, U32]),
+    "gx_part_kcore_check": (INT, [P, U32, U32, C.POINTER(U64)]),
 }
 
 _lib: C.CDLL | None = None

@@ -21,13 +21,14 @@
 
 namespace gx {
 
-// Identity frontier: vertex i of [0, count).  Otherwise q[i].  With ADMIT,
-// op.admit(v) is asked once per frontier vertex and a vertex it rejects
-// contributes no edges (a fused filter).
+// Identity frontier (q null): vertex first + i for i in [0, count).
+// Otherwise q[i].  With ADMIT, op.admit(v) is asked once per frontier vertex
+// and a vertex it rejects contributes no edges (a fused filter).
 template <int BLOCK, class Op, bool ADMIT = false>
 __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, const uint32_t* __restrict__ q,
                                              unsigned qcount, uint32_t hub_thresh, uint32_t* __restrict__ hubq,
-                                             unsigned* hub_count, unsigned cap, unsigned* overflow, Op& op) {
+                                             unsigned* hub_count, unsigned cap, unsigned* overflow, Op& op,
+                                             uint32_t first = 0) {
   __shared__ int32_t s_start[BLOCK];
   __shared__ uint32_t s_off[BLOCK];
   __shared__ uint32_t s_src[BLOCK];
@@ -39,7 +40,7 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
     int32_t start = 0;
     bool hub = false;
     if (i < qcount) {
-      v = q ? q[i] : i;
+      v = q ? q[i] : first + i;
       bool in = true;
       if constexpr (ADMIT) in = op.admit(v);
       if (in) {

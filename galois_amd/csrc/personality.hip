@@ -1,0 +1,636 @@
+// D-IrGL personality boundary: the per-operator GPU kernels that D-IrGL host
+// code launches on one host's partition.
+//
+// Reference
+//  - context and graph: get_CUDA_context / init_CUDA_context /
+//    load_graph_CUDA (libgluon/include/galois/cuda/HostDecls.h:38-54,98-102);
+//  - BFS push operators InitializeGraph, FirstItr_BFS, BFS, BFSSanityCheck
+//    (lonestar/analytics/distributed/bfs/bfs_push.cpp:95-340, declared in
+//    bfs_push_cuda.h:103-140) over the fields dist_current / dist_old;
+//  - k-core push operators InitializeGraph1/2, KCoreStep1/2, KCoreSanityCheck
+//    (lonestar/analytics/distributed/k-core/kcore_push.cpp:93-360, declared
+//    in kcore_push_cuda.h:133-167) over current_degree / trim / flag;
+//  - field accessors get/set/add/min_node_<field>_cuda, batch_reset, and the
+//    change bitsets get_bitset_<field>_cuda / bitset_<field>_reset_cuda.
+//
+// Every operator runs over local ids [begin, end) (the _allNodes /
+// _masterNodes / _nodesWithEdges variants are three such ranges, see
+// gx_part_range) and returns its reductions through out-parameters, as the
+// reference's DGAccumulator arguments.  Operators that walk edges use the
+// load-balanced expansion of frontier.hpp with the range as an identity
+// frontier and the operator's vertex test as the admission filter, so hub
+// vertices are spread over the grid instead of serialising one thread.
+// The multi-host mirror exchange (batch_get/set/min over master/mirror
+// lists) belongs to gluon and is not part of this one-GPU build.
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "frontier.hpp"
+#include "gx_device.hpp"
+#include "gx_internal.hpp"
+
+using namespace gx;
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kMaxGrid = 2048;
+constexpr int kFields = 5;
+
+// Device counters of one operator launch.
+struct PartCounters {
+  unsigned hub_count;
+  unsigned overflow;
+  unsigned active;
+  unsigned changed;
+  unsigned long long work;
+  unsigned long long sum;
+  unsigned max;
+  unsigned pad;
+};
+
+}  // namespace
+
+struct gx_part {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  int64_t n = 0, m = 0;
+  uint32_t num_owned = 0, begin_master = 0, with_edges = 0;
+  bool loaded = false;
+  gx::DevBuf row, col, gid, field[kFields], bits[kFields], hubq, hub_off, counters;
+  PartCounters* host = nullptr;  // pinned
+  ~gx_part() {
+    if (host) (void)hipHostFree(host);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  uint32_t* f(int k) const { return field[k].as<uint32_t>(); }
+  unsigned long long* b(int k) const { return bits[k].as<unsigned long long>(); }
+  int64_t words() const { return (n + 63) / 64; }
+};
+
+namespace {
+
+void check_part(const gx_part* p) {
+  GX_REQUIRE(p != nullptr, "partition handle is NULL");
+  GX_REQUIRE(p->loaded, "partition has no graph (gx_part_load)");
+}
+void check_field(int k) { GX_REQUIRE(k >= 0 && k < kFields, "unknown node field"); }
+void check_range(const gx_part* p, uint32_t begin, uint32_t end) {
+  GX_REQUIRE(begin <= end && static_cast<int64_t>(end) <= p->n, "range [begin, end) outside the partition");
+}
+
+__device__ __forceinline__ void set_bit(unsigned long long* bits, uint32_t v) {
+  atomicOr(&bits[v >> 6], 1ull << (v & 63));
+}
+
+// Per-lane counters reduced once per wavefront at the end of a launch.
+__device__ __forceinline__ void flush_counts(PartCounters* c, unsigned active, unsigned long long work) {
+  active = wave_sum(active);
+  work = wave_sum(work);
+  if (lane_id() == 0) {
+    if (active) atomicAdd(&c->active, active);
+    if (work) atomicAdd(&c->work, work);
+  }
+}
+
+__global__ void fill_kernel_u32(uint32_t* __restrict__ f, uint32_t begin, uint32_t end, uint32_t v) {
+  for (uint32_t i = begin + blockIdx.x * blockDim.x + threadIdx.x; i < end; i += gridDim.x * blockDim.x) f[i] = v;
+}
+
+// Clears the bits of [begin, end): whole words stored, edge words masked.
+__global__ void bitset_clear_kernel(unsigned long long* __restrict__ bits, uint32_t begin, uint32_t end) {
+  const uint32_t w0 = begin >> 6, w1 = (end - 1) >> 6;
+  for (uint32_t w = w0 + blockIdx.x * blockDim.x + threadIdx.x; w <= w1; w += gridDim.x * blockDim.x) {
+    const uint32_t lo = max(begin, w << 6), hi = min(end, (w << 6) + 64);  // bits [lo, hi) of word w
+    const unsigned long long mask =
+        (hi - lo == 64) ? ~0ull : (((1ull << (hi - lo)) - 1) << (lo & 63));
+    if (mask == ~0ull)
+      bits[w] = 0;
+    else
+      atomicAnd(&bits[w], ~mask);
+  }
+}
+
+__global__ void node_update_kernel(uint32_t* __restrict__ f, uint32_t lid, uint32_t v, int op,
+                                   PartCounters* __restrict__ c) {
+  if (threadIdx.x != 0) return;
+  if (op == 0) {
+    atomicAdd(&f[lid], v);
+  } else {
+    const uint32_t old = atomicMin(&f[lid], v);
+    c->changed = old > v ? 1u : 0u;
+  }
+}
+
+// ---------------------------------------------------------------- BFS push
+// InitializeGraph (bfs_push.cpp:118-125): dist = 0 at the source's global
+// id, infinity elsewhere.
+__global__ void bfs_init_kernel(uint32_t* __restrict__ cur, uint32_t* __restrict__ old,
+                                const uint32_t* __restrict__ gid, uint32_t begin, uint32_t end, uint32_t inf,
+                                uint64_t src) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const uint64_t g = gid ? gid[v] : v;
+    const uint32_t d = g == src ? 0u : inf;
+    cur[v] = d;
+    old[v] = d;
+  }
+}
+
+// FirstItr_BFS / BFS (bfs_push.cpp:170-181, 254-273): an admitted vertex
+// copies dist_current into dist_old and pushes dist_current + 1 to every
+// out-neighbour with atomicMin, marking lowered neighbours in the bitset.
+// BFS admits vertices with dist_old > dist_current (counted active) whose
+// dist_current is below the priority; FirstItr_BFS admits its whole range.
+struct BfsOp {
+  const int32_t* __restrict__ col;
+  uint32_t* cur;
+  uint32_t* old;
+  unsigned long long* bits;
+  PartCounters* c;
+  uint32_t priority;
+  bool first;
+  unsigned active = 0;
+  unsigned long long work = 0;
+  __device__ __forceinline__ bool admit(uint32_t v) {
+    const uint32_t dc = cur[v];
+    if (first) {
+      old[v] = dc;
+      return true;
+    }
+    if (old[v] > dc) {
+      ++active;
+      if (priority > dc) {
+        old[v] = dc;
+        return true;
+      }
+    }
+    return false;
+  }
+  __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
+    if (!valid) return;
+    const uint32_t v = static_cast<uint32_t>(col[e]);
+    const uint32_t nd = 1u + cur[u];
+    const uint32_t prev = atomicMin(&cur[v], nd);
+    if (prev > nd) set_bit(bits, v);
+    ++work;
+  }
+  __device__ __forceinline__ void after_step() {}
+  __device__ __forceinline__ void finish() { flush_counts(c, active, first ? 0ull : work); }
+};
+
+// BFSSanityCheck (bfs_push.cpp:323-330): reached count and maximum distance.
+__global__ void __launch_bounds__(kBlock) bfs_check_kernel(const uint32_t* __restrict__ cur, uint32_t begin,
+                                                           uint32_t end, uint32_t inf, PartCounters* c) {
+  unsigned long long cnt = 0;
+  uint32_t mx = 0;
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const uint32_t d = cur[v];
+    if (d < inf) {
+      ++cnt;
+      mx = max(mx, d);
+    }
+  }
+  cnt = wave_sum(cnt);
+  mx = wave_max(mx);
+  if (lane_id() == 0) {
+    if (cnt) atomicAdd(&c->sum, cnt);
+    atomicMax(&c->max, mx);
+  }
+}
+
+// ------------------------------------------------------------- k-core push
+// InitializeGraph1 (kcore_push.cpp:140-181): flag set,
+// trim and current_degree zero.
+__global__ void kcore_init1_kernel(uint32_t* __restrict__ deg, uint32_t* __restrict__ trim,
+                                   uint32_t* __restrict__ flag, uint32_t begin, uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    flag[v] = 1;
+    trim[v] = 0;
+    deg[v] = 0;
+  }
+}
+
+// InitializeGraph2 (kcore_push.cpp:93-137: every out-edge adds one to the
+// destination's degree) and KCoreStep1 (kcore_push.cpp:227-318: a live
+// vertex below k dies, counted active, and adds one
+// to the trim of every out-neighbour); both mark the bitset of the field
+// they add to.
+struct KcoreOp {
+  const int32_t* __restrict__ col;
+  uint32_t* deg;
+  uint32_t* trim;
+  uint32_t* flag;
+  uint32_t* target;  // deg (InitializeGraph2) or trim (KCoreStep1)
+  unsigned long long* bits;
+  PartCounters* c;
+  uint32_t k;
+  bool step;  // KCoreStep1
+  unsigned active = 0;
+  __device__ __forceinline__ bool admit(uint32_t v) {
+    if (!step) return true;
+    if (flag[v] && deg[v] < k) {
+      flag[v] = 0;
+      ++active;
+      return true;
+    }
+    return false;
+  }
+  __device__ __forceinline__ void operator()(bool valid, uint32_t, int32_t e) {
+    if (!valid) return;
+    const uint32_t v = static_cast<uint32_t>(col[e]);
+    atomicAdd(&target[v], 1u);
+    set_bit(bits, v);
+  }
+  __device__ __forceinline__ void after_step() {}
+  __device__ __forceinline__ void finish() { flush_counts(c, active, 0ull); }
+};
+
+// KCoreStep2 (kcore_push.cpp:184-224): a live vertex loses
+// its trim from its degree; trim is cleared everywhere in the range.
+__global__ void kcore_step2_kernel(uint32_t* __restrict__ deg, uint32_t* __restrict__ trim,
+                                   const uint32_t* __restrict__ flag, uint32_t begin, uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const uint32_t t = trim[v];
+    if (flag[v] && t > 0) deg[v] -= t;
+    trim[v] = 0;
+  }
+}
+
+// KCoreSanityCheck (kcore_push.cpp:321-365): live vertices in the range.
+__global__ void __launch_bounds__(kBlock) kcore_check_kernel(const uint32_t* __restrict__ flag, uint32_t begin,
+                                                             uint32_t end, PartCounters* c) {
+  unsigned long long cnt = 0;
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x)
+    cnt += flag[v] ? 1 : 0;
+  cnt = wave_sum(cnt);
+  if (lane_id() == 0 && cnt) atomicAdd(&c->sum, cnt);
+}
+
+// ------------------------------------------------------- edge operators
+template <class Op>
+__global__ void __launch_bounds__(kBlock) range_expand_kernel(const int32_t* __restrict__ row, uint32_t begin,
+                                                              uint32_t count, uint32_t hub, uint32_t* hubq,
+                                                              PartCounters* c, unsigned cap, Op op) {
+  expand_queue<kBlock, Op, true>(row, nullptr, count, hub, hubq, &c->hub_count, cap, &c->overflow, op, begin);
+  op.finish();
+}
+
+template <class Op>
+__global__ void __launch_bounds__(kBlock) range_hubs_kernel(const int32_t* __restrict__ row,
+                                                            const uint32_t* __restrict__ hubq, PartCounters* c,
+                                                            unsigned cap, const uint32_t* __restrict__ off, Op op) {
+  expand_hubs<kBlock>(row, hubq, c->hub_count, cap, off, op);
+  op.finish();
+}
+
+unsigned grid_of(uint32_t count) {
+  return static_cast<unsigned>(std::max<int64_t>(1, std::min<int64_t>((count + kBlock - 1) / kBlock, kMaxGrid)));
+}
+
+void reset_counters(gx_part* p) {
+  GX_HIP(hipMemsetAsync(p->counters.p, 0, sizeof(PartCounters), p->stream));
+}
+
+// Waits for the operator and copies its counters to the pinned host block.
+const PartCounters& read_counters(gx_part* p) {
+  GX_HIP(hipMemcpyAsync(p->host, p->counters.p, sizeof(PartCounters), hipMemcpyDeviceToHost, p->stream));
+  GX_HIP(hipStreamSynchronize(p->stream));
+  if (p->host->overflow) fail(GX_ERR_CAPACITY, "partition hub queue overflow");
+  return *p->host;
+}
+
+// Range expansion of an edge operator: admitted vertices of [begin, end)
+// expanded by workgroups, hubs spread over the grid.
+template <class Op>
+void run_edges(gx_part* p, uint32_t begin, uint32_t end, const Op& op) {
+  const uint32_t count = end - begin;
+  if (count == 0) return;
+  PartCounters* c = p->counters.as<PartCounters>();
+  const unsigned cap = static_cast<unsigned>(p->n);
+  const int32_t* row = p->row.as<int32_t>();
+  range_expand_kernel<Op><<<grid_of(count), kBlock, 0, p->stream>>>(row, begin, count, hub_degree(),
+                                                                     p->hubq.as<uint32_t>(), c, cap, op);
+  GX_LAUNCHED("range_expand_kernel");
+  hub_offsets_kernel<kBlock><<<1, 1024, 0, p->stream>>>(row, p->hubq.as<uint32_t>(), &c->hub_count, cap,
+                                                       p->hub_off.as<uint32_t>());
+  GX_LAUNCHED("hub_offsets_kernel");
+  range_hubs_kernel<Op><<<kMaxGrid, kBlock, 0, p->stream>>>(row, p->hubq.as<uint32_t>(), c, cap,
+                                                           p->hub_off.as<uint32_t>(), op);
+  GX_LAUNCHED("range_hubs_kernel");
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI
+extern "C" gx_status gx_part_create(int device, gx_part** out) {
+  GX_API_BEGIN
+  check_ptr(out, "out");
+  *out = nullptr;
+  int count = 0;
+  GX_HIP(hipGetDeviceCount(&count));
+  GX_REQUIRE(device >= 0 && device < count, "device index out of range");
+  GX_HIP(hipSetDevice(device));
+  auto p = std::make_unique<gx_part>();
+  p->device = device;
+  GX_HIP(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+  GX_HIP(hipHostMalloc(reinterpret_cast<void**>(&p->host), sizeof(PartCounters), hipHostMallocDefault));
+  p->counters.ensure(sizeof(PartCounters));
+  *out = p.release();
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_free(gx_part* p) {
+  GX_API_BEGIN
+  if (p) {
+    (void)hipStreamSynchronize(p->stream);
+    delete p;
+  }
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint32_t* row_start,
+                                  const uint32_t* edge_dst, const uint32_t* node_gid, uint32_t num_owned,
+                                  uint32_t begin_master, uint32_t num_nodes_with_edges) {
+  GX_API_BEGIN
+  GX_REQUIRE(p != nullptr, "partition handle is NULL");
+  check_ptr(row_start, "row_start");
+  GX_REQUIRE(n >= 1 && n < (uint64_t(1) << 31), "node count must be in [1, 2^31)");
+  GX_REQUIRE(m < (uint64_t(1) << 31), "edge count must be < 2^31");
+  if (m) check_ptr(edge_dst, "edge_dst");
+  GX_REQUIRE(static_cast<uint64_t>(begin_master) + num_owned <= n, "master range outside the partition");
+  GX_REQUIRE(num_nodes_with_edges <= n, "nodes-with-edges count exceeds the node count");
+  // The kernels index by these values: validate before anything is uploaded.
+  GX_REQUIRE(row_start[0] == 0 && row_start[n] == m, "row_start must start at 0 and end at m");
+  for (uint64_t v = 0; v < n; ++v)
+    GX_REQUIRE(row_start[v] <= row_start[v + 1], "row_start must be non-decreasing");
+  for (uint64_t e = 0; e < m; ++e) GX_REQUIRE(edge_dst[e] < n, "edge destination out of range");
+  GX_HIP(hipSetDevice(p->device));
+  p->loaded = false;
+  p->n = static_cast<int64_t>(n);
+  p->m = static_cast<int64_t>(m);
+  p->num_owned = num_owned;
+  p->begin_master = begin_master;
+  p->with_edges = num_nodes_with_edges;
+  p->row.ensure((n + 1) * 4);
+  p->col.ensure(std::max<uint64_t>(m, 1) * 4);
+  GX_HIP(hipMemcpy(p->row.p, row_start, (n + 1) * 4, hipMemcpyHostToDevice));
+  if (m) GX_HIP(hipMemcpy(p->col.p, edge_dst, m * 4, hipMemcpyHostToDevice));
+  if (node_gid) {
+    p->gid.ensure(n * 4);
+    GX_HIP(hipMemcpy(p->gid.p, node_gid, n * 4, hipMemcpyHostToDevice));
+  } else {
+    p->gid.release();
+  }
+  for (int k = 0; k < kFields; ++k) {
+    p->field[k].ensure(n * 4);
+    GX_HIP(hipMemset(p->field[k].p, 0, n * 4));
+    p->bits[k].ensure(p->words() * 8);
+    GX_HIP(hipMemset(p->bits[k].p, 0, p->words() * 8));
+  }
+  p->hubq.ensure(n * 4);
+  p->hub_off.ensure((n + 1) * 4);
+  p->loaded = true;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_range(const gx_part* p, int which, uint32_t* begin, uint32_t* end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_ptr(begin, "begin");
+  check_ptr(end, "end");
+  switch (which) {
+    case GX_RANGE_ALL: *begin = 0; *end = static_cast<uint32_t>(p->n); break;
+    case GX_RANGE_MASTERS: *begin = p->begin_master; *end = p->begin_master + p->num_owned; break;
+    case GX_RANGE_WITH_EDGES: *begin = 0; *end = p->with_edges; break;
+    default: fail(GX_ERR_INVALID, "unknown range kind");
+  }
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_get(const gx_part* p, int field, uint32_t lid, uint32_t* value) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_ptr(value, "value");
+  check_range(p, lid, lid + 1);
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_HIP(hipMemcpy(value, p->f(field) + lid, 4, hipMemcpyDeviceToHost));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_set(gx_part* p, int field, uint32_t lid, uint32_t value) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_range(p, lid, lid + 1);
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_HIP(hipMemcpy(p->f(field) + lid, &value, 4, hipMemcpyHostToDevice));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_add(gx_part* p, int field, uint32_t lid, uint32_t value) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_range(p, lid, lid + 1);
+  reset_counters(p);
+  node_update_kernel<<<1, 64, 0, p->stream>>>(p->f(field), lid, value, 0, p->counters.as<PartCounters>());
+  GX_LAUNCHED("node_update_kernel");
+  read_counters(p);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_min(gx_part* p, int field, uint32_t lid, uint32_t value, int* changed) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_range(p, lid, lid + 1);
+  reset_counters(p);
+  node_update_kernel<<<1, 64, 0, p->stream>>>(p->f(field), lid, value, 1, p->counters.as<PartCounters>());
+  GX_LAUNCHED("node_update_kernel");
+  const PartCounters& c = read_counters(p);
+  if (changed) *changed = static_cast<int>(c.changed);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_fill(gx_part* p, int field, uint32_t begin, uint32_t end, uint32_t value) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_range(p, begin, end);
+  if (end > begin) {
+    fill_kernel_u32<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(field), begin, end, value);
+    GX_LAUNCHED("fill_kernel_u32");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_read(const gx_part* p, int field, uint32_t begin, uint32_t end, uint32_t* host) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_range(p, begin, end);
+  if (end > begin) {
+    check_ptr(host, "host");
+    GX_HIP(hipStreamSynchronize(p->stream));
+    GX_HIP(hipMemcpy(host, p->f(field) + begin, static_cast<size_t>(end - begin) * 4, hipMemcpyDeviceToHost));
+  }
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bitset_get(const gx_part* p, int field, uint64_t* words) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_ptr(words, "words");
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_HIP(hipMemcpy(words, p->bits[field].p, p->words() * 8, hipMemcpyDeviceToHost));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bitset_reset(gx_part* p, int field, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_field(field);
+  check_range(p, begin, end);
+  if (end > begin) {
+    const uint32_t nw = ((end - 1) >> 6) - (begin >> 6) + 1;
+    bitset_clear_kernel<<<grid_of(nw), kBlock, 0, p->stream>>>(p->b(field), begin, end);
+    GX_LAUNCHED("bitset_clear_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bfs_init(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity,
+                                      uint64_t src_gid) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    bfs_init_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_DIST_CURRENT), p->f(GX_FIELD_DIST_OLD), p->gid.p ? p->gid.as<uint32_t>() : nullptr, begin,
+        end, infinity, src_gid);
+    GX_LAUNCHED("bfs_init_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+static void bfs_edges(gx_part* p, uint32_t begin, uint32_t end, uint32_t priority, bool first) {
+  BfsOp op{p->col.as<int32_t>(), p->f(GX_FIELD_DIST_CURRENT), p->f(GX_FIELD_DIST_OLD), p->b(GX_FIELD_DIST_CURRENT),
+           p->counters.as<PartCounters>(), priority, first};
+  run_edges(p, begin, end, op);
+}
+
+extern "C" gx_status gx_part_bfs_first(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  bfs_edges(p, begin, end, 0, true);
+  read_counters(p);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bfs(gx_part* p, uint32_t begin, uint32_t end, uint32_t priority,
+                                 uint32_t* active_vertices, uint32_t* work_items) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  bfs_edges(p, begin, end, priority, false);
+  const PartCounters& c = read_counters(p);
+  if (active_vertices) *active_vertices = c.active;
+  if (work_items) *work_items = static_cast<uint32_t>(c.work);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bfs_check(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity,
+                                       uint64_t* sum, uint32_t* max_dist) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  if (end > begin) {
+    bfs_check_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_DIST_CURRENT), begin, end,
+                                                                     infinity, p->counters.as<PartCounters>());
+    GX_LAUNCHED("bfs_check_kernel");
+  }
+  const PartCounters& c = read_counters(p);
+  if (sum) *sum = c.sum;
+  if (max_dist) *max_dist = c.max;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_kcore_init1(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    kcore_init1_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_CURRENT_DEGREE), p->f(GX_FIELD_TRIM), p->f(GX_FIELD_FLAG), begin, end);
+    GX_LAUNCHED("kcore_init1_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+static void kcore_edges(gx_part* p, uint32_t begin, uint32_t end, uint32_t k, bool step) {
+  const int target = step ? GX_FIELD_TRIM : GX_FIELD_CURRENT_DEGREE;
+  KcoreOp op{p->col.as<int32_t>(), p->f(GX_FIELD_CURRENT_DEGREE), p->f(GX_FIELD_TRIM), p->f(GX_FIELD_FLAG),
+             p->f(target), p->b(target), p->counters.as<PartCounters>(), k, step};
+  run_edges(p, begin, end, op);
+}
+
+extern "C" gx_status gx_part_kcore_init2(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  kcore_edges(p, begin, end, 0, false);
+  read_counters(p);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_kcore_step1(gx_part* p, uint32_t begin, uint32_t end, uint32_t k,
+                                         uint32_t* active_vertices) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  kcore_edges(p, begin, end, k, true);
+  const PartCounters& c = read_counters(p);
+  if (active_vertices) *active_vertices = c.active;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_kcore_step2(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    kcore_step2_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_CURRENT_DEGREE), p->f(GX_FIELD_TRIM), p->f(GX_FIELD_FLAG), begin, end);
+    GX_LAUNCHED("kcore_step2_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_kcore_check(gx_part* p, uint32_t begin, uint32_t end, uint64_t* alive) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  if (end > begin) {
+    kcore_check_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_FLAG), begin, end,
+                                                                       p->counters.as<PartCounters>());
+    GX_LAUNCHED("kcore_check_kernel");
+  }
+  const PartCounters& c = read_counters(p);
+  if (alive) *alive = c.sum;
+  GX_API_END
+}

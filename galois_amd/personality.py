@@ -1,0 +1,177 @@
+"""D-IrGL personality partition (the gx_part_* entry points of the C ABI)
+and the host loops of the reference's D-IrGL apps for one host.
+
+The loops follow lonestar/analytics/distributed/bfs/bfs_push.cpp:95-300
+(InitializeGraph over all nodes, FirstItr_BFS on the source, BFS rounds over
+the nodes with edges until no vertex is active) and
+lonestar/analytics/distributed/k-core/kcore_push.cpp:93-318
+(InitializeGraph1 over all nodes, InitializeGraph2 over the nodes with edges,
+KCoreStep1 + KCoreStep2 rounds until no vertex dies).  With one host there is
+no gluon synchronisation between the operators; every other step is the
+reference's."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import check, lib
+
+DIST_CURRENT, DIST_OLD, CURRENT_DEGREE, TRIM, FLAG = range(5)
+RANGE_ALL, RANGE_MASTERS, RANGE_WITH_EDGES = range(3)
+INFINITY = 0xFFFFFFFF // 4  # bfs_push.cpp:72 (numeric_limits<uint32_t>::max() / 4)
+
+
+class Partition:
+    """One host's partition: the CSR of its local nodes (uint32 row_start and
+    edge_dst, as MarshalGraph), local-to-global ids, the master range and the
+    count of nodes with edges (default: up to the last node with out-edges)."""
+
+    def __init__(self, row, col, gid=None, num_owned=None, begin_master=0, num_with_edges=None, device=0):
+        h = C.c_void_p()
+        check(lib().gx_part_create(device, C.byref(h)))
+        self._h = h.value
+        row = np.ascontiguousarray(row, dtype=np.uint32)
+        col = np.ascontiguousarray(col, dtype=np.uint32)
+        self.n, self.m = len(row) - 1, len(col)
+        if num_owned is None:
+            num_owned = self.n - begin_master
+        if num_with_edges is None:
+            nz = np.nonzero(np.diff(row.astype(np.int64)))[0]
+            num_with_edges = int(nz[-1]) + 1 if len(nz) else 0
+        self.gid = None if gid is None else np.ascontiguousarray(gid, dtype=np.uint32)
+        check(lib().gx_part_load(self._h, self.n, self.m, row.ctypes.data, col.ctypes.data if self.m else None,
+                                 None if self.gid is None else self.gid.ctypes.data, num_owned, begin_master,
+                                 num_with_edges))
+
+    def free(self) -> None:
+        if getattr(self, "_h", None):
+            lib().gx_part_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001  (interpreter shutdown)
+            pass
+
+    # ------------------------------------------------------------- ranges
+    def range(self, which: int) -> tuple[int, int]:
+        b, e = C.c_uint32(), C.c_uint32()
+        check(lib().gx_part_range(self._h, which, C.byref(b), C.byref(e)))
+        return b.value, e.value
+
+    # ------------------------------------------------------------- fields
+    def get(self, field: int, lid: int) -> int:
+        v = C.c_uint32()
+        check(lib().gx_part_get(self._h, field, lid, C.byref(v)))
+        return v.value
+
+    def set(self, field: int, lid: int, value: int) -> None:
+        check(lib().gx_part_set(self._h, field, lid, value))
+
+    def add(self, field: int, lid: int, value: int) -> None:
+        check(lib().gx_part_add(self._h, field, lid, value))
+
+    def min(self, field: int, lid: int, value: int) -> bool:
+        changed = C.c_int()
+        check(lib().gx_part_min(self._h, field, lid, value, C.byref(changed)))
+        return bool(changed.value)
+
+    def fill(self, field: int, begin: int, end: int, value: int) -> None:
+        check(lib().gx_part_fill(self._h, field, begin, end, value))
+
+    def read(self, field: int, begin: int = 0, end: int | None = None) -> np.ndarray:
+        end = self.n if end is None else end
+        out = np.empty(max(end - begin, 1), dtype=np.uint32)
+        check(lib().gx_part_read(self._h, field, begin, end, out.ctypes.data))
+        return out[:end - begin]
+
+    def bitset(self, field: int) -> np.ndarray:
+        words = np.empty((self.n + 63) // 64, dtype=np.uint64)
+        check(lib().gx_part_bitset_get(self._h, field, words.ctypes.data))
+        return words
+
+    def bitset_reset(self, field: int, begin: int = 0, end: int | None = None) -> None:
+        check(lib().gx_part_bitset_reset(self._h, field, begin, self.n if end is None else end))
+
+    # ---------------------------------------------------------- operators
+    def bfs_init(self, begin: int, end: int, infinity: int, src_gid: int) -> None:
+        check(lib().gx_part_bfs_init(self._h, begin, end, infinity, src_gid))
+
+    def bfs_first(self, begin: int, end: int) -> None:
+        check(lib().gx_part_bfs_first(self._h, begin, end))
+
+    def bfs(self, begin: int, end: int, priority: int) -> tuple[int, int]:
+        a, w = C.c_uint32(), C.c_uint32()
+        check(lib().gx_part_bfs(self._h, begin, end, priority, C.byref(a), C.byref(w)))
+        return a.value, w.value
+
+    def bfs_check(self, begin: int, end: int, infinity: int) -> tuple[int, int]:
+        s, mx = C.c_uint64(), C.c_uint32()
+        check(lib().gx_part_bfs_check(self._h, begin, end, infinity, C.byref(s), C.byref(mx)))
+        return s.value, mx.value
+
+    def kcore_init1(self, begin: int, end: int) -> None:
+        check(lib().gx_part_kcore_init1(self._h, begin, end))
+
+    def kcore_init2(self, begin: int, end: int) -> None:
+        check(lib().gx_part_kcore_init2(self._h, begin, end))
+
+    def kcore_step1(self, begin: int, end: int, k: int) -> int:
+        a = C.c_uint32()
+        check(lib().gx_part_kcore_step1(self._h, begin, end, k, C.byref(a)))
+        return a.value
+
+    def kcore_step2(self, begin: int, end: int) -> None:
+        check(lib().gx_part_kcore_step2(self._h, begin, end))
+
+    def kcore_check(self, begin: int, end: int) -> int:
+        a = C.c_uint64()
+        check(lib().gx_part_kcore_check(self._h, begin, end, C.byref(a)))
+        return a.value
+
+    def local_id(self, gid: int) -> int | None:
+        if self.gid is None:
+            return gid if gid < self.n else None
+        hit = np.nonzero(self.gid == gid)[0]
+        return int(hit[0]) if len(hit) else None
+
+
+def dirgl_bfs(part: Partition, src_gid: int, delta: int = 0, max_iterations: int = 1000):
+    """bfs_push.cpp BFS::go for one host; returns (dist_current of all local
+    nodes, rounds, work items of the BFS rounds)."""
+    a0, a1 = part.range(RANGE_ALL)
+    w0, w1 = part.range(RANGE_WITH_EDGES)
+    part.bfs_init(a0, a1, INFINITY, src_gid)
+    lid = part.local_id(src_gid)
+    if lid is not None:
+        part.bfs_first(lid, lid + 1)
+    priority = 0xFFFFFFFF if delta == 0 else 0
+    rounds, work = 1, 0
+    while True:
+        priority = (priority + delta) & 0xFFFFFFFF
+        active, w = part.bfs(w0, w1, priority)
+        work += w
+        rounds += 1
+        if not (rounds < max_iterations and active):
+            break
+    return part.read(DIST_CURRENT), rounds, work
+
+
+def dirgl_kcore(part: Partition, k: int, max_iterations: int = 10000):
+    """kcore_push.cpp InitializeGraph1::go + KCoreStep1::go for one host;
+    returns (flag of all local nodes, alive masters, rounds)."""
+    a0, a1 = part.range(RANGE_ALL)
+    w0, w1 = part.range(RANGE_WITH_EDGES)
+    part.kcore_init1(a0, a1)
+    part.kcore_init2(w0, w1)
+    rounds = 0
+    while True:
+        active = part.kcore_step1(w0, w1, k)
+        part.kcore_step2(w0, w1)
+        rounds += 1
+        if not (rounds < max_iterations and active):
+            break
+    alive = part.kcore_check(*part.range(RANGE_MASTERS))
+    return part.read(FLAG), alive, rounds

@@ -299,6 +299,93 @@ gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
 gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_t count,
                         double* bc, gx_stats* st);
 
+/* ------------------------------------------------- D-IrGL personality
+ * The per-operator GPU kernels that D-IrGL host code launches on one host's
+ * partition (lonestar/analytics/distributed/bfs/bfs_push_cuda.h:103-140,
+ * k-core/kcore_push_cuda.h:133-167).  Node fields live on the device in a
+ * partition context; every operator runs over local ids [begin, end) and
+ * returns its reductions through out-parameters (the reference's
+ * DGAccumulator / DGReduceMax arguments).  The multi-host mirror exchange
+ * (batch_get/set/min_*_cuda over master/mirror lists) is gluon's and not part
+ * of this one-GPU build.  A partition is not thread-safe; each call returns
+ * after its work has finished. */
+typedef struct gx_part gx_part;
+
+/* get_CUDA_context + init_CUDA_context (libgluon/include/galois/cuda/
+ * HostDecls.h:98-99). */
+gx_status gx_part_create(int device, gx_part** out);
+/* reset_CUDA_context / context teardown (HostDecls.h:102). */
+gx_status gx_part_free(gx_part* p);
+/* load_graph_CUDA (HostDecls.h:100-101) from the MarshalGraph fields
+ * (HostDecls.h:38-54): host arrays row_start[n+1] and edge_dst[m] (uint32, the
+ * reference's index_type), node_gid[n] = global id of each local node (the
+ * MarshalGraph node_data; NULL = identity), the master range [begin_master,
+ * begin_master + num_owned) and numNodesWithEdges.  The arrays are validated
+ * (offsets monotone from 0 to m, destinations < n) before upload.  Fields and
+ * bitsets start at zero. */
+gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint32_t* row_start,
+                       const uint32_t* edge_dst, const uint32_t* node_gid, uint32_t num_owned,
+                       uint32_t begin_master, uint32_t num_nodes_with_edges);
+/* Ranges of the reference's _allNodes_cuda / _masterNodes_cuda /
+ * _nodesWithEdges_cuda variants. */
+#define GX_RANGE_ALL 0
+#define GX_RANGE_MASTERS 1
+#define GX_RANGE_WITH_EDGES 2
+gx_status gx_part_range(const gx_part* p, int which, uint32_t* begin, uint32_t* end);
+
+/* Node fields (uint32 each; flag is uint8 in the reference, 0/1 here). */
+#define GX_FIELD_DIST_CURRENT 0   /* bfs_push NodeData (bfs_push.cpp:74-77) */
+#define GX_FIELD_DIST_OLD 1
+#define GX_FIELD_CURRENT_DEGREE 2 /* kcore_push NodeData (kcore_push.cpp:69) */
+#define GX_FIELD_TRIM 3
+#define GX_FIELD_FLAG 4
+/* get/set/add/min_node_<field>_cuda (bfs_push_cuda.h:11-17); *changed = 1 when
+ * min lowered the value. */
+gx_status gx_part_get(const gx_part* p, int field, uint32_t lid, uint32_t* value);
+gx_status gx_part_set(gx_part* p, int field, uint32_t lid, uint32_t value);
+gx_status gx_part_add(gx_part* p, int field, uint32_t lid, uint32_t value);
+gx_status gx_part_min(gx_part* p, int field, uint32_t lid, uint32_t value, int* changed);
+/* batch_reset_node_<field>_cuda(ctx, begin, end, v) (bfs_push_cuda.h:55-56). */
+gx_status gx_part_fill(gx_part* p, int field, uint32_t begin, uint32_t end, uint32_t value);
+/* Field values of [begin, end) to a host array (the per-node get loop of
+ * the reference's output stage, bfs_push.cpp:375, in one copy). */
+gx_status gx_part_read(const gx_part* p, int field, uint32_t begin, uint32_t end, uint32_t* host);
+/* get_bitset_<field>_cuda (ceil(n/64) words to host) and
+ * bitset_<field>_reset_cuda, whole ([0, n)) or [begin, end)
+ * (bfs_push_cuda.h:6-10).  Operators set bits on the field they change:
+ * BFS on dist_current, InitializeGraph2 on current_degree, KCoreStep1 on trim. */
+gx_status gx_part_bitset_get(const gx_part* p, int field, uint64_t* words);
+gx_status gx_part_bitset_reset(gx_part* p, int field, uint32_t begin, uint32_t end);
+
+/* BFS push operators (bfs_push.cpp:95-330).
+ * InitializeGraph_cuda: dist_current = dist_old = (gid == src_gid ? 0 : infinity).
+ * FirstItr_BFS_cuda: every node of the range copies dist_current to dist_old
+ *   and pushes dist_current + 1 to its out-neighbours (atomic min, bitset).
+ * BFS_cuda: nodes with dist_old > dist_current count as active; those with
+ *   dist_current < priority also push; *work_items counts pushed edges.
+ * BFSSanityCheck_cuda: *sum = nodes with dist_current < infinity, *max_dist =
+ *   their largest distance. */
+gx_status gx_part_bfs_init(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity, uint64_t src_gid);
+gx_status gx_part_bfs_first(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_bfs(gx_part* p, uint32_t begin, uint32_t end, uint32_t priority,
+                      uint32_t* active_vertices, uint32_t* work_items);
+gx_status gx_part_bfs_check(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity, uint64_t* sum,
+                            uint32_t* max_dist);
+/* k-core push operators (kcore_push.cpp:93-360: InitializeGraph2, InitializeGraph1,
+ * KCoreStep2, KCoreStep1, KCoreSanityCheck).
+ * InitializeGraph1_cuda: flag = 1, trim = current_degree = 0.
+ * InitializeGraph2_cuda: each out-edge adds 1 to its destination's
+ *   current_degree (bitset current_degree).
+ * KCoreStep1_cuda: a node with flag and current_degree < k dies (flag = 0,
+ *   counted active) and adds 1 to each out-neighbour's trim (bitset trim).
+ * KCoreStep2_cuda: a live node subtracts its trim from current_degree; trim = 0.
+ * KCoreSanityCheck_cuda: *alive = nodes with flag set. */
+gx_status gx_part_kcore_init1(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_kcore_init2(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_kcore_step1(gx_part* p, uint32_t begin, uint32_t end, uint32_t k, uint32_t* active_vertices);
+gx_status gx_part_kcore_step2(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_kcore_check(gx_part* p, uint32_t begin, uint32_t end, uint64_t* alive);
+
 #ifdef __cplusplus
 }
 #endif

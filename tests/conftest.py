@@ -179,6 +179,15 @@ def golden() -> dict:
         return json.load(f)
 
 
+@pytest.fixture(scope="session")
+def rmat16(gx):
+    """Symmetric weighted RMAT-16 generated on the device, with its host CSR."""
+    print("generate rmat16 symmetric weighted", flush=True)
+    G = gx.Graph.rmat(16, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
+    row, col, w = G.to_host()
+    return G, row, col, w
+
+
 def csr_arrays(g: dict):
     row = np.asarray(g["row"], dtype=np.int32)
     col = np.asarray(g["col"], dtype=np.int32)

@@ -208,14 +208,6 @@ def test_grid_generator_properties(gx):
 
 
 # ------------------------------------------------------------- RMAT parity
-@pytest.fixture(scope="module")
-def rmat16(gx):
-    say("generate rmat16 symmetric weighted")
-    G = gx.Graph.rmat(16, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
-    row, col, w = G.to_host()
-    return G, row, col, w
-
-
 def test_rmat16_bfs_sssp(gx, oracle, rmat16):
     G, row, col, w = rmat16
     deg = np.diff(row)

@@ -1,0 +1,142 @@
+"""D-IrGL personality partition (gx_part_*, galois_amd.personality): the
+reference's D-IrGL host loops for one host (bfs_push.cpp, kcore_push.cpp)
+driven through the C ABI, against the oracle and the golden vectors: BFS
+distances bit-exact (with D-IrGL's infinity), k-core membership bit-exact,
+sanity-check reductions, split ranges, change bitsets and field accessors."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import csr_arrays
+
+pytestmark = pytest.mark.gpu
+
+ORACLE_INF = 2147483646  # BFS_SSSP.h:31
+
+
+def _part(row, col, **kw):
+    from galois_amd.personality import Partition
+    return Partition(np.asarray(row, np.uint32), np.asarray(col, np.uint32), **kw)
+
+
+def _expected_dist(levels):
+    from galois_amd.personality import INFINITY
+    lv = np.asarray(levels, np.uint32)
+    return np.where(lv == ORACLE_INF, INFINITY, lv).astype(np.uint32)
+
+
+def test_bfs_push_golden(gx, golden):
+    from galois_amd.personality import RANGE_MASTERS, dirgl_bfs, INFINITY
+    for g in golden["undirected"]:
+        row, col, _ = csr_arrays(g)
+        P = _part(row, col)
+        for s, exp in g["bfs"].items():
+            print(f"dirgl bfs {g['name']} src={s}", flush=True)
+            dist, rounds, work = dirgl_bfs(P, int(s))
+            np.testing.assert_array_equal(dist, _expected_dist(exp), err_msg=g["name"])
+            reached, mx = P.bfs_check(*P.range(RANGE_MASTERS), INFINITY)
+            e = np.asarray(exp)
+            assert reached == int((e != ORACLE_INF).sum()) and mx == int(e[e != ORACLE_INF].max())
+
+
+@pytest.mark.parametrize("delta", [0, 2])
+def test_bfs_push_rmat(gx, oracle, rmat16, delta):
+    """RMAT-16 from the max-degree vertex; delta > 0 defers vertices above the
+    round's priority (bfs_push.cpp:219-221) without changing the result."""
+    from galois_amd.personality import dirgl_bfs
+    G, row, col, _ = rmat16
+    s = int(np.argmax(np.diff(row)))
+    print(f"dirgl bfs rmat16 delta={delta}", flush=True)
+    P = _part(row, col)
+    dist, rounds, work = dirgl_bfs(P, s, delta=delta)
+    exp = oracle.bfs(row, col, s, threads=8)
+    np.testing.assert_array_equal(dist, _expected_dist(exp))
+    lv = exp[exp != ORACLE_INF]
+    assert rounds >= int(lv.max()) + 1
+    # every reached vertex with edges pushes once per lowering; at least its degree
+    assert work >= int(np.diff(row)[exp != ORACLE_INF].sum()) - int(np.diff(row)[s])
+
+
+def test_bfs_push_split_ranges_and_bitset(gx, oracle, rmat16):
+    """The operators over three sub-ranges per round give the full-range
+    result; FirstItr marks exactly the source's out-neighbours in the
+    dist_current bitset, and a range reset clears only that range."""
+    from galois_amd.personality import (DIST_CURRENT, DIST_OLD, INFINITY, RANGE_ALL, RANGE_WITH_EDGES)
+    G, row, col, _ = rmat16
+    s = int(np.argmax(np.diff(row)))
+    print("dirgl bfs split ranges", flush=True)
+    P = _part(row, col)
+    a0, a1 = P.range(RANGE_ALL)
+    w0, w1 = P.range(RANGE_WITH_EDGES)
+    P.bfs_init(a0, a1, INFINITY, s)
+    assert P.get(DIST_CURRENT, s) == 0 and P.get(DIST_OLD, s) == 0
+    P.bfs_first(s, s + 1)
+    words = P.bitset(DIST_CURRENT)
+    bits = np.unpackbits(words.view(np.uint8), bitorder="little")[:P.n].astype(bool)
+    nbrs = np.zeros(P.n, bool)
+    nbrs[col[row[s]:row[s + 1]]] = True
+    np.testing.assert_array_equal(bits, nbrs)
+    lo, hi = 1000, 40000
+    P.bitset_reset(DIST_CURRENT, lo, hi)
+    bits2 = np.unpackbits(P.bitset(DIST_CURRENT).view(np.uint8), bitorder="little")[:P.n].astype(bool)
+    keep = nbrs.copy()
+    keep[lo:hi] = False
+    np.testing.assert_array_equal(bits2, keep)
+    cuts = [w0, (w0 + w1) // 3, 2 * (w0 + w1) // 3, w1]
+    for _ in range(1000):
+        active = sum(P.bfs(cuts[i], cuts[i + 1], 0xFFFFFFFF)[0] for i in range(3))
+        if not active:
+            break
+    assert not active
+    np.testing.assert_array_equal(P.read(DIST_CURRENT), _expected_dist(oracle.bfs(row, col, s, threads=8)))
+
+
+def test_kcore_push(gx, golden, oracle, rmat16):
+    """k-core membership (flag) against the oracle's peeling (alive iff final
+    degree >= k) and the golden vectors; nodes past the last node with
+    edges are never visited by KCoreStep1 and keep their flag, as in the
+    reference; KCoreSanityCheck counts the masters with the flag set."""
+    from galois_amd.personality import CURRENT_DEGREE, dirgl_kcore
+    cases = [(g["name"], *csr_arrays(g)[:2], int(k), np.asarray(e["alive"], bool))
+             for g in golden["undirected"] for k, e in g["kcore"].items()]
+    G, row, col, _ = rmat16
+    for k in (2, 10, 40):
+        cases.append((f"rmat16 k={k}", row, col, k, oracle.kcore(row, col, k, threads=8) >= k))
+    for name, row, col, k, alive in cases:
+        print(f"dirgl kcore {name} k={k}", flush=True)
+        P = _part(row, col)
+        flag, count, rounds = dirgl_kcore(P, k)
+        deg = np.diff(np.asarray(row, np.int64))
+        nz = np.nonzero(deg)[0]
+        with_edges = int(nz[-1]) + 1 if len(nz) else 0
+        exp = alive.copy()
+        exp[with_edges:] = True
+        np.testing.assert_array_equal(flag.astype(bool), exp, err_msg=name)
+        assert count == int(exp.sum())
+        # survivors keep a degree of at least k counted over live neighbours
+        cur = P.read(CURRENT_DEGREE)
+        assert np.all(cur[:with_edges][exp[:with_edges]] >= k), name
+
+
+def test_field_accessors_and_validation(gx):
+    from galois_amd._lib import GxError
+    from galois_amd.personality import DIST_CURRENT, RANGE_MASTERS, TRIM
+    row = np.array([0, 2, 3, 3, 4], np.uint32)
+    col = np.array([1, 2, 3, 0], np.uint32)
+    P = _part(row, col, num_owned=3, begin_master=1)
+    assert P.range(RANGE_MASTERS) == (1, 4)
+    P.fill(TRIM, 0, 4, 7)
+    P.add(TRIM, 2, 5)
+    assert list(P.read(TRIM)) == [7, 7, 12, 7]
+    P.set(DIST_CURRENT, 3, 9)
+    assert P.min(DIST_CURRENT, 3, 4) and not P.min(DIST_CURRENT, 3, 6)
+    assert P.get(DIST_CURRENT, 3) == 4
+    with pytest.raises(GxError):
+        P.get(DIST_CURRENT, 4)  # outside the partition
+    with pytest.raises(GxError):
+        P.bfs(0, 5, 1)
+    with pytest.raises(GxError):
+        _part(row, np.array([1, 2, 9, 0], np.uint32))  # destination out of range
+    with pytest.raises(GxError):
+        _part(np.array([0, 2, 1, 3, 4], np.uint32), col)  # offsets not monotone
