@@ -1,25 +1,28 @@
 #!/usr/bin/env python3
 """Benchmark: lonestar analytics on one MI355X.
 
-Headline (`value`): PageRank pull, FP32 with Kahan-compensated row sums, on a
-directed RMAT scale-26 graph (67,108,864 vertices, edge factor 16,
-a/b/c/d = .57/.19/.19/.05, seeded, self-loops and duplicate edges removed;
-CSR + transposed CSC in HBM), damping 0.85, uniform teleport, dangling mass
-redistributed, stop at L1 residual < 1e-9 or 100 iterations.  One step = one
-full PageRank solve; value = GTEPS = edges x iterations / time.
+Headline (`value`): PageRank, FP32, on a directed RMAT scale-26 graph
+(67,108,864 vertices, edge factor 16, a/b/c/d = .57/.19/.19/.05, seeded,
+self-loops and duplicate edges removed), damping 0.85, uniform teleport,
+dangling mass redistributed, stop at L1 residual < 1e-9 or 100 iterations,
+through the default propagation-blocked sweep (exact fixed-point row sums).
+One step = one full PageRank solve; value = GTEPS = edges x iterations / time.
 
 Also reported, one line of JSON in total:
-  roofline      the dominant kernel (the pull sweep) against the 8 TB/s HBM
-                peak, algorithmic bytes 4m + 24n + 4 per sweep, timed with HIP
-                events on the library's stream during the timed steps;
+  roofline      the dominant kernel (one sweep: phase 1 + phase 2 + sliced-bin
+                finalisation) against the 8 TB/s HBM peak, algorithmic bytes
+                12m + 20n per sweep, timed with HIP events on the library's
+                stream during the timed steps; measured HBM traffic from the
+                committed rocprofv3 summary;
   cpu_baseline  the CPU restatement (oracle, FP64 pull) timed on this host's
                 cores on a bounded sample (a few PageRank iterations on the
                 same graph);
   kernels       GTEPS and %HBM of every kernel on the other workloads (BFS on
                 symmetric RMAT-18; SSSP/CC/k-core on symmetric RMAT-22 and the
                 2048x2048 road-like grid; batched vs one-source-per-pass BC;
-                adjoint vs forward PageRank), plus the FP32-vs-FP64 PageRank
-                error.
+                adjoint vs forward PageRank; the D-IrGL personality loops;
+                BFS/SSSP/CC/k-core on symmetric RMAT-26), plus the
+                FP32-vs-FP64 PageRank error.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -165,6 +168,7 @@ def kernel_table(gx, quick: bool) -> dict:
                                         "TEPS_G": round(m / per_src_plain / 1e6, 3)}
             out[f"bc_speedup_batched_vs_plain_{gname}"] = round(per_src_plain * nsrc / stb.elapsed_ms, 2)
             out[f"bc_max_rel_diff_plain_vs_batched_{gname}"] = rel
+            out.update(personality_rows(gx, row, col, gname))
         del G
     # Config 4b: one batch of 64 uneven graphs (RMAT scales 12-18, edge
     # factors 8-32 from the seed), 4 seeded sources per graph, batched BC over
@@ -249,6 +253,47 @@ def kernel_table(gx, quick: bool) -> dict:
     del G
     if not quick:
         out.update(rmat26_table(gx))
+    return out
+
+
+def personality_rows(gx, row, col, gname) -> dict:
+    """The D-IrGL host loops of one host through the personality partition
+    (gx_part_*): BFS push from the max-degree vertex and k-core push (k=10),
+    timed around the whole loop (every operator returns its reductions to the
+    host, as in the reference); the partition upload is not timed.  Results
+    are checked against the library's own BFS levels and k-core membership."""
+    from galois_amd import personality as pers
+    out = {}
+    n, m = len(row) - 1, len(col)
+    P = pers.Partition(row, col)
+    src = int(np.argmax(np.diff(row)))
+    G = gx.Graph.from_csr(row, col, None, symmetric=True)
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        dist, rounds, work = pers.dirgl_bfs(P, src)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    lv = gx.bfs(G, src)[0].cpu().numpy().view(np.uint32)
+    same = bool(np.array_equal(np.where(lv == gx.GX_DIST_INFINITY, pers.INFINITY, lv), dist))
+    out[f"dirgl_bfs_push_{gname}"] = {"ms": round(best * 1e3, 3), "GTEPS": round(m / best / 1e9, 3), "rounds": rounds,
+                                      "work_items": work, "levels_equal_gx_bfs": same}
+    best = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        flag, alive, rounds = pers.dirgl_kcore(P, 10)
+        dt = time.perf_counter() - t0
+        best = dt if best is None else min(best, dt)
+    # Nodes past the last node with edges are not visited by KCoreStep1 and
+    # keep their flag (the reference's D-IrGL semantics).
+    we = P.range(pers.RANGE_WITH_EDGES)[1]
+    _, kalive, kcount, _ = gx.kcore(G, 10)
+    ka = kalive.cpu().numpy().astype(bool)
+    same = (bool(np.array_equal(flag[:we].astype(bool), ka[:we])) and bool(flag[we:].all())
+            and alive == int(ka[:we].sum()) + (n - we))
+    out[f"dirgl_kcore10_push_{gname}"] = {"ms": round(best * 1e3, 3), "GTEPS": round(m / best / 1e9, 3),
+                                          "rounds": rounds, "alive": alive, "membership_equal_gx_kcore": same}
+    P.free()
     return out
 
 
