@@ -125,7 +125,7 @@ def kernel_table(gx, quick: bool) -> dict:
                   ("grid256", lambda: gx.Graph.grid(256, 50000, 10000, SEED, 1, 1000))]
     for gname, make in graphs:
         G = make()
-        row, col, _ = G.to_host()
+        row, col, wts = G.to_host()
         n, m = G.n, G.m
         deg = np.diff(row).astype(np.int64)
         runs = {"sssp": lambda: gx.sssp(G, 0, 6), "cc": lambda: gx.connected_components(G),
@@ -168,7 +168,7 @@ def kernel_table(gx, quick: bool) -> dict:
                                         "TEPS_G": round(m / per_src_plain / 1e6, 3)}
             out[f"bc_speedup_batched_vs_plain_{gname}"] = round(per_src_plain * nsrc / stb.elapsed_ms, 2)
             out[f"bc_max_rel_diff_plain_vs_batched_{gname}"] = rel
-            out.update(personality_rows(gx, row, col, gname))
+            out.update(personality_rows(gx, row, col, wts, gname))
         del G
     # Config 4b: one batch of 64 uneven graphs (RMAT scales 12-18, edge
     # factors 8-32 from the seed), 4 seeded sources per graph, batched BC over
@@ -256,18 +256,38 @@ def kernel_table(gx, quick: bool) -> dict:
     return out
 
 
-def personality_rows(gx, row, col, gname) -> dict:
+def personality_rows(gx, row, col, wts, gname) -> dict:
     """The D-IrGL host loops of one host through the personality partition
-    (gx_part_*): BFS push from the max-degree vertex and k-core push (k=10),
-    timed around the whole loop (every operator returns its reductions to the
-    host, as in the reference); the partition upload is not timed.  Results
-    are checked against the library's own BFS levels and k-core membership."""
+    (gx_part_*): BFS push from the max-degree vertex, SSSP push from vertex 0,
+    CC push and k-core push (k=10), timed around the whole loop (every
+    operator returns its reductions to the host, as in the reference); the
+    partition upload is not timed.  Results are checked against the library's
+    own BFS levels, SSSP distances, CC labels and k-core membership."""
     from galois_amd import personality as pers
     out = {}
     n, m = len(row) - 1, len(col)
-    P = pers.Partition(row, col)
+    P = pers.Partition(row, col, weights=wts)
     src = int(np.argmax(np.diff(row)))
-    G = gx.Graph.from_csr(row, col, None, symmetric=True)
+    G = gx.Graph.from_csr(row, col, wts, symmetric=True)
+
+    def timed(fn):
+        best, res = None, None
+        for _ in range(3):
+            t0 = time.perf_counter()
+            res = fn()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best, res
+
+    dt, (dist, rounds, work) = timed(lambda: pers.dirgl_sssp(P, 0))
+    d = gx.sssp(G, 0, 6)[0].cpu().numpy().view(np.uint32)
+    same = bool(np.array_equal(np.where(d == gx.GX_DIST_INFINITY, pers.INFINITY, d), dist))
+    out[f"dirgl_sssp_push_{gname}"] = {"ms": round(dt * 1e3, 3), "GTEPS": round(m / dt / 1e9, 3), "rounds": rounds,
+                                       "work_edges": work, "dist_equal_gx_sssp": same}
+    dt, (label, comps, rounds) = timed(lambda: pers.dirgl_cc(P))
+    lab = gx.connected_components(G)[0].cpu().numpy().view(np.uint32)
+    out[f"dirgl_cc_push_{gname}"] = {"ms": round(dt * 1e3, 3), "GTEPS": round(m / dt / 1e9, 3), "rounds": rounds,
+                                     "components": comps, "labels_equal_gx_cc": bool(np.array_equal(lab, label))}
     best = None
     for _ in range(3):
         t0 = time.perf_counter()

@@ -112,7 +112,7 @@ SIGNATURES: dict[str, tuple] = {
     # D-IrGL personality partition
     "gx_part_create": (INT, [INT, PG]),
     "gx_part_free": (INT, [P]),
-    "gx_part_load": (INT, [P, U64, U64, P, P, P, U32, U32, U32]),
+    "gx_part_load": (INT, [P, U64, U64, P, P, P, P, U32, U32, U32]),
     "gx_part_range": (INT, [P, INT, C.POINTER(U32), C.POINTER(U32)]),
     "gx_part_get": (INT, [P, INT, U32, C.POINTER(U32)]),
     "gx_part_set": (INT, [P, INT, U32, U32]),
@@ -126,6 +126,14 @@ SIGNATURES: dict[str, tuple] = {
     "gx_part_bfs_first": (INT, [P, U32, U32]),
     "gx_part_bfs": (INT, [P, U32, U32, U32, C.POINTER(U32), C.POINTER(U32)]),
     "gx_part_bfs_check": (INT, [P, U32, U32, U32, C.POINTER(U64), C.POINTER(U32)]),
+    "gx_part_sssp_init": (INT, [P, U32, U32, U32, U64]),
+    "gx_part_sssp_first": (INT, [P, U32, U32]),
+    "gx_part_sssp": (INT, [P, U32, U32, U32, C.POINTER(U32), C.POINTER(U32)]),
+    "gx_part_sssp_check": (INT, [P, U32, U32, U32, C.POINTER(U64), C.POINTER(U64), C.POINTER(U32)]),
+    "gx_part_cc_init": (INT, [P, U32, U32]),
+    "gx_part_cc_first": (INT, [P, U32, U32]),
+    "gx_part_cc": (INT, [P, U32, U32, C.POINTER(U32)]),
+    "gx_part_cc_check": (INT, [P, U32, U32, C.POINTER(U64)]),
     "gx_part_kcore_init1": (INT, [P, U32, U32]),
     "gx_part_kcore_init2": (INT, [P, U32, U32]),
     "gx_part_kcore_step1": (INT, [P, U32, U32, U32, C.POINTER(U32)]),

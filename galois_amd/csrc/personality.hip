@@ -7,6 +7,11 @@
 //  - BFS push operators InitializeGraph, FirstItr_BFS, BFS, BFSSanityCheck
 //    (lonestar/analytics/distributed/bfs/bfs_push.cpp:95-340, declared in
 //    bfs_push_cuda.h:103-140) over the fields dist_current / dist_old;
+//  - SSSP push (sssp/sssp_push.cpp:95-360, sssp_push_cuda.h:103-128): BFS
+//    push with dist_current + edge weight, and a distance sum in the check;
+//  - CC push (connected-components/cc_push.cpp:85-300, cc_push_cuda.h:
+//    103-130) over comp_current / comp_old: labels start at the global id,
+//    every node with a lowered label pushes it, active counted per edge;
 //  - k-core push operators InitializeGraph1/2, KCoreStep1/2, KCoreSanityCheck
 //    (lonestar/analytics/distributed/k-core/kcore_push.cpp:93-360, declared
 //    in kcore_push_cuda.h:133-167) over current_degree / trim / flag;
@@ -36,7 +41,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxGrid = 2048;
-constexpr int kFields = 5;
+constexpr int kFields = 7;
 
 // Device counters of one operator launch.
 struct PartCounters {
@@ -46,6 +51,7 @@ struct PartCounters {
   unsigned changed;
   unsigned long long work;
   unsigned long long sum;
+  unsigned long long dsum;
   unsigned max;
   unsigned pad;
 };
@@ -58,7 +64,7 @@ struct gx_part {
   int64_t n = 0, m = 0;
   uint32_t num_owned = 0, begin_master = 0, with_edges = 0;
   bool loaded = false;
-  gx::DevBuf row, col, gid, field[kFields], bits[kFields], hubq, hub_off, counters;
+  gx::DevBuf row, col, w, gid, field[kFields], bits[kFields], hubq, hub_off, counters;
   PartCounters* host = nullptr;  // pinned
   ~gx_part() {
     if (host) (void)hipHostFree(host);
@@ -137,18 +143,26 @@ __global__ void bfs_init_kernel(uint32_t* __restrict__ cur, uint32_t* __restrict
   }
 }
 
-// FirstItr_BFS / BFS (bfs_push.cpp:170-181, 254-273): an admitted vertex
-// copies dist_current into dist_old and pushes dist_current + 1 to every
-// out-neighbour with atomicMin, marking lowered neighbours in the bitset.
-// BFS admits vertices with dist_old > dist_current (counted active) whose
-// dist_current is below the priority; FirstItr_BFS admits its whole range.
-struct BfsOp {
+// The min-push operators of BFS, SSSP and CC push.  FirstItr_* (bfs_push.cpp:
+// 170-181, sssp_push.cpp:173-185, cc_push.cpp:152-164) admits its whole
+// range; BFS / SSSP (bfs_push.cpp:254-273, sssp_push.cpp:265-286) admit
+// nodes with dist_old > dist_current (counted active) whose dist_current is
+// below the priority; ConnectedComp (cc_push.cpp:229-246) admits nodes with
+// comp_old > comp_current and counts one active per pushed edge.  An admitted
+// node copies its current value into old and pushes current + 1 (BFS),
+// current + weight (SSSP) or current (CC) to every out-neighbour with
+// atomicMin, marking lowered neighbours in the current field's bitset.
+enum PushKind : int { kPushBfs = 0, kPushSssp = 1, kPushCc = 2 };
+
+struct MinPushOp {
   const int32_t* __restrict__ col;
+  const uint32_t* __restrict__ w;
   uint32_t* cur;
   uint32_t* old;
   unsigned long long* bits;
   PartCounters* c;
   uint32_t priority;
+  int kind;
   bool first;
   unsigned active = 0;
   unsigned long long work = 0;
@@ -159,6 +173,10 @@ struct BfsOp {
       return true;
     }
     if (old[v] > dc) {
+      if (kind == kPushCc) {
+        old[v] = dc;
+        return true;
+      }
       ++active;
       if (priority > dc) {
         old[v] = dc;
@@ -170,33 +188,65 @@ struct BfsOp {
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     if (!valid) return;
     const uint32_t v = static_cast<uint32_t>(col[e]);
-    const uint32_t nd = 1u + cur[u];
+    const uint32_t base = cur[u];
+    const uint32_t nd = kind == kPushBfs ? base + 1u : (kind == kPushSssp ? base + w[e] : base);
     const uint32_t prev = atomicMin(&cur[v], nd);
     if (prev > nd) set_bit(bits, v);
-    ++work;
+    if (!first) {
+      if (kind == kPushCc)
+        ++active;
+      else
+        ++work;
+    }
   }
   __device__ __forceinline__ void after_step() {}
-  __device__ __forceinline__ void finish() { flush_counts(c, active, first ? 0ull : work); }
+  __device__ __forceinline__ void finish() { flush_counts(c, active, work); }
 };
 
-// BFSSanityCheck (bfs_push.cpp:323-330): reached count and maximum distance.
-__global__ void __launch_bounds__(kBlock) bfs_check_kernel(const uint32_t* __restrict__ cur, uint32_t begin,
-                                                           uint32_t end, uint32_t inf, PartCounters* c) {
-  unsigned long long cnt = 0;
+// BFSSanityCheck / SSSPSanityCheck (bfs_push.cpp:323-330, sssp_push.cpp:
+// 351-358): reached count, distance sum and maximum distance.
+__global__ void __launch_bounds__(kBlock) dist_check_kernel(const uint32_t* __restrict__ cur, uint32_t begin,
+                                                            uint32_t end, uint32_t inf, PartCounters* c) {
+  unsigned long long cnt = 0, dsum = 0;
   uint32_t mx = 0;
   for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
     const uint32_t d = cur[v];
     if (d < inf) {
       ++cnt;
+      dsum += d;
       mx = max(mx, d);
     }
   }
   cnt = wave_sum(cnt);
+  dsum = wave_sum(dsum);
   mx = wave_max(mx);
   if (lane_id() == 0) {
     if (cnt) atomicAdd(&c->sum, cnt);
+    if (dsum) atomicAdd(&c->dsum, dsum);
     atomicMax(&c->max, mx);
   }
+}
+
+// CC InitializeGraph (cc_push.cpp:109-113): both labels = the global id.
+__global__ void cc_init_kernel(uint32_t* __restrict__ cur, uint32_t* __restrict__ old,
+                               const uint32_t* __restrict__ gid, uint32_t begin, uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const uint32_t g = gid ? gid[v] : v;
+    cur[v] = g;
+    old[v] = g;
+  }
+}
+
+// ConnectedCompSanityCheck (cc_push.cpp:290-296): nodes whose label is their
+// own global id (one per component).
+__global__ void __launch_bounds__(kBlock) cc_check_kernel(const uint32_t* __restrict__ cur,
+                                                          const uint32_t* __restrict__ gid, uint32_t begin,
+                                                          uint32_t end, PartCounters* c) {
+  unsigned long long cnt = 0;
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x)
+    cnt += cur[v] == (gid ? gid[v] : v) ? 1 : 0;
+  cnt = wave_sum(cnt);
+  if (lane_id() == 0 && cnt) atomicAdd(&c->sum, cnt);
 }
 
 // ------------------------------------------------------------- k-core push
@@ -350,8 +400,8 @@ extern "C" gx_status gx_part_free(gx_part* p) {
 }
 
 extern "C" gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint32_t* row_start,
-                                  const uint32_t* edge_dst, const uint32_t* node_gid, uint32_t num_owned,
-                                  uint32_t begin_master, uint32_t num_nodes_with_edges) {
+                                  const uint32_t* edge_dst, const uint32_t* edge_data, const uint32_t* node_gid,
+                                  uint32_t num_owned, uint32_t begin_master, uint32_t num_nodes_with_edges) {
   GX_API_BEGIN
   GX_REQUIRE(p != nullptr, "partition handle is NULL");
   check_ptr(row_start, "row_start");
@@ -376,6 +426,12 @@ extern "C" gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint
   p->col.ensure(std::max<uint64_t>(m, 1) * 4);
   GX_HIP(hipMemcpy(p->row.p, row_start, (n + 1) * 4, hipMemcpyHostToDevice));
   if (m) GX_HIP(hipMemcpy(p->col.p, edge_dst, m * 4, hipMemcpyHostToDevice));
+  if (edge_data && m) {
+    p->w.ensure(m * 4);
+    GX_HIP(hipMemcpy(p->w.p, edge_data, m * 4, hipMemcpyHostToDevice));
+  } else {
+    p->w.release();
+  }
   if (node_gid) {
     p->gid.ensure(n * 4);
     GX_HIP(hipMemcpy(p->gid.p, node_gid, n * 4, hipMemcpyHostToDevice));
@@ -504,11 +560,17 @@ extern "C" gx_status gx_part_bitset_reset(gx_part* p, int field, uint32_t begin,
   GX_API_END
 }
 
-extern "C" gx_status gx_part_bfs_init(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity,
-                                      uint64_t src_gid) {
-  GX_API_BEGIN
-  check_part(p);
-  check_range(p, begin, end);
+static void push_edges(gx_part* p, uint32_t begin, uint32_t end, int kind, uint32_t priority, bool first) {
+  const bool cc = kind == kPushCc;
+  const int cur = cc ? GX_FIELD_COMP_CURRENT : GX_FIELD_DIST_CURRENT;
+  const int old = cc ? GX_FIELD_COMP_OLD : GX_FIELD_DIST_OLD;
+  if (kind == kPushSssp) GX_REQUIRE(p->w.p != nullptr, "SSSP push needs edge data (gx_part_load edge_data)");
+  MinPushOp op{p->col.as<int32_t>(), p->w.as<uint32_t>(), p->f(cur), p->f(old), p->b(cur),
+               p->counters.as<PartCounters>(), priority, kind, first};
+  run_edges(p, begin, end, op);
+}
+
+static void dist_init(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity, uint64_t src_gid) {
   if (end > begin) {
     bfs_init_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
         p->f(GX_FIELD_DIST_CURRENT), p->f(GX_FIELD_DIST_OLD), p->gid.p ? p->gid.as<uint32_t>() : nullptr, begin,
@@ -516,13 +578,25 @@ extern "C" gx_status gx_part_bfs_init(gx_part* p, uint32_t begin, uint32_t end, 
     GX_LAUNCHED("bfs_init_kernel");
   }
   GX_HIP(hipStreamSynchronize(p->stream));
-  GX_API_END
 }
 
-static void bfs_edges(gx_part* p, uint32_t begin, uint32_t end, uint32_t priority, bool first) {
-  BfsOp op{p->col.as<int32_t>(), p->f(GX_FIELD_DIST_CURRENT), p->f(GX_FIELD_DIST_OLD), p->b(GX_FIELD_DIST_CURRENT),
-           p->counters.as<PartCounters>(), priority, first};
-  run_edges(p, begin, end, op);
+static const PartCounters& dist_check(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity) {
+  reset_counters(p);
+  if (end > begin) {
+    dist_check_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_DIST_CURRENT), begin, end,
+                                                                      infinity, p->counters.as<PartCounters>());
+    GX_LAUNCHED("dist_check_kernel");
+  }
+  return read_counters(p);
+}
+
+extern "C" gx_status gx_part_bfs_init(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity,
+                                      uint64_t src_gid) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  dist_init(p, begin, end, infinity, src_gid);
+  GX_API_END
 }
 
 extern "C" gx_status gx_part_bfs_first(gx_part* p, uint32_t begin, uint32_t end) {
@@ -530,7 +604,7 @@ extern "C" gx_status gx_part_bfs_first(gx_part* p, uint32_t begin, uint32_t end)
   check_part(p);
   check_range(p, begin, end);
   reset_counters(p);
-  bfs_edges(p, begin, end, 0, true);
+  push_edges(p, begin, end, kPushBfs, 0, true);
   read_counters(p);
   GX_API_END
 }
@@ -541,7 +615,7 @@ extern "C" gx_status gx_part_bfs(gx_part* p, uint32_t begin, uint32_t end, uint3
   check_part(p);
   check_range(p, begin, end);
   reset_counters(p);
-  bfs_edges(p, begin, end, priority, false);
+  push_edges(p, begin, end, kPushBfs, priority, false);
   const PartCounters& c = read_counters(p);
   if (active_vertices) *active_vertices = c.active;
   if (work_items) *work_items = static_cast<uint32_t>(c.work);
@@ -553,15 +627,104 @@ extern "C" gx_status gx_part_bfs_check(gx_part* p, uint32_t begin, uint32_t end,
   GX_API_BEGIN
   check_part(p);
   check_range(p, begin, end);
-  reset_counters(p);
-  if (end > begin) {
-    bfs_check_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_DIST_CURRENT), begin, end,
-                                                                     infinity, p->counters.as<PartCounters>());
-    GX_LAUNCHED("bfs_check_kernel");
-  }
-  const PartCounters& c = read_counters(p);
+  const PartCounters& c = dist_check(p, begin, end, infinity);
   if (sum) *sum = c.sum;
   if (max_dist) *max_dist = c.max;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_sssp_init(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity,
+                                       uint64_t src_gid) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  dist_init(p, begin, end, infinity, src_gid);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_sssp_first(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  push_edges(p, begin, end, kPushSssp, 0, true);
+  read_counters(p);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_sssp(gx_part* p, uint32_t begin, uint32_t end, uint32_t priority,
+                                  uint32_t* active_vertices, uint32_t* work_edges) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  push_edges(p, begin, end, kPushSssp, priority, false);
+  const PartCounters& c = read_counters(p);
+  if (active_vertices) *active_vertices = c.active;
+  if (work_edges) *work_edges = static_cast<uint32_t>(c.work);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_sssp_check(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity,
+                                        uint64_t* sum, uint64_t* dist_sum, uint32_t* max_dist) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  const PartCounters& c = dist_check(p, begin, end, infinity);
+  if (sum) *sum = c.sum;
+  if (dist_sum) *dist_sum = c.dsum;
+  if (max_dist) *max_dist = c.max;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_cc_init(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    cc_init_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_COMP_CURRENT), p->f(GX_FIELD_COMP_OLD), p->gid.p ? p->gid.as<uint32_t>() : nullptr, begin,
+        end);
+    GX_LAUNCHED("cc_init_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_cc_first(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  push_edges(p, begin, end, kPushCc, 0, true);
+  read_counters(p);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_cc(gx_part* p, uint32_t begin, uint32_t end, uint32_t* active_vertices) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  push_edges(p, begin, end, kPushCc, 0, false);
+  const PartCounters& c = read_counters(p);
+  if (active_vertices) *active_vertices = c.active;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_cc_check(gx_part* p, uint32_t begin, uint32_t end, uint64_t* components) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  if (end > begin) {
+    cc_check_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_COMP_CURRENT), p->gid.p ? p->gid.as<uint32_t>() : nullptr, begin, end,
+        p->counters.as<PartCounters>());
+    GX_LAUNCHED("cc_check_kernel");
+  }
+  const PartCounters& c = read_counters(p);
+  if (components) *components = c.sum;
   GX_API_END
 }
 

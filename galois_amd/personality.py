@@ -3,7 +3,9 @@ and the host loops of the reference's D-IrGL apps for one host.
 
 The loops follow lonestar/analytics/distributed/bfs/bfs_push.cpp:95-300
 (InitializeGraph over all nodes, FirstItr_BFS on the source, BFS rounds over
-the nodes with edges until no vertex is active) and
+the nodes with edges until no vertex is active), the same structure in
+sssp/sssp_push.cpp:95-300 (weights pushed) and connected-components/
+cc_push.cpp:85-230 (labels pushed, FirstItr over the nodes with edges), and
 lonestar/analytics/distributed/k-core/kcore_push.cpp:93-318
 (InitializeGraph1 over all nodes, InitializeGraph2 over the nodes with edges,
 KCoreStep1 + KCoreStep2 rounds until no vertex dies).  With one host there is
@@ -17,7 +19,7 @@ import numpy as np
 
 from ._lib import check, lib
 
-DIST_CURRENT, DIST_OLD, CURRENT_DEGREE, TRIM, FLAG = range(5)
+DIST_CURRENT, DIST_OLD, CURRENT_DEGREE, TRIM, FLAG, COMP_CURRENT, COMP_OLD = range(7)
 RANGE_ALL, RANGE_MASTERS, RANGE_WITH_EDGES = range(3)
 INFINITY = 0xFFFFFFFF // 4  # bfs_push.cpp:72 (numeric_limits<uint32_t>::max() / 4)
 
@@ -27,7 +29,8 @@ class Partition:
     edge_dst, as MarshalGraph), local-to-global ids, the master range and the
     count of nodes with edges (default: up to the last node with out-edges)."""
 
-    def __init__(self, row, col, gid=None, num_owned=None, begin_master=0, num_with_edges=None, device=0):
+    def __init__(self, row, col, gid=None, num_owned=None, begin_master=0, num_with_edges=None, weights=None,
+                 device=0):
         h = C.c_void_p()
         check(lib().gx_part_create(device, C.byref(h)))
         self._h = h.value
@@ -40,7 +43,9 @@ class Partition:
             nz = np.nonzero(np.diff(row.astype(np.int64)))[0]
             num_with_edges = int(nz[-1]) + 1 if len(nz) else 0
         self.gid = None if gid is None else np.ascontiguousarray(gid, dtype=np.uint32)
+        w = None if weights is None else np.ascontiguousarray(weights, dtype=np.uint32)
         check(lib().gx_part_load(self._h, self.n, self.m, row.ctypes.data, col.ctypes.data if self.m else None,
+                                 None if w is None or not self.m else w.ctypes.data,
                                  None if self.gid is None else self.gid.ctypes.data, num_owned, begin_master,
                                  num_with_edges))
 
@@ -112,6 +117,38 @@ class Partition:
         check(lib().gx_part_bfs_check(self._h, begin, end, infinity, C.byref(s), C.byref(mx)))
         return s.value, mx.value
 
+    def sssp_init(self, begin: int, end: int, infinity: int, src_gid: int) -> None:
+        check(lib().gx_part_sssp_init(self._h, begin, end, infinity, src_gid))
+
+    def sssp_first(self, begin: int, end: int) -> None:
+        check(lib().gx_part_sssp_first(self._h, begin, end))
+
+    def sssp(self, begin: int, end: int, priority: int) -> tuple[int, int]:
+        a, w = C.c_uint32(), C.c_uint32()
+        check(lib().gx_part_sssp(self._h, begin, end, priority, C.byref(a), C.byref(w)))
+        return a.value, w.value
+
+    def sssp_check(self, begin: int, end: int, infinity: int) -> tuple[int, int, int]:
+        s, d, mx = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        check(lib().gx_part_sssp_check(self._h, begin, end, infinity, C.byref(s), C.byref(d), C.byref(mx)))
+        return s.value, d.value, mx.value
+
+    def cc_init(self, begin: int, end: int) -> None:
+        check(lib().gx_part_cc_init(self._h, begin, end))
+
+    def cc_first(self, begin: int, end: int) -> None:
+        check(lib().gx_part_cc_first(self._h, begin, end))
+
+    def cc(self, begin: int, end: int) -> int:
+        a = C.c_uint32()
+        check(lib().gx_part_cc(self._h, begin, end, C.byref(a)))
+        return a.value
+
+    def cc_check(self, begin: int, end: int) -> int:
+        a = C.c_uint64()
+        check(lib().gx_part_cc_check(self._h, begin, end, C.byref(a)))
+        return a.value
+
     def kcore_init1(self, begin: int, end: int) -> None:
         check(lib().gx_part_kcore_init1(self._h, begin, end))
 
@@ -157,6 +194,43 @@ def dirgl_bfs(part: Partition, src_gid: int, delta: int = 0, max_iterations: int
         if not (rounds < max_iterations and active):
             break
     return part.read(DIST_CURRENT), rounds, work
+
+
+def dirgl_sssp(part: Partition, src_gid: int, delta: int = 0, max_iterations: int = 100000):
+    """sssp_push.cpp SSSP::go for one host; returns (dist_current of all local
+    nodes, rounds, work edges of the SSSP rounds)."""
+    a0, a1 = part.range(RANGE_ALL)
+    w0, w1 = part.range(RANGE_WITH_EDGES)
+    part.sssp_init(a0, a1, INFINITY, src_gid)
+    lid = part.local_id(src_gid)
+    if lid is not None:
+        part.sssp_first(lid, lid + 1)
+    priority = 0xFFFFFFFF if delta == 0 else 0
+    rounds, work = 1, 0
+    while True:
+        priority = (priority + delta) & 0xFFFFFFFF
+        active, w = part.sssp(w0, w1, priority)
+        work += w
+        rounds += 1
+        if not (rounds < max_iterations and active):
+            break
+    return part.read(DIST_CURRENT), rounds, work
+
+
+def dirgl_cc(part: Partition, max_iterations: int = 100000):
+    """cc_push.cpp ConnectedComp::go for one host; returns (comp_current of all
+    local nodes, components among the masters, rounds)."""
+    a0, a1 = part.range(RANGE_ALL)
+    w0, w1 = part.range(RANGE_WITH_EDGES)
+    part.cc_init(a0, a1)
+    part.cc_first(w0, w1)
+    rounds = 1
+    while True:
+        active = part.cc(w0, w1)
+        rounds += 1
+        if not (rounds < max_iterations and active):
+            break
+    return part.read(COMP_CURRENT), part.cc_check(*part.range(RANGE_MASTERS)), rounds
 
 
 def dirgl_kcore(part: Partition, k: int, max_iterations: int = 10000):

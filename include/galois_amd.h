@@ -318,14 +318,15 @@ gx_status gx_part_create(int device, gx_part** out);
 gx_status gx_part_free(gx_part* p);
 /* load_graph_CUDA (HostDecls.h:100-101) from the MarshalGraph fields
  * (HostDecls.h:38-54): host arrays row_start[n+1] and edge_dst[m] (uint32, the
- * reference's index_type), node_gid[n] = global id of each local node (the
- * MarshalGraph node_data; NULL = identity), the master range [begin_master,
- * begin_master + num_owned) and numNodesWithEdges.  The arrays are validated
- * (offsets monotone from 0 to m, destinations < n) before upload.  Fields and
- * bitsets start at zero. */
+ * reference's index_type), edge_data[m] (uint32 weights, needed by SSSP push;
+ * NULL = none), node_gid[n] = global id of each local node (the MarshalGraph
+ * node_data; NULL = identity), the master range [begin_master, begin_master +
+ * num_owned) and numNodesWithEdges.  The arrays are validated (offsets
+ * monotone from 0 to m, destinations < n) before upload.  Fields and bitsets
+ * start at zero. */
 gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint32_t* row_start,
-                       const uint32_t* edge_dst, const uint32_t* node_gid, uint32_t num_owned,
-                       uint32_t begin_master, uint32_t num_nodes_with_edges);
+                       const uint32_t* edge_dst, const uint32_t* edge_data, const uint32_t* node_gid,
+                       uint32_t num_owned, uint32_t begin_master, uint32_t num_nodes_with_edges);
 /* Ranges of the reference's _allNodes_cuda / _masterNodes_cuda /
  * _nodesWithEdges_cuda variants. */
 #define GX_RANGE_ALL 0
@@ -339,6 +340,8 @@ gx_status gx_part_range(const gx_part* p, int which, uint32_t* begin, uint32_t* 
 #define GX_FIELD_CURRENT_DEGREE 2 /* kcore_push NodeData (kcore_push.cpp:69) */
 #define GX_FIELD_TRIM 3
 #define GX_FIELD_FLAG 4
+#define GX_FIELD_COMP_CURRENT 5   /* cc_push NodeData (cc_push.cpp:63-66) */
+#define GX_FIELD_COMP_OLD 6
 /* get/set/add/min_node_<field>_cuda (bfs_push_cuda.h:11-17); *changed = 1 when
  * min lowered the value. */
 gx_status gx_part_get(const gx_part* p, int field, uint32_t lid, uint32_t* value);
@@ -353,7 +356,8 @@ gx_status gx_part_read(const gx_part* p, int field, uint32_t begin, uint32_t end
 /* get_bitset_<field>_cuda (ceil(n/64) words to host) and
  * bitset_<field>_reset_cuda, whole ([0, n)) or [begin, end)
  * (bfs_push_cuda.h:6-10).  Operators set bits on the field they change:
- * BFS on dist_current, InitializeGraph2 on current_degree, KCoreStep1 on trim. */
+ * BFS/SSSP on dist_current, CC on comp_current, InitializeGraph2 on
+ * current_degree, KCoreStep1 on trim. */
 gx_status gx_part_bitset_get(const gx_part* p, int field, uint64_t* words);
 gx_status gx_part_bitset_reset(gx_part* p, int field, uint32_t begin, uint32_t end);
 
@@ -371,6 +375,27 @@ gx_status gx_part_bfs(gx_part* p, uint32_t begin, uint32_t end, uint32_t priorit
                       uint32_t* active_vertices, uint32_t* work_items);
 gx_status gx_part_bfs_check(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity, uint64_t* sum,
                             uint32_t* max_dist);
+/* SSSP push operators (sssp/sssp_push.cpp:95-360, sssp_push_cuda.h:103-128):
+ * as BFS push with dist_current + edge weight pushed; the check also returns
+ * the sum of the reached distances (dg_avg). */
+gx_status gx_part_sssp_init(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity, uint64_t src_gid);
+gx_status gx_part_sssp_first(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_sssp(gx_part* p, uint32_t begin, uint32_t end, uint32_t priority,
+                       uint32_t* active_vertices, uint32_t* work_edges);
+gx_status gx_part_sssp_check(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity, uint64_t* sum,
+                             uint64_t* dist_sum, uint32_t* max_dist);
+/* CC push operators (connected-components/cc_push.cpp:85-300,
+ * cc_push_cuda.h:103-130).
+ * InitializeGraph_cuda: comp_current = comp_old = global id.
+ * FirstItr_ConnectedComp_cuda: every node of the range pushes its label.
+ * ConnectedComp_cuda: nodes with comp_old > comp_current push their label
+ *   (atomic min, bitset comp_current); one active per pushed edge.
+ * ConnectedCompSanityCheck_cuda: *components = nodes whose label is their
+ *   own global id. */
+gx_status gx_part_cc_init(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_cc_first(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_cc(gx_part* p, uint32_t begin, uint32_t end, uint32_t* active_vertices);
+gx_status gx_part_cc_check(gx_part* p, uint32_t begin, uint32_t end, uint64_t* components);
 /* k-core push operators (kcore_push.cpp:93-360: InitializeGraph2, InitializeGraph1,
  * KCoreStep2, KCoreStep1, KCoreSanityCheck).
  * InitializeGraph1_cuda: flag = 1, trim = current_degree = 0.

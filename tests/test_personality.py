@@ -1,8 +1,9 @@
 """D-IrGL personality partition (gx_part_*, galois_amd.personality): the
-reference's D-IrGL host loops for one host (bfs_push.cpp, kcore_push.cpp)
-driven through the C ABI, against the oracle and the golden vectors: BFS
-distances bit-exact (with D-IrGL's infinity), k-core membership bit-exact,
-sanity-check reductions, split ranges, change bitsets and field accessors."""
+reference's D-IrGL host loops for one host (bfs_push.cpp, sssp_push.cpp,
+cc_push.cpp, kcore_push.cpp) driven through the C ABI, against the oracle and
+the golden vectors: BFS and SSSP distances bit-exact (with D-IrGL's
+infinity), CC labels and k-core membership bit-exact, sanity-check
+reductions, split ranges, change bitsets and field accessors."""
 from __future__ import annotations
 
 import numpy as np
@@ -90,6 +91,46 @@ def test_bfs_push_split_ranges_and_bitset(gx, oracle, rmat16):
             break
     assert not active
     np.testing.assert_array_equal(P.read(DIST_CURRENT), _expected_dist(oracle.bfs(row, col, s, threads=8)))
+
+
+def test_sssp_push(gx, golden, oracle, rmat16):
+    """SSSP push distances bit-exact against the golden vectors and the
+    oracle (RMAT-16, delta 0 and 5: the priority only defers nodes); the
+    sanity check's count, distance sum and maximum."""
+    from galois_amd.personality import INFINITY, RANGE_MASTERS, dirgl_sssp
+    cases = []
+    for g in golden["undirected"]:
+        row, col, w = csr_arrays(g)
+        for s_, exp in g["sssp"].items():
+            cases.append((g["name"], row, col, w, int(s_), np.asarray(exp, np.uint32), 0))
+    G, row, col, w = rmat16
+    for s_ in (0, int(np.argmax(np.diff(row)))):
+        exp = oracle.sssp(row, col, w, s_, 6, threads=8)
+        for delta in (0, 5):
+            cases.append((f"rmat16 src={s_}", row, col, w, s_, exp, delta))
+    for name, row, col, w, src, exp, delta in cases:
+        print(f"dirgl sssp {name} src={src} delta={delta}", flush=True)
+        P = _part(row, col, weights=w)
+        dist, rounds, work = dirgl_sssp(P, src, delta=delta)
+        e = _expected_dist(exp)
+        np.testing.assert_array_equal(dist, e, err_msg=name)
+        reached = e[e != INFINITY].astype(np.int64)
+        assert P.sssp_check(*P.range(RANGE_MASTERS), INFINITY) == (len(reached), int(reached.sum()), int(reached.max()))
+
+
+def test_cc_push(gx, golden, oracle, rmat16):
+    """CC push labels (minimum global id of the component) bit-exact against
+    the golden vectors and the oracle; the sanity check counts components."""
+    from galois_amd.personality import dirgl_cc
+    cases = [(g["name"], *csr_arrays(g)[:2], np.asarray(g["cc"], np.uint32)) for g in golden["undirected"]]
+    G, row, col, _ = rmat16
+    cases.append(("rmat16", row, col, oracle.cc(row, col, threads=8)))
+    for name, row, col, exp in cases:
+        print(f"dirgl cc {name}", flush=True)
+        P = _part(row, col)
+        label, comps, rounds = dirgl_cc(P)
+        np.testing.assert_array_equal(label, exp, err_msg=name)
+        assert comps == int((exp == np.arange(len(exp))).sum())
 
 
 def test_kcore_push(gx, golden, oracle, rmat16):
