@@ -16,15 +16,21 @@
 //    if its distance changed since it was last expanded (`done` word), the
 //    GPU form of the reference's stale-item test (SSSP.cpp:138).
 //  - far pile: vertices improved to a later bucket; each vertex is in the far
-//    pile at most once (farmark word), so the pile is bounded by n.  When the
-//    near frontier drains, one pass finds the smallest live far bucket and a
-//    second pass moves that bucket into the near frontier and keeps the rest.
+//    pile at most once (farmark word), so the pile is bounded by n.  A
+//    running minimum of the far buckets (every improvement into a later
+//    bucket, and every item a split keeps) names the next bucket; when the
+//    near frontier drains, one pass moves that bucket into the near frontier
+//    and keeps the rest.  The running minimum can name a bucket whose items
+//    have meanwhile moved nearer; that split then moves nothing and the next
+//    one uses the exact minimum of the items it kept.
 //  - The loop is driven on the device: every step is the same fixed kernel
-//    sequence (begin, filter, relax, hubs, far-min, far-split, end) whose
-//    kernels read the step's mode from a device state word and return at
-//    once when it is not theirs.  Steps are captured once into a HIP graph
-//    (kStepsPerGraph steps) and replayed until the state says done, so the
-//    host synchronises once per graph replay instead of once per round.
+//    sequence (step, relax with the stale-item filter fused in as the
+//    expansion's admission test, hub layout, hubs, far-split) whose kernels
+//    read the step's mode from a device state word and return at once when
+//    it is not theirs; the step kernel also closes the previous step.  Steps
+//    are captured once into a HIP graph (kStepsPerGraph steps and a closing
+//    kernel) and replayed until the state says done, so the host
+//    synchronises once per graph replay instead of once per round.
 #include <algorithm>
 
 #include "frontier.hpp"
@@ -45,10 +51,9 @@ enum Mode : unsigned { kIdle = 0, kRelax = 1, kAdvance = 2 };
 struct SsspState {
   unsigned near_count[2];  // near queues (double buffered by q)
   unsigned far_count[2];   // far piles (double buffered by far_sel)
-  unsigned filt_count;     // filtered near items of the current round
   unsigned hub_count;
   unsigned overflow;
-  unsigned min_bucket;
+  unsigned far_min;        // running minimum of the far buckets
   unsigned mode;
   unsigned done;
   unsigned q;              // near queue read this step
@@ -73,7 +78,6 @@ struct Bufs {
   uint32_t* done;
   uint32_t* nearq[2];
   uint32_t* farq[2];
-  uint32_t* filt;
   uint32_t* hubq;
   SsspState* S;
   uint32_t shift;
@@ -101,10 +105,9 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
     S->near_count[0] = 1;
     S->near_count[1] = 0;
     S->far_count[0] = S->far_count[1] = 0;
-    S->filt_count = 0;
     S->hub_count = 0;
     S->overflow = 0;
-    S->min_bucket = 0xffffffffu;
+    S->far_min = 0xffffffffu;
     S->mode = kIdle;
     S->done = 0;
     S->q = 0;
@@ -119,34 +122,51 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
   }
 }
 
-// Chooses the step's mode and clears the counters the step appends to.
-__global__ void step_begin_kernel(SsspState* S) {
-  if (threadIdx.x != 0) return;
-  if (S->done || S->overflow) {
-    S->mode = kIdle;
-    return;
+// Closes a step: the queue it filled becomes the input; after a split the
+// far piles swap.
+__device__ __forceinline__ void close_step(SsspState* S) {
+  if (S->mode == kRelax) {
+    S->q ^= 1;
+  } else if (S->mode == kAdvance) {
+    S->far_count[S->far_sel] = 0;
+    S->far_sel ^= 1;
+    S->q ^= 1;
   }
+  S->mode = kIdle;
+}
+
+// Closes the previous step, then chooses this step's mode and clears the
+// counters it appends to.  An advance takes the running far minimum as the
+// new bucket; with far items left but no live far bucket (minimum unset)
+// every far item is settled and the run is done.
+__global__ void step_kernel(SsspState* S) {
+  if (threadIdx.x != 0) return;
+  close_step(S);
+  if (S->done || S->overflow) return;
   S->round += 1;
-  S->filt_count = 0;
   S->hub_count = 0;
   S->near_count[S->q ^ 1] = 0;
   if (S->near_count[S->q] > 0) {
     S->mode = kRelax;
-  } else if (S->far_count[S->far_sel] > 0) {
+  } else if (S->far_count[S->far_sel] > 0 && S->far_min != 0xffffffffu) {
     S->mode = kAdvance;
     S->phase += 1;
-    S->min_bucket = 0xffffffffu;
+    S->bucket = S->far_min;
+    S->far_min = 0xffffffffu;
     S->far_count[S->far_sel ^ 1] = 0;
   } else {
-    S->mode = kIdle;
     S->done = 1;
+    return;
   }
   S->steps += 1;
 }
 
-// Relax step, part 1: drop near items whose distance has not changed since
-// their last expansion; mark the rest expanded at their current distance.
-__global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B);
+// Last kernel of a captured batch: closes its last step, so the state is
+// whole when the host reads it.
+__global__ void batch_end_kernel(SsspState* S) {
+  if (threadIdx.x != 0) return;
+  close_step(S);
+}
 
 constexpr int kQCap = 2048;
 
@@ -163,6 +183,18 @@ struct RelaxOp {
   uint32_t* farq;
   unsigned* far_count;
   uint32_t bucket, round, phase;
+  uint32_t far_min = 0xffffffffu;  // this lane's smallest far bucket improved into
+  unsigned long long edges = 0, items = 0;
+  // Stale-item filter (the reference's test at SSSP.cpp:138): a near item is
+  // expanded only if its distance changed since it was last expanded.
+  __device__ __forceinline__ bool admit(uint32_t u) {
+    const uint32_t d = dist[u];
+    if (B.done[u] == d) return false;
+    B.done[u] = d;
+    edges += static_cast<unsigned long long>(B.row[u + 1] - B.row[u]);
+    items += 1;
+    return true;
+  }
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     bool to_near = false, to_far = false;
     uint32_t v = 0;
@@ -172,10 +204,13 @@ struct RelaxOp {
       if (nd < dist[v]) {
         const uint32_t old = atomicMin(&dist[v], nd);
         if (nd < old) {
-          if ((nd >> B.shift) <= bucket)
+          const uint32_t nb = nd >> B.shift;
+          if (nb <= bucket) {
             to_near = atomicExch(&B.stamp[v], round) != round;
-          else
+          } else {
             to_far = atomicExch(&B.farmark[v], phase) != phase;
+            far_min = min(far_min, nb);
+          }
         }
       }
     }
@@ -187,6 +222,20 @@ struct RelaxOp {
     lq_flush(*qf, farq, far_count, B.cap, &B.S->overflow, force);
   }
   __device__ __forceinline__ void after_step() { flush(false); }
+  // End of a launch: queues flushed, far minimum and filter counts reduced
+  // once per wavefront.
+  __device__ __forceinline__ void finish() {
+    flush(true);
+    const uint32_t fm = wave_min(far_min);
+    const unsigned long long ed = wave_sum(edges), it = wave_sum(items);
+    if (lane_id() == 0) {
+      if (fm != 0xffffffffu) atomicMin(&B.S->far_min, fm);
+      if (it) {
+        atomicAdd(&B.S->edges, ed);
+        atomicAdd(&B.S->items, it);
+      }
+    }
+  }
 };
 
 __device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn, LdsQueue<kQCap>* qf) {
@@ -196,7 +245,7 @@ __device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn
                  S->bucket, S->round, S->phase};
 }
 
-// Relax step, part 2: expand the filtered items.
+// Relax step: expand the near items that pass the stale-item filter.
 __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
   SsspState* S = B.S;
   if (S->mode != kRelax) return;
@@ -204,8 +253,10 @@ __global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
   lq_init(qn);
   lq_init(qf);
   RelaxOp op = make_relax(B, &qn, &qf);
-  expand_queue<kBlock>(B.row, B.filt, S->filt_count, B.hub, B.hubq, &S->hub_count, B.cap, &S->overflow, op);
-  op.flush(true);
+  const unsigned q = S->q;
+  expand_queue<kBlock, RelaxOp, true>(B.row, B.nearq[q], S->near_count[q], B.hub, B.hubq, &S->hub_count, B.cap,
+                                      &S->overflow, op);
+  op.finish();
 }
 
 __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
@@ -216,71 +267,17 @@ __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
   lq_init(qf);
   RelaxOp op = make_relax(B, &qn, &qf);
   expand_hubs<kBlock>(B.row, B.hubq, S->hub_count, B.cap, B.hub_off, op);
-  op.flush(true);
+  op.finish();
 }
 
-__global__ void __launch_bounds__(kBlock) filter_kernel(Bufs B) {
-  SsspState* S = B.S;
-  if (S->mode != kRelax) return;
-  const uint32_t* __restrict__ dist = S->dist_out;
-  __shared__ LdsQueue<kQCap> qo;
-  lq_init(qo);
-  const unsigned q = S->q;
-  const unsigned cnt = S->near_count[q];
-  const uint32_t* in = B.nearq[q];
-  unsigned long long edges = 0, items = 0;
-  for (unsigned b = blockIdx.x * kBlock; b < cnt; b += gridDim.x * kBlock) {
-    const unsigned i = b + threadIdx.x;
-    bool keep = false;
-    uint32_t u = 0;
-    if (i < cnt) {
-      u = in[i];
-      const uint32_t d = dist[u];
-      keep = B.done[u] != d;
-      if (keep) {
-        B.done[u] = d;
-        edges += static_cast<unsigned long long>(B.row[u + 1] - B.row[u]);
-        items += 1;
-      }
-    }
-    lq_push(qo, keep, u);
-    lq_flush(qo, B.filt, &S->filt_count, B.cap, &S->overflow, false);
-  }
-  lq_flush(qo, B.filt, &S->filt_count, B.cap, &S->overflow, true);
-  edges = wave_sum(edges);
-  items = wave_sum(items);
-  if (lane_id() == 0 && items) {
-    atomicAdd(&S->edges, edges);
-    atomicAdd(&S->items, items);
-  }
-}
-
-// Advance step, part 1: smallest bucket among far items above the current.
-__global__ void __launch_bounds__(kBlock) far_min_kernel(Bufs B) {
-  SsspState* S = B.S;
-  if (S->mode != kAdvance) return;
-  const uint32_t* __restrict__ dist = S->dist_out;
-  const unsigned fs = S->far_sel;
-  const unsigned cnt = S->far_count[fs];
-  const uint32_t* farq = B.farq[fs];
-  const uint32_t bucket = S->bucket;
-  uint32_t best = 0xffffffffu;
-  for (unsigned i = blockIdx.x * kBlock + threadIdx.x; i < cnt; i += gridDim.x * kBlock) {
-    const uint32_t b = dist[farq[i]] >> B.shift;
-    if (b > bucket && b < best) best = b;
-  }
-  best = wave_min(best);
-  if (lane_id() == 0 && best != 0xffffffffu) atomicMin(&S->min_bucket, best);
-}
-
-// Advance step, part 2: far items of the new bucket go to the near queue,
-// later ones are kept (one copy per vertex), settled ones are dropped.
+// Advance step: far items of the new bucket go to the near queue, later
+// ones are kept (one copy per vertex, their buckets fold into the running far
+// minimum), settled ones are dropped.
 __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
   SsspState* S = B.S;
   if (S->mode != kAdvance) return;
   const uint32_t* __restrict__ dist = S->dist_out;
-  const uint32_t next = S->min_bucket;
-  if (next == 0xffffffffu) return;
+  const uint32_t next = S->bucket;
   __shared__ LdsQueue<kQCap> qn, qk;
   lq_init(qn);
   lq_init(qk);
@@ -288,6 +285,7 @@ __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
   const unsigned cnt = S->far_count[fs];
   const uint32_t* farq = B.farq[fs];
   const uint32_t round = S->round, phase = S->phase;
+  uint32_t kept_min = 0xffffffffu;
   for (unsigned b0 = blockIdx.x * kBlock; b0 < cnt; b0 += gridDim.x * kBlock) {
     const unsigned i = b0 + threadIdx.x;
     bool to_near = false, keep = false;
@@ -295,10 +293,12 @@ __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
     if (i < cnt) {
       v = farq[i];
       const uint32_t b = dist[v] >> B.shift;
-      if (b == next)
+      if (b == next) {
         to_near = atomicExch(&B.stamp[v], round) != round;
-      else if (b > next)
+      } else if (b > next) {
         keep = atomicExch(&B.farmark[v], phase) != phase;
+        kept_min = min(kept_min, b);
+      }
     }
     lq_push(qn, to_near, v);
     lq_push(qk, keep, v);
@@ -307,34 +307,16 @@ __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
   }
   lq_flush(qn, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.cap, &S->overflow, true);
   lq_flush(qk, B.farq[fs ^ 1], &S->far_count[fs ^ 1], B.cap, &S->overflow, true);
-}
-
-__global__ void step_end_kernel(SsspState* S) {
-  if (threadIdx.x != 0) return;
-  if (S->mode == kRelax) {
-    S->q ^= 1;
-  } else if (S->mode == kAdvance) {
-    if (S->min_bucket == 0xffffffffu) {
-      S->done = 1;  // every far item is settled
-    } else {
-      S->bucket = S->min_bucket;
-      S->far_count[S->far_sel] = 0;
-      S->far_sel ^= 1;
-      S->q ^= 1;
-    }
-  }
-  S->mode = kIdle;
+  kept_min = wave_min(kept_min);
+  if (lane_id() == 0 && kept_min != 0xffffffffu) atomicMin(&S->far_min, kept_min);
 }
 
 void enqueue_step(const Bufs& B, hipStream_t s) {
-  step_begin_kernel<<<1, 64, 0, s>>>(B.S);
-  filter_kernel<<<kGrid, kBlock, 0, s>>>(B);
+  step_kernel<<<1, 64, 0, s>>>(B.S);
   relax_kernel<<<kGrid, kBlock, 0, s>>>(B);
   hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off);
   hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
-  far_min_kernel<<<kGrid, kBlock, 0, s>>>(B);
   far_split_kernel<<<kGrid, kBlock, 0, s>>>(B);
-  step_end_kernel<<<1, 64, 0, s>>>(B.S);
   GX_LAUNCHED("sssp step");
 }
 
@@ -363,7 +345,6 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.stamp = g->scratch<uint32_t>(WS_V1, cap);
   B.farmark = g->scratch<uint32_t>(WS_V2, cap);
   B.done = g->scratch<uint32_t>(WS_V3, cap);
-  B.filt = g->scratch<uint32_t>(WS_V4, cap);
   B.S = g->scratch<SsspState>(WS_COUNTERS, 1);
   B.shift = shift;
   B.cap = cap;
@@ -381,7 +362,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w,
                         (const void*)B.nearq[0], (const void*)B.nearq[1], (const void*)B.farq[0],
                         (const void*)B.farq[1], (const void*)B.hubq, (const void*)B.stamp, (const void*)B.farmark,
-                        (const void*)B.done, (const void*)B.filt, (const void*)B.S, (const void*)B.hub_off})
+                        (const void*)B.done, (const void*)B.S, (const void*)B.hub_off})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
   key = mix_key(mix_key(mix_key(key, shift), cap), B.hub);
   uint32_t replays = 0;
@@ -389,6 +370,8 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
     ktimer.begin();
     g->replay(g->sssp_batch, key, [&](hipStream_t cs) {
       for (int i = 0; i < kStepsPerGraph; ++i) enqueue_step(B, cs);
+      batch_end_kernel<<<1, 64, 0, cs>>>(B.S);
+      GX_LAUNCHED("batch_end_kernel");
     });
     ktimer.end();
     ++replays;
@@ -401,7 +384,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = host->steps;
-    st->launches = 2 + replays * kStepsPerGraph * 8;
+    st->launches = 2 + replays * (kStepsPerGraph * 5 + 1);
     st->edges_visited = host->edges;
     st->items = host->items;
     st->elapsed_ms = ms;
