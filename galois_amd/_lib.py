@@ -134,6 +134,11 @@ SIGNATURES: dict[str, tuple] = {
     "gx_part_cc_first": (INT, [P, U32, U32]),
     "gx_part_cc": (INT, [P, U32, U32, C.POINTER(U32)]),
     "gx_part_cc_check": (INT, [P, U32, U32, C.POINTER(U64)]),
+    "gx_part_pr_reset": (INT, [P, U32, U32]),
+    "gx_part_pr_init": (INT, [P, U32, U32, F]),
+    "gx_part_pr_delta": (INT, [P, U32, U32, F, F]),
+    "gx_part_pr": (INT, [P, U32, U32, C.POINTER(U32)]),
+    "gx_part_pr_check": (INT, [P, U32, U32, F, P]),
     "gx_part_kcore_init1": (INT, [P, U32, U32]),
     "gx_part_kcore_init2": (INT, [P, U32, U32]),
     "gx_part_kcore_step1": (INT, [P, U32, U32, U32, C.POINTER(U32)]),

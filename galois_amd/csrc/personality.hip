@@ -12,6 +12,10 @@
 //  - CC push (connected-components/cc_push.cpp:85-300, cc_push_cuda.h:
 //    103-130) over comp_current / comp_old: labels start at the global id,
 //    every node with a lowered label pushes it, active counted per edge;
+//  - PageRank push (pagerank/pagerank_push.cpp:110-430, pagerank_push_cuda.h)
+//    over value / nout / residual / delta (FP32, stored as bit patterns):
+//    residual flows to value, deltas above the tolerance are pushed with
+//    float atomics (order dependent, as the reference's);
 //  - k-core push operators InitializeGraph1/2, KCoreStep1/2, KCoreSanityCheck
 //    (lonestar/analytics/distributed/k-core/kcore_push.cpp:93-360, declared
 //    in kcore_push_cuda.h:133-167) over current_degree / trim / flag;
@@ -41,7 +45,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxGrid = 2048;
-constexpr int kFields = 7;
+constexpr int kFields = 11;
 
 // Device counters of one operator launch.
 struct PartCounters {
@@ -54,6 +58,10 @@ struct PartCounters {
   unsigned long long dsum;
   unsigned max;
   unsigned pad;
+  // PageRank sanity (non-negative floats compared as bit patterns)
+  unsigned max_value, min_value, max_res, min_res;
+  double sum_value, sum_res;
+  unsigned long long over_tol;
 };
 
 }  // namespace
@@ -64,7 +72,7 @@ struct gx_part {
   int64_t n = 0, m = 0;
   uint32_t num_owned = 0, begin_master = 0, with_edges = 0;
   bool loaded = false;
-  gx::DevBuf row, col, w, gid, field[kFields], bits[kFields], hubq, hub_off, counters;
+  gx::DevBuf row, col, w, gid, field[kFields], bits[kFields], hubq, hub_off, counters, push;
   PartCounters* host = nullptr;  // pinned
   ~gx_part() {
     if (host) (void)hipHostFree(host);
@@ -247,6 +255,115 @@ __global__ void __launch_bounds__(kBlock) cc_check_kernel(const uint32_t* __rest
     cnt += cur[v] == (gid ? gid[v] : v) ? 1 : 0;
   cnt = wave_sum(cnt);
   if (lane_id() == 0 && cnt) atomicAdd(&c->sum, cnt);
+}
+
+// --------------------------------------------------------- PageRank push
+__device__ __forceinline__ float as_f(uint32_t b) { return __uint_as_float(b); }
+__device__ __forceinline__ uint32_t as_u(float f) { return __float_as_uint(f); }
+
+// ResetGraph (pagerank_push.cpp:118-124): all four fields zero.
+__global__ void pr_reset_kernel(uint32_t* __restrict__ value, uint32_t* __restrict__ nout,
+                                uint32_t* __restrict__ res, uint32_t* __restrict__ delta, uint32_t begin,
+                                uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    value[v] = 0;
+    nout[v] = 0;
+    res[v] = 0;
+    delta[v] = 0;
+  }
+}
+
+// InitializeGraph (pagerank_push.cpp:168-175): residual = alpha, nout += the
+// node's out-degree (bitset nout).
+__global__ void pr_init_kernel(const int32_t* __restrict__ row, uint32_t* __restrict__ nout,
+                               uint32_t* __restrict__ res, unsigned long long* __restrict__ nout_bits, float alpha,
+                               uint32_t begin, uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    res[v] = as_u(alpha);
+    atomicAdd(&nout[v], static_cast<uint32_t>(row[v + 1] - row[v]));
+    set_bit(nout_bits, v);
+  }
+}
+
+// PageRank_delta (pagerank_push.cpp:210-223): residual moves into value; a
+// residual above the tolerance becomes the delta residual*(1-alpha)/nout.
+__global__ void pr_delta_kernel(uint32_t* __restrict__ value, const uint32_t* __restrict__ nout,
+                                uint32_t* __restrict__ res, uint32_t* __restrict__ delta, float alpha, float tol,
+                                uint32_t begin, uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const float r = as_f(res[v]);
+    if (r > 0.0f) {
+      res[v] = 0;
+      value[v] = as_u(as_f(value[v]) + r);
+      const uint32_t k = nout[v];
+      if (r > tol && k > 0) delta[v] = as_u(r * (1.0f - alpha) / static_cast<float>(k));
+    }
+  }
+}
+
+// PageRank (pagerank_push.cpp:290-307): a node with a positive delta (counted
+// active) clears it and adds it to every out-neighbour's residual (float
+// atomics, bitset residual).  The delta is parked in `push` for the edges.
+struct PrPushOp {
+  const int32_t* __restrict__ col;
+  uint32_t* delta;
+  float* push;
+  float* res;
+  unsigned long long* bits;
+  PartCounters* c;
+  unsigned active = 0;
+  __device__ __forceinline__ bool admit(uint32_t v) {
+    const float d = as_f(delta[v]);
+    if (!(d > 0.0f)) return false;
+    delta[v] = 0;
+    push[v] = d;
+    ++active;
+    return true;
+  }
+  __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
+    if (!valid) return;
+    const uint32_t v = static_cast<uint32_t>(col[e]);
+    atomicAdd(&res[v], push[u]);
+    set_bit(bits, v);
+  }
+  __device__ __forceinline__ void after_step() {}
+  __device__ __forceinline__ void finish() { flush_counts(c, active, 0ull); }
+};
+
+// PageRankSanity (pagerank_push.cpp:415-429): value and residual extremes and
+// sums, residuals above the tolerance.
+__global__ void __launch_bounds__(kBlock) pr_check_kernel(const uint32_t* __restrict__ value,
+                                                          const uint32_t* __restrict__ res, float tol,
+                                                          uint32_t begin, uint32_t end, PartCounters* c) {
+  uint32_t mxv = 0, mnv = 0x7f800000u, mxr = 0, mnr = 0x7f800000u;
+  double sv = 0, sr = 0;
+  unsigned long long over = 0;
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const uint32_t bv = value[v], br = res[v];
+    mxv = max(mxv, bv);
+    mnv = min(mnv, bv);
+    mxr = max(mxr, br);
+    mnr = min(mnr, br);
+    sv += as_f(bv);
+    sr += as_f(br);
+    over += as_f(br) > tol ? 1 : 0;
+  }
+  mxv = wave_max(mxv);
+  mnv = wave_min(mnv);
+  mxr = wave_max(mxr);
+  mnr = wave_min(mnr);
+  sv = wave_sum(sv);
+  sr = wave_sum(sr);
+  over = wave_sum(over);
+  if (lane_id() == 0) {
+    atomicMax(&c->max_value, mxv);
+    atomicMin(&c->min_value, mnv);
+    atomicMax(&c->max_res, mxr);
+    atomicMin(&c->min_res, mnr);
+    atomicAdd(&c->sum_value, sv);
+    atomicAdd(&c->sum_res, sr);
+    if (over) atomicAdd(&c->over_tol, over);
+  }
 }
 
 // ------------------------------------------------------------- k-core push
@@ -446,6 +563,7 @@ extern "C" gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint
   }
   p->hubq.ensure(n * 4);
   p->hub_off.ensure((n + 1) * 4);
+  p->push.ensure(n * 4);
   p->loaded = true;
   GX_API_END
 }
@@ -725,6 +843,94 @@ extern "C" gx_status gx_part_cc_check(gx_part* p, uint32_t begin, uint32_t end, 
   }
   const PartCounters& c = read_counters(p);
   if (components) *components = c.sum;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_pr_reset(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    pr_reset_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_PR_VALUE), p->f(GX_FIELD_PR_NOUT), p->f(GX_FIELD_PR_RESIDUAL), p->f(GX_FIELD_PR_DELTA), begin,
+        end);
+    GX_LAUNCHED("pr_reset_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_pr_init(gx_part* p, uint32_t begin, uint32_t end, float alpha) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    pr_init_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->row.as<int32_t>(), p->f(GX_FIELD_PR_NOUT), p->f(GX_FIELD_PR_RESIDUAL), p->b(GX_FIELD_PR_NOUT), alpha,
+        begin, end);
+    GX_LAUNCHED("pr_init_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_pr_delta(gx_part* p, uint32_t begin, uint32_t end, float alpha, float tolerance) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    pr_delta_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_PR_VALUE), p->f(GX_FIELD_PR_NOUT), p->f(GX_FIELD_PR_RESIDUAL), p->f(GX_FIELD_PR_DELTA), alpha,
+        tolerance, begin, end);
+    GX_LAUNCHED("pr_delta_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_pr(gx_part* p, uint32_t begin, uint32_t end, uint32_t* active_vertices) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  PrPushOp op{p->col.as<int32_t>(), p->f(GX_FIELD_PR_DELTA), p->push.as<float>(),
+              reinterpret_cast<float*>(p->f(GX_FIELD_PR_RESIDUAL)), p->b(GX_FIELD_PR_RESIDUAL),
+              p->counters.as<PartCounters>()};
+  run_edges(p, begin, end, op);
+  const PartCounters& c = read_counters(p);
+  if (active_vertices) *active_vertices = c.active;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_pr_check(gx_part* p, uint32_t begin, uint32_t end, float tolerance,
+                                      gx_pr_sanity* out) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  check_ptr(out, "out");
+  // Minima start at +inf (bit pattern); staged through the pinned block,
+  // which the stream reads before read_counters writes it again.
+  *p->host = PartCounters{};
+  p->host->min_value = p->host->min_res = 0x7f800000u;
+  GX_HIP(hipMemcpyAsync(p->counters.p, p->host, sizeof(PartCounters), hipMemcpyHostToDevice, p->stream));
+  if (end > begin) {
+    pr_check_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_PR_VALUE), p->f(GX_FIELD_PR_RESIDUAL), tolerance, begin, end, p->counters.as<PartCounters>());
+    GX_LAUNCHED("pr_check_kernel");
+  }
+  const PartCounters& c = read_counters(p);
+  auto f = [](uint32_t b) {
+    float x;
+    std::memcpy(&x, &b, 4);
+    return x;
+  };
+  out->max_value = f(c.max_value);
+  out->min_value = f(c.min_value);
+  out->max_residual = f(c.max_res);
+  out->min_residual = f(c.min_res);
+  out->sum_value = c.sum_value;
+  out->sum_residual = c.sum_res;
+  out->residual_over_tolerance = c.over_tol;
   GX_API_END
 }
 

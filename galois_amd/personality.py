@@ -6,6 +6,8 @@ The loops follow lonestar/analytics/distributed/bfs/bfs_push.cpp:95-300
 the nodes with edges until no vertex is active), the same structure in
 sssp/sssp_push.cpp:95-300 (weights pushed) and connected-components/
 cc_push.cpp:85-230 (labels pushed, FirstItr over the nodes with edges), and
+pagerank/pagerank_push.cpp:110-300 (ResetGraph, InitializeGraph, then
+PageRank_delta + PageRank rounds until no node pushes), and
 lonestar/analytics/distributed/k-core/kcore_push.cpp:93-318
 (InitializeGraph1 over all nodes, InitializeGraph2 over the nodes with edges,
 KCoreStep1 + KCoreStep2 rounds until no vertex dies).  With one host there is
@@ -20,6 +22,15 @@ import numpy as np
 from ._lib import check, lib
 
 DIST_CURRENT, DIST_OLD, CURRENT_DEGREE, TRIM, FLAG, COMP_CURRENT, COMP_OLD = range(7)
+PR_VALUE, PR_NOUT, PR_RESIDUAL, PR_DELTA = range(7, 11)
+PR_ALPHA = 1.0 - 0.85  # pagerank_push.cpp:68 (reset probability)
+
+
+class PrSanity(C.Structure):
+    """gx_pr_sanity: PageRankSanity's reductions."""
+    _fields_ = [("max_value", C.c_float), ("min_value", C.c_float), ("max_residual", C.c_float),
+                ("min_residual", C.c_float), ("sum_value", C.c_double), ("sum_residual", C.c_double),
+                ("residual_over_tolerance", C.c_uint64)]
 RANGE_ALL, RANGE_MASTERS, RANGE_WITH_EDGES = range(3)
 INFINITY = 0xFFFFFFFF // 4  # bfs_push.cpp:72 (numeric_limits<uint32_t>::max() / 4)
 
@@ -149,6 +160,28 @@ class Partition:
         check(lib().gx_part_cc_check(self._h, begin, end, C.byref(a)))
         return a.value
 
+    def pr_reset(self, begin: int, end: int) -> None:
+        check(lib().gx_part_pr_reset(self._h, begin, end))
+
+    def pr_init(self, begin: int, end: int, alpha: float = PR_ALPHA) -> None:
+        check(lib().gx_part_pr_init(self._h, begin, end, alpha))
+
+    def pr_delta(self, begin: int, end: int, alpha: float, tolerance: float) -> None:
+        check(lib().gx_part_pr_delta(self._h, begin, end, alpha, tolerance))
+
+    def pr(self, begin: int, end: int) -> int:
+        a = C.c_uint32()
+        check(lib().gx_part_pr(self._h, begin, end, C.byref(a)))
+        return a.value
+
+    def pr_check(self, begin: int, end: int, tolerance: float) -> PrSanity:
+        out = PrSanity()
+        check(lib().gx_part_pr_check(self._h, begin, end, tolerance, C.byref(out)))
+        return out
+
+    def read_float(self, field: int, begin: int = 0, end: int | None = None) -> np.ndarray:
+        return self.read(field, begin, end).view(np.float32)
+
     def kcore_init1(self, begin: int, end: int) -> None:
         check(lib().gx_part_kcore_init1(self._h, begin, end))
 
@@ -231,6 +264,23 @@ def dirgl_cc(part: Partition, max_iterations: int = 100000):
         if not (rounds < max_iterations and active):
             break
     return part.read(COMP_CURRENT), part.cc_check(*part.range(RANGE_MASTERS)), rounds
+
+
+def dirgl_pagerank(part: Partition, tolerance: float, alpha: float = PR_ALPHA, max_iterations: int = 1000):
+    """pagerank_push.cpp InitializeGraph::go + PageRank::go for one host;
+    returns (value of all local nodes as float32, rounds)."""
+    a0, a1 = part.range(RANGE_ALL)
+    w0, w1 = part.range(RANGE_WITH_EDGES)
+    part.pr_reset(a0, a1)
+    part.pr_init(w0, w1, alpha)
+    rounds = 0
+    while True:
+        part.pr_delta(w0, w1, alpha, tolerance)
+        active = part.pr(w0, w1)
+        rounds += 1
+        if not (rounds < max_iterations and active):
+            break
+    return part.read_float(PR_VALUE), rounds
 
 
 def dirgl_kcore(part: Partition, k: int, max_iterations: int = 10000):

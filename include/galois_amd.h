@@ -342,6 +342,10 @@ gx_status gx_part_range(const gx_part* p, int which, uint32_t* begin, uint32_t* 
 #define GX_FIELD_FLAG 4
 #define GX_FIELD_COMP_CURRENT 5   /* cc_push NodeData (cc_push.cpp:63-66) */
 #define GX_FIELD_COMP_OLD 6
+#define GX_FIELD_PR_VALUE 7       /* pagerank_push NodeData (pagerank_push.cpp:69-74); */
+#define GX_FIELD_PR_NOUT 8        /* value, residual and delta are float bit patterns */
+#define GX_FIELD_PR_RESIDUAL 9
+#define GX_FIELD_PR_DELTA 10
 /* get/set/add/min_node_<field>_cuda (bfs_push_cuda.h:11-17); *changed = 1 when
  * min lowered the value. */
 gx_status gx_part_get(const gx_part* p, int field, uint32_t lid, uint32_t* value);
@@ -356,8 +360,8 @@ gx_status gx_part_read(const gx_part* p, int field, uint32_t begin, uint32_t end
 /* get_bitset_<field>_cuda (ceil(n/64) words to host) and
  * bitset_<field>_reset_cuda, whole ([0, n)) or [begin, end)
  * (bfs_push_cuda.h:6-10).  Operators set bits on the field they change:
- * BFS/SSSP on dist_current, CC on comp_current, InitializeGraph2 on
- * current_degree, KCoreStep1 on trim. */
+ * BFS/SSSP on dist_current, CC on comp_current, PageRank on nout (init)
+ * and residual, InitializeGraph2 on current_degree, KCoreStep1 on trim. */
 gx_status gx_part_bitset_get(const gx_part* p, int field, uint64_t* words);
 gx_status gx_part_bitset_reset(gx_part* p, int field, uint32_t begin, uint32_t end);
 
@@ -396,6 +400,28 @@ gx_status gx_part_cc_init(gx_part* p, uint32_t begin, uint32_t end);
 gx_status gx_part_cc_first(gx_part* p, uint32_t begin, uint32_t end);
 gx_status gx_part_cc(gx_part* p, uint32_t begin, uint32_t end, uint32_t* active_vertices);
 gx_status gx_part_cc_check(gx_part* p, uint32_t begin, uint32_t end, uint64_t* components);
+/* PageRank push operators (pagerank/pagerank_push.cpp:110-430), FP32 fields;
+ * alpha is the reference's reset probability (1 - 0.85, pagerank_push.cpp:68).
+ * ResetGraph_cuda: value = nout = residual = delta = 0.
+ * InitializeGraph_cuda: residual = alpha, nout += out-degree (bitset nout).
+ * PageRank_delta_cuda: a positive residual moves into value; above the
+ *   tolerance it also sets delta = residual * (1 - alpha) / nout.
+ * PageRank_cuda: a node with positive delta (counted active) clears it and
+ *   adds it to every out-neighbour's residual (FP32 atomics: the sum order,
+ *   and so the last bits, vary run to run as in the reference; bitset
+ *   residual).
+ * PageRankSanity_cuda: extremes and sums of value and residual, residuals
+ *   above the tolerance (sums in FP64). */
+typedef struct gx_pr_sanity {
+  float max_value, min_value, max_residual, min_residual;
+  double sum_value, sum_residual;
+  uint64_t residual_over_tolerance;
+} gx_pr_sanity;
+gx_status gx_part_pr_reset(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_pr_init(gx_part* p, uint32_t begin, uint32_t end, float alpha);
+gx_status gx_part_pr_delta(gx_part* p, uint32_t begin, uint32_t end, float alpha, float tolerance);
+gx_status gx_part_pr(gx_part* p, uint32_t begin, uint32_t end, uint32_t* active_vertices);
+gx_status gx_part_pr_check(gx_part* p, uint32_t begin, uint32_t end, float tolerance, gx_pr_sanity* out);
 /* k-core push operators (kcore_push.cpp:93-360: InitializeGraph2, InitializeGraph1,
  * KCoreStep2, KCoreStep1, KCoreSanityCheck).
  * InitializeGraph1_cuda: flag = 1, trim = current_degree = 0.

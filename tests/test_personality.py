@@ -1,9 +1,10 @@
 """D-IrGL personality partition (gx_part_*, galois_amd.personality): the
 reference's D-IrGL host loops for one host (bfs_push.cpp, sssp_push.cpp,
-cc_push.cpp, kcore_push.cpp) driven through the C ABI, against the oracle and
-the golden vectors: BFS and SSSP distances bit-exact (with D-IrGL's
-infinity), CC labels and k-core membership bit-exact, sanity-check
-reductions, split ranges, change bitsets and field accessors."""
+cc_push.cpp, pagerank_push.cpp, kcore_push.cpp) driven through the C ABI,
+against the oracle and the golden vectors: BFS and SSSP distances bit-exact
+(with D-IrGL's infinity), CC labels and k-core membership bit-exact,
+PageRank values within the FP32-atomics tolerance, sanity-check reductions,
+split ranges, change bitsets and field accessors."""
 from __future__ import annotations
 
 import numpy as np
@@ -131,6 +132,37 @@ def test_cc_push(gx, golden, oracle, rmat16):
         label, comps, rounds = dirgl_cc(P)
         np.testing.assert_array_equal(label, exp, err_msg=name)
         assert comps == int((exp == np.arange(len(exp))).sum())
+
+
+def test_pagerank_push(gx, golden):
+    """PageRank push values (unnormalised, reset probability 0.15) against the
+    golden closed form and the library's own residual push, within the
+    tolerance the FP32 atomics allow; PageRankSanity's reductions against the
+    fields read back.  InitializeGraph and PageRank_delta run over the nodes
+    with edges only (pagerank_push.cpp:127-128, 189), which D-IrGL partitions
+    number first: nodes past the last node with out-edges never receive a
+    residual and keep value 0."""
+    from galois_amd.personality import PR_RESIDUAL, RANGE_MASTERS, RANGE_WITH_EDGES, dirgl_pagerank
+    tol = 1e-7
+    for g in golden["undirected"] + golden["directed"]:
+        print(f"dirgl pagerank {g['name']}", flush=True)
+        row, col, _ = csr_arrays(g)
+        P = _part(row, col)
+        value, rounds = dirgl_pagerank(P, tol)
+        we = P.range(RANGE_WITH_EDGES)[1]
+        closed = np.asarray(g["pagerank_unnormalised"]).copy()
+        closed[we:] = 0.0
+        np.testing.assert_allclose(value, closed, atol=2e-5, err_msg=g["name"])
+        G = gx.Graph.from_csr(row, col, None, symmetric=False)
+        ref = gx.pagerank_push(G, tol=tol)[0].cpu().numpy().copy()
+        ref[we:] = 0.0
+        np.testing.assert_allclose(value, ref, atol=2e-5, err_msg=g["name"])
+        san = P.pr_check(*P.range(RANGE_MASTERS), tol)
+        res = P.read_float(PR_RESIDUAL)
+        assert san.max_value == value.max() and san.min_value == value.min()
+        assert san.max_residual == res.max() and san.min_residual == res.min()
+        assert abs(san.sum_value - float(value.astype(np.float64).sum())) < 1e-4
+        assert san.residual_over_tolerance == int((res > np.float32(tol)).sum())
 
 
 def test_kcore_push(gx, golden, oracle, rmat16):
