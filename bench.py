@@ -288,6 +288,15 @@ def personality_rows(gx, row, col, wts, gname) -> dict:
     lab = gx.connected_components(G)[0].cpu().numpy().view(np.uint32)
     out[f"dirgl_cc_push_{gname}"] = {"ms": round(dt * 1e3, 3), "GTEPS": round(m / dt / 1e9, 3), "rounds": rounds,
                                      "components": comps, "labels_equal_gx_cc": bool(np.array_equal(lab, label))}
+    # PageRank push at the reference app's default tolerance 1e-6
+    # (pagerank_push.cpp:48-50); compared with the library's residual push on
+    # the nodes with edges (the others keep 0 in the D-IrGL loop).
+    dt, (value, rounds) = timed(lambda: pers.dirgl_pagerank(P, 1e-6))
+    ref = gx.pagerank_push(G, tol=1e-6)[0].cpu().numpy()
+    we = P.range(pers.RANGE_WITH_EDGES)[1]
+    rel = np.abs(value[:we] - ref[:we]) / np.maximum(np.abs(ref[:we]), 1.0)
+    out[f"dirgl_pagerank_push_{gname}"] = {"ms": round(dt * 1e3, 3), "rounds": rounds,
+                                           "max_rel_diff_vs_gx_push": float(rel.max())}
     best = None
     for _ in range(3):
         t0 = time.perf_counter()

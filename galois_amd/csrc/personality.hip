@@ -94,8 +94,12 @@ void check_range(const gx_part* p, uint32_t begin, uint32_t end) {
   GX_REQUIRE(begin <= end && static_cast<int64_t>(end) <= p->n, "range [begin, end) outside the partition");
 }
 
+// Marks v in a change bitset.  A plain load first: once a word's bit is set
+// the atomic is skipped, so hot words (64 neighbouring nodes, pushed to by
+// every in-edge) do not serialise on atomics.
 __device__ __forceinline__ void set_bit(unsigned long long* bits, uint32_t v) {
-  atomicOr(&bits[v >> 6], 1ull << (v & 63));
+  const unsigned long long m = 1ull << (v & 63);
+  if (!(__atomic_load_n(&bits[v >> 6], __ATOMIC_RELAXED) & m)) atomicOr(&bits[v >> 6], m);
 }
 
 // Per-lane counters reduced once per wavefront at the end of a launch.
