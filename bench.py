@@ -259,7 +259,8 @@ def kernel_table(gx, quick: bool) -> dict:
 def personality_rows(gx, row, col, wts, gname) -> dict:
     """The D-IrGL host loops of one host through the personality partition
     (gx_part_*): BFS push from the max-degree vertex, SSSP push from vertex 0,
-    CC push and k-core push (k=10), timed around the whole loop (every
+    CC push, PageRank push, BC level (16 sources) and k-core push (k=10),
+    timed around the whole loop (every
     operator returns its reductions to the host, as in the reference); the
     partition upload is not timed.  Results are checked against the library's
     own BFS levels, SSSP distances, CC labels and k-core membership."""
@@ -297,6 +298,14 @@ def personality_rows(gx, row, col, wts, gname) -> dict:
     rel = np.abs(value[:we] - ref[:we]) / np.maximum(np.abs(ref[:we]), 1.0)
     out[f"dirgl_pagerank_push_{gname}"] = {"ms": round(dt * 1e3, 3), "rounds": rounds,
                                            "max_rel_diff_vs_gx_push": float(rel.max())}
+    # BC level over 16 seeded sources against the library's FP64 Brandes BC.
+    srcs = [int(x) for x in gx.pick_sources(row, 16)]
+    dt, bcv = timed(lambda: pers.dirgl_bc(P, srcs))
+    ref = gx.betweenness_centrality(G, np.asarray(srcs, np.uint32))[0].cpu().numpy()
+    out[f"dirgl_bc_level_{gname}"] = {"sources": len(srcs), "ms": round(dt * 1e3, 3),
+                                      "ms_per_source": round(dt * 1e3 / len(srcs), 3),
+                                      "max_rel_diff_vs_gx_bc": float(np.max(np.abs(bcv - ref)
+                                                                            / np.maximum(np.abs(ref), 1.0)))}
     best = None
     for _ in range(3):
         t0 = time.perf_counter()

@@ -139,6 +139,14 @@ SIGNATURES: dict[str, tuple] = {
     "gx_part_pr_delta": (INT, [P, U32, U32, F, F]),
     "gx_part_pr": (INT, [P, U32, U32, C.POINTER(U32)]),
     "gx_part_pr_check": (INT, [P, U32, U32, F, P]),
+    "gx_part_bc_init": (INT, [P, U32, U32]),
+    "gx_part_bc_iter": (INT, [P, U32, U32, U32, U64]),
+    "gx_part_bc_forward": (INT, [P, U32, U32, U32, C.POINTER(U32)]),
+    "gx_part_bc_middle": (INT, [P, U32, U32, U32]),
+    "gx_part_bc_backward": (INT, [P, U32, U32, U32]),
+    "gx_part_bc_accumulate": (INT, [P, U32, U32]),
+    "gx_part_bc_check": (INT, [P, U32, U32, C.POINTER(F), C.POINTER(F), C.POINTER(F)]),
+    "gx_part_read_paths": (INT, [P, U32, U32, P]),
     "gx_part_kcore_init1": (INT, [P, U32, U32]),
     "gx_part_kcore_init2": (INT, [P, U32, U32]),
     "gx_part_kcore_step1": (INT, [P, U32, U32, U32, C.POINTER(U32)]),

@@ -16,6 +16,11 @@
 //    over value / nout / residual / delta (FP32, stored as bit patterns):
 //    residual flows to value, deltas above the tolerance are pushed with
 //    float atomics (order dependent, as the reference's);
+//  - BC level (betweennesscentrality/bc_level.cpp:150-560, bc_level_cuda.h:
+//    223-275) over current_length, num_shortest_paths (FP64),
+//    dependency and betweeness_centrality (FP32): level-synchronous forward
+//    passes counting shortest paths, backward passes accumulating
+//    dependencies one node per wavefront (fixed summation order);
 //  - k-core push operators InitializeGraph1/2, KCoreStep1/2, KCoreSanityCheck
 //    (lonestar/analytics/distributed/k-core/kcore_push.cpp:93-360, declared
 //    in kcore_push_cuda.h:133-167) over current_degree / trim / flag;
@@ -45,7 +50,7 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kMaxGrid = 2048;
-constexpr int kFields = 11;
+constexpr int kFields = 15;  // field GX_FIELD_BC_PATHS holds doubles
 
 // Device counters of one operator launch.
 struct PartCounters {
@@ -72,7 +77,7 @@ struct gx_part {
   int64_t n = 0, m = 0;
   uint32_t num_owned = 0, begin_master = 0, with_edges = 0;
   bool loaded = false;
-  gx::DevBuf row, col, w, gid, field[kFields], bits[kFields], hubq, hub_off, counters, push;
+  gx::DevBuf row, col, w, gid, field[kFields], bits[kFields], hubq, hub_off, counters, push, bq;
   PartCounters* host = nullptr;  // pinned
   ~gx_part() {
     if (host) (void)hipHostFree(host);
@@ -89,7 +94,12 @@ void check_part(const gx_part* p) {
   GX_REQUIRE(p != nullptr, "partition handle is NULL");
   GX_REQUIRE(p->loaded, "partition has no graph (gx_part_load)");
 }
-void check_field(int k) { GX_REQUIRE(k >= 0 && k < kFields, "unknown node field"); }
+// Fields read and written as uint32 words (all but the FP64 path counts).
+void check_field(int k) {
+  GX_REQUIRE(k >= 0 && k < kFields, "unknown node field");
+  GX_REQUIRE(k != GX_FIELD_BC_PATHS, "num_shortest_paths is FP64: use gx_part_read_paths");
+}
+void check_bitset_field(int k) { GX_REQUIRE(k >= 0 && k < kFields, "unknown node field"); }
 void check_range(const gx_part* p, uint32_t begin, uint32_t end) {
   GX_REQUIRE(begin <= end && static_cast<int64_t>(end) <= p->n, "range [begin, end) outside the partition");
 }
@@ -370,6 +380,150 @@ __global__ void __launch_bounds__(kBlock) pr_check_kernel(const uint32_t* __rest
   }
 }
 
+// ------------------------------------------------------------------ BC level
+// InitializeGraph (bc_level.cpp:154-160): centrality, path counts and
+// dependencies zero.
+__global__ void bc_init_kernel(uint32_t* __restrict__ bc, uint32_t* __restrict__ dep, double* __restrict__ nsp,
+                               uint32_t begin, uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    bc[v] = 0;
+    dep[v] = 0;
+    nsp[v] = 0.0;
+  }
+}
+
+// InitializeIteration (bc_level.cpp:203-216): the source at length 0 with
+// one path, every other node at infinity with none; dependencies zero.
+__global__ void bc_iter_kernel(uint32_t* __restrict__ len, double* __restrict__ nsp, uint32_t* __restrict__ dep,
+                               const uint32_t* __restrict__ gid, uint32_t begin, uint32_t end, uint32_t inf,
+                               uint64_t src) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const bool is_src = (gid ? static_cast<uint64_t>(gid[v]) : static_cast<uint64_t>(v)) == src;
+    len[v] = is_src ? 0u : inf;
+    nsp[v] = is_src ? 1.0 : 0.0;
+    dep[v] = 0;
+  }
+}
+
+// ForwardPass (bc_level.cpp:290-322): nodes at length r push r + 1; a
+// neighbour reached at r + 1 (first or again) adds the node's path count
+// (bitsets current_length / num_shortest_paths), each counted in dga.
+struct BcForwardOp {
+  const int32_t* __restrict__ col;
+  uint32_t* len;
+  double* nsp;
+  unsigned long long* len_bits;
+  unsigned long long* nsp_bits;
+  PartCounters* c;
+  uint32_t r;
+  unsigned dga = 0;
+  __device__ __forceinline__ bool admit(uint32_t v) { return len[v] == r; }
+  __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
+    if (!valid) return;
+    const uint32_t v = static_cast<uint32_t>(col[e]);
+    const uint32_t nd = r + 1;
+    const uint32_t old = atomicMin(&len[v], nd);
+    if (old >= nd) {
+      if (old > nd) set_bit(len_bits, v);
+      atomicAdd(&nsp[v], nsp[u]);
+      set_bit(nsp_bits, v);
+      ++dga;
+    }
+  }
+  __device__ __forceinline__ void after_step() {}
+  __device__ __forceinline__ void finish() { flush_counts(c, dga, 0ull); }
+};
+
+// MiddleSync (bc_level.cpp:368-374): reached nodes marked in the path-count
+// bitset.
+__global__ void bc_middle_kernel(const uint32_t* __restrict__ len, unsigned long long* __restrict__ nsp_bits,
+                                 uint32_t begin, uint32_t end, uint32_t inf) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x)
+    if (len[v] != inf) set_bit(nsp_bits, v);
+}
+
+// BackwardPass, part 1: the nodes of [begin, end) at length r, in a queue.
+__global__ void __launch_bounds__(kBlock) bc_level_queue_kernel(const uint32_t* __restrict__ len, uint32_t begin,
+                                                                uint32_t end, uint32_t r, uint32_t* __restrict__ q,
+                                                                PartCounters* c) {
+  __shared__ LdsQueue<1024> lq;
+  lq_init(lq);
+  const uint32_t n = end - begin;
+  for (uint32_t b0 = blockIdx.x * kBlock; b0 < n; b0 += gridDim.x * kBlock) {
+    const uint32_t i = b0 + threadIdx.x;
+    const bool hit = i < n && len[begin + i] == r;
+    lq_push(lq, hit, begin + i);
+    lq_flush(lq, q, &c->active, n, &c->overflow, false);
+  }
+  lq_flush(lq, q, &c->active, n, &c->overflow, true);
+}
+
+// BackwardPass, part 2 (bc_level.cpp:415-433): one wavefront per queued node
+// sums (1 + dependency) / paths over its successors at length r + 1 in a
+// fixed tree order, then scales by the node's own path count; FP32 result
+// as the reference's field (bitset dependency when a successor exists).
+__global__ void __launch_bounds__(kBlock) bc_backward_kernel(const int32_t* __restrict__ row,
+                                                             const int32_t* __restrict__ col,
+                                                             const uint32_t* __restrict__ len,
+                                                             const double* __restrict__ nsp,
+                                                             uint32_t* __restrict__ dep,
+                                                             unsigned long long* __restrict__ dep_bits,
+                                                             const uint32_t* __restrict__ q, const PartCounters* c,
+                                                             uint32_t r) {
+  const unsigned count = c->active;
+  const unsigned lane = lane_id();
+  const unsigned waves = gridDim.x * (kBlock / kWave);
+  for (unsigned i = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; i < count; i += waves) {
+    const uint32_t u = q[i];
+    double sum = 0.0;
+    unsigned found = 0;
+    for (int32_t e = row[u] + static_cast<int32_t>(lane); e < row[u + 1]; e += kWave) {
+      const uint32_t v = static_cast<uint32_t>(col[e]);
+      if (len[v] == r + 1) {
+        sum += (1.0 + static_cast<double>(__uint_as_float(dep[v]))) / nsp[v];
+        found = 1;
+      }
+    }
+    sum = wave_sum(sum);
+    found = wave_sum(found);
+    if (lane == 0 && found) {
+      const float d = static_cast<float>(static_cast<double>(__uint_as_float(dep[u])) + sum);
+      dep[u] = __float_as_uint(static_cast<float>(static_cast<double>(d) * nsp[u]));
+      set_bit(dep_bits, u);
+    }
+  }
+}
+
+// BC (bc_level.cpp:480-485): a positive dependency adds to the centrality.
+__global__ void bc_accumulate_kernel(uint32_t* __restrict__ bc, const uint32_t* __restrict__ dep, uint32_t begin,
+                                     uint32_t end) {
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const float d = __uint_as_float(dep[v]);
+    if (d > 0.0f) bc[v] = __float_as_uint(__uint_as_float(bc[v]) + d);
+  }
+}
+
+// Sanity (bc_level.cpp:525-545): sum, maximum and minimum centrality.
+__global__ void __launch_bounds__(kBlock) bc_check_kernel(const uint32_t* __restrict__ bc, uint32_t begin,
+                                                          uint32_t end, PartCounters* c) {
+  uint32_t mx = 0, mn = 0x7f800000u;
+  double sum = 0;
+  for (uint32_t v = begin + blockIdx.x * blockDim.x + threadIdx.x; v < end; v += gridDim.x * blockDim.x) {
+    const uint32_t b = bc[v];
+    mx = max(mx, b);
+    mn = min(mn, b);
+    sum += __uint_as_float(b);
+  }
+  mx = wave_max(mx);
+  mn = wave_min(mn);
+  sum = wave_sum(sum);
+  if (lane_id() == 0) {
+    atomicMax(&c->max_value, mx);
+    atomicMin(&c->min_value, mn);
+    atomicAdd(&c->sum_value, sum);
+  }
+}
+
 // ------------------------------------------------------------- k-core push
 // InitializeGraph1 (kcore_push.cpp:140-181): flag set,
 // trim and current_degree zero.
@@ -560,14 +714,16 @@ extern "C" gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint
     p->gid.release();
   }
   for (int k = 0; k < kFields; ++k) {
-    p->field[k].ensure(n * 4);
-    GX_HIP(hipMemset(p->field[k].p, 0, n * 4));
+    const uint64_t fb = k == GX_FIELD_BC_PATHS ? 8 : 4;  // path counts are FP64
+    p->field[k].ensure(n * fb);
+    GX_HIP(hipMemset(p->field[k].p, 0, n * fb));
     p->bits[k].ensure(p->words() * 8);
     GX_HIP(hipMemset(p->bits[k].p, 0, p->words() * 8));
   }
   p->hubq.ensure(n * 4);
   p->hub_off.ensure((n + 1) * 4);
   p->push.ensure(n * 4);
+  p->bq.ensure(n * 4);
   p->loaded = true;
   GX_API_END
 }
@@ -661,7 +817,7 @@ extern "C" gx_status gx_part_read(const gx_part* p, int field, uint32_t begin, u
 extern "C" gx_status gx_part_bitset_get(const gx_part* p, int field, uint64_t* words) {
   GX_API_BEGIN
   check_part(p);
-  check_field(field);
+  check_bitset_field(field);
   check_ptr(words, "words");
   GX_HIP(hipStreamSynchronize(p->stream));
   GX_HIP(hipMemcpy(words, p->bits[field].p, p->words() * 8, hipMemcpyDeviceToHost));
@@ -671,7 +827,7 @@ extern "C" gx_status gx_part_bitset_get(const gx_part* p, int field, uint64_t* w
 extern "C" gx_status gx_part_bitset_reset(gx_part* p, int field, uint32_t begin, uint32_t end) {
   GX_API_BEGIN
   check_part(p);
-  check_field(field);
+  check_bitset_field(field);
   check_range(p, begin, end);
   if (end > begin) {
     const uint32_t nw = ((end - 1) >> 6) - (begin >> 6) + 1;
@@ -935,6 +1091,130 @@ extern "C" gx_status gx_part_pr_check(gx_part* p, uint32_t begin, uint32_t end, 
   out->sum_value = c.sum_value;
   out->sum_residual = c.sum_res;
   out->residual_over_tolerance = c.over_tol;
+  GX_API_END
+}
+
+static double* paths(gx_part* p) { return p->field[GX_FIELD_BC_PATHS].as<double>(); }
+
+extern "C" gx_status gx_part_bc_init(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    bc_init_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_BC_VALUE),
+                                                                    p->f(GX_FIELD_BC_DEPENDENCY), paths(p), begin, end);
+    GX_LAUNCHED("bc_init_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bc_iter(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity,
+                                     uint64_t src_gid) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    bc_iter_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(
+        p->f(GX_FIELD_BC_LENGTH), paths(p), p->f(GX_FIELD_BC_DEPENDENCY), p->gid.p ? p->gid.as<uint32_t>() : nullptr,
+        begin, end, infinity, src_gid);
+    GX_LAUNCHED("bc_iter_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bc_forward(gx_part* p, uint32_t begin, uint32_t end, uint32_t round, uint32_t* dga) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  BcForwardOp op{p->col.as<int32_t>(), p->f(GX_FIELD_BC_LENGTH), paths(p), p->b(GX_FIELD_BC_LENGTH),
+                 p->b(GX_FIELD_BC_PATHS), p->counters.as<PartCounters>(), round};
+  run_edges(p, begin, end, op);
+  const PartCounters& c = read_counters(p);
+  if (dga) *dga = c.active;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bc_middle(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    bc_middle_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_BC_LENGTH),
+                                                                      p->b(GX_FIELD_BC_PATHS), begin, end, infinity);
+    GX_LAUNCHED("bc_middle_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bc_backward(gx_part* p, uint32_t begin, uint32_t end, uint32_t round) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  reset_counters(p);
+  if (end > begin) {
+    PartCounters* c = p->counters.as<PartCounters>();
+    bc_level_queue_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_BC_LENGTH), begin, end,
+                                                                         round, p->bq.as<uint32_t>(), c);
+    GX_LAUNCHED("bc_level_queue_kernel");
+    bc_backward_kernel<<<kMaxGrid, kBlock, 0, p->stream>>>(
+        p->row.as<int32_t>(), p->col.as<int32_t>(), p->f(GX_FIELD_BC_LENGTH), paths(p), p->f(GX_FIELD_BC_DEPENDENCY),
+        p->b(GX_FIELD_BC_DEPENDENCY), p->bq.as<uint32_t>(), c, round);
+    GX_LAUNCHED("bc_backward_kernel");
+  }
+  read_counters(p);
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bc_accumulate(gx_part* p, uint32_t begin, uint32_t end) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    bc_accumulate_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_BC_VALUE),
+                                                                          p->f(GX_FIELD_BC_DEPENDENCY), begin, end);
+    GX_LAUNCHED("bc_accumulate_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(p->stream));
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_bc_check(gx_part* p, uint32_t begin, uint32_t end, float* sum, float* max_bc,
+                                      float* min_bc) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  *p->host = PartCounters{};
+  p->host->min_value = 0x7f800000u;
+  GX_HIP(hipMemcpyAsync(p->counters.p, p->host, sizeof(PartCounters), hipMemcpyHostToDevice, p->stream));
+  if (end > begin) {
+    bc_check_kernel<<<grid_of(end - begin), kBlock, 0, p->stream>>>(p->f(GX_FIELD_BC_VALUE), begin, end,
+                                                                     p->counters.as<PartCounters>());
+    GX_LAUNCHED("bc_check_kernel");
+  }
+  const PartCounters& c = read_counters(p);
+  float mx, mn;
+  std::memcpy(&mx, &c.max_value, 4);
+  std::memcpy(&mn, &c.min_value, 4);
+  if (sum) *sum = static_cast<float>(c.sum_value);
+  if (max_bc) *max_bc = mx;
+  if (min_bc) *min_bc = mn;
+  GX_API_END
+}
+
+extern "C" gx_status gx_part_read_paths(const gx_part* p, uint32_t begin, uint32_t end, double* host) {
+  GX_API_BEGIN
+  check_part(p);
+  check_range(p, begin, end);
+  if (end > begin) {
+    check_ptr(host, "host");
+    GX_HIP(hipStreamSynchronize(p->stream));
+    GX_HIP(hipMemcpy(host, p->field[GX_FIELD_BC_PATHS].as<double>() + begin, static_cast<size_t>(end - begin) * 8,
+                     hipMemcpyDeviceToHost));
+  }
   GX_API_END
 }
 

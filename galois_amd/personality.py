@@ -7,7 +7,10 @@ the nodes with edges until no vertex is active), the same structure in
 sssp/sssp_push.cpp:95-300 (weights pushed) and connected-components/
 cc_push.cpp:85-230 (labels pushed, FirstItr over the nodes with edges), and
 pagerank/pagerank_push.cpp:110-300 (ResetGraph, InitializeGraph, then
-PageRank_delta + PageRank rounds until no node pushes), and
+PageRank_delta + PageRank rounds until no node pushes),
+betweennesscentrality/bc_level.cpp:440-720 (per source: InitializeIteration,
+forward passes until no update, then MiddleSync, backward passes from the
+deepest level and the accumulation into the centrality), and
 lonestar/analytics/distributed/k-core/kcore_push.cpp:93-318
 (InitializeGraph1 over all nodes, InitializeGraph2 over the nodes with edges,
 KCoreStep1 + KCoreStep2 rounds until no vertex dies).  With one host there is
@@ -23,6 +26,7 @@ from ._lib import check, lib
 
 DIST_CURRENT, DIST_OLD, CURRENT_DEGREE, TRIM, FLAG, COMP_CURRENT, COMP_OLD = range(7)
 PR_VALUE, PR_NOUT, PR_RESIDUAL, PR_DELTA = range(7, 11)
+BC_LENGTH, BC_DEPENDENCY, BC_VALUE, BC_PATHS = range(11, 15)
 PR_ALPHA = 1.0 - 0.85  # pagerank_push.cpp:68 (reset probability)
 
 
@@ -182,6 +186,37 @@ class Partition:
     def read_float(self, field: int, begin: int = 0, end: int | None = None) -> np.ndarray:
         return self.read(field, begin, end).view(np.float32)
 
+    def bc_init(self, begin: int, end: int) -> None:
+        check(lib().gx_part_bc_init(self._h, begin, end))
+
+    def bc_iter(self, begin: int, end: int, infinity: int, src_gid: int) -> None:
+        check(lib().gx_part_bc_iter(self._h, begin, end, infinity, src_gid))
+
+    def bc_forward(self, begin: int, end: int, round_: int) -> int:
+        d = C.c_uint32()
+        check(lib().gx_part_bc_forward(self._h, begin, end, round_, C.byref(d)))
+        return d.value
+
+    def bc_middle(self, begin: int, end: int, infinity: int) -> None:
+        check(lib().gx_part_bc_middle(self._h, begin, end, infinity))
+
+    def bc_backward(self, begin: int, end: int, round_: int) -> None:
+        check(lib().gx_part_bc_backward(self._h, begin, end, round_))
+
+    def bc_accumulate(self, begin: int, end: int) -> None:
+        check(lib().gx_part_bc_accumulate(self._h, begin, end))
+
+    def bc_check(self, begin: int, end: int) -> tuple[float, float, float]:
+        s, mx, mn = C.c_float(), C.c_float(), C.c_float()
+        check(lib().gx_part_bc_check(self._h, begin, end, C.byref(s), C.byref(mx), C.byref(mn)))
+        return s.value, mx.value, mn.value
+
+    def read_paths(self, begin: int = 0, end: int | None = None) -> np.ndarray:
+        end = self.n if end is None else end
+        out = np.empty(max(end - begin, 1), dtype=np.float64)
+        check(lib().gx_part_read_paths(self._h, begin, end, out.ctypes.data))
+        return out[:end - begin]
+
     def kcore_init1(self, begin: int, end: int) -> None:
         check(lib().gx_part_kcore_init1(self._h, begin, end))
 
@@ -281,6 +316,29 @@ def dirgl_pagerank(part: Partition, tolerance: float, alpha: float = PR_ALPHA, m
         if not (rounds < max_iterations and active):
             break
     return part.read_float(PR_VALUE), rounds
+
+
+def dirgl_bc(part: Partition, sources):
+    """bc_level.cpp's BC::go over the given source global ids for one host;
+    returns the betweeness_centrality of all local nodes (float32)."""
+    a0, a1 = part.range(RANGE_ALL)
+    w0, w1 = part.range(RANGE_WITH_EDGES)
+    m0, m1 = part.range(RANGE_MASTERS)
+    part.bc_init(a0, a1)
+    for s in sources:
+        part.bc_iter(a0, a1, INFINITY, int(s))
+        rounds = 0
+        while True:
+            dga = part.bc_forward(w0, w1, rounds)
+            rounds += 1
+            if not dga:
+                break
+        if rounds > 2:
+            part.bc_middle(m0, m1, INFINITY)
+            for r in range(rounds - 2, 0, -1):
+                part.bc_backward(w0, w1, r)
+            part.bc_accumulate(m0, m1)
+    return part.read_float(BC_VALUE)
 
 
 def dirgl_kcore(part: Partition, k: int, max_iterations: int = 10000):

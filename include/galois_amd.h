@@ -346,6 +346,10 @@ gx_status gx_part_range(const gx_part* p, int which, uint32_t* begin, uint32_t* 
 #define GX_FIELD_PR_NOUT 8        /* value, residual and delta are float bit patterns */
 #define GX_FIELD_PR_RESIDUAL 9
 #define GX_FIELD_PR_DELTA 10
+#define GX_FIELD_BC_LENGTH 11     /* bc_level NodeData (bc_level.cpp:86-92): current_length, */
+#define GX_FIELD_BC_DEPENDENCY 12 /* dependency and betweeness_centrality (float bit */
+#define GX_FIELD_BC_VALUE 13      /* patterns), num_shortest_paths (FP64: read with */
+#define GX_FIELD_BC_PATHS 14      /* gx_part_read_paths; bitset only otherwise) */
 /* get/set/add/min_node_<field>_cuda (bfs_push_cuda.h:11-17); *changed = 1 when
  * min lowered the value. */
 gx_status gx_part_get(const gx_part* p, int field, uint32_t lid, uint32_t* value);
@@ -422,6 +426,28 @@ gx_status gx_part_pr_init(gx_part* p, uint32_t begin, uint32_t end, float alpha)
 gx_status gx_part_pr_delta(gx_part* p, uint32_t begin, uint32_t end, float alpha, float tolerance);
 gx_status gx_part_pr(gx_part* p, uint32_t begin, uint32_t end, uint32_t* active_vertices);
 gx_status gx_part_pr_check(gx_part* p, uint32_t begin, uint32_t end, float tolerance, gx_pr_sanity* out);
+/* BC level operators (betweennesscentrality/bc_level.cpp:150-560,
+ * bc_level_cuda.h:223-275).
+ * InitializeGraph_cuda: centrality, path counts, dependencies zero.
+ * InitializeIteration_cuda: the source (global id) at length 0 with one
+ *   path, every other node at infinity with none; dependency zero.
+ * ForwardPass_cuda(round): nodes at length round push round + 1; a neighbour
+ *   reached at round + 1 adds the node's path count (FP64); *dga counts those
+ *   updates (bitsets current_length, num_shortest_paths).
+ * MiddleSync_cuda: reached nodes marked in the num_shortest_paths bitset.
+ * BackwardPass_cuda(round): nodes at length round take dependency =
+ *   paths * sum over successors at round + 1 of (1 + dependency) / paths
+ *   (FP32 field, per-node sum in a fixed order; bitset dependency).
+ * BC_cuda: a positive dependency adds to betweeness_centrality.
+ * Sanity_cuda: sum, maximum and minimum centrality. */
+gx_status gx_part_bc_init(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_bc_iter(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity, uint64_t src_gid);
+gx_status gx_part_bc_forward(gx_part* p, uint32_t begin, uint32_t end, uint32_t round, uint32_t* dga);
+gx_status gx_part_bc_middle(gx_part* p, uint32_t begin, uint32_t end, uint32_t infinity);
+gx_status gx_part_bc_backward(gx_part* p, uint32_t begin, uint32_t end, uint32_t round);
+gx_status gx_part_bc_accumulate(gx_part* p, uint32_t begin, uint32_t end);
+gx_status gx_part_bc_check(gx_part* p, uint32_t begin, uint32_t end, float* sum, float* max_bc, float* min_bc);
+gx_status gx_part_read_paths(const gx_part* p, uint32_t begin, uint32_t end, double* host);
 /* k-core push operators (kcore_push.cpp:93-360: InitializeGraph2, InitializeGraph1,
  * KCoreStep2, KCoreStep1, KCoreSanityCheck).
  * InitializeGraph1_cuda: flag = 1, trim = current_degree = 0.

@@ -1,9 +1,11 @@
 """D-IrGL personality partition (gx_part_*, galois_amd.personality): the
 reference's D-IrGL host loops for one host (bfs_push.cpp, sssp_push.cpp,
-cc_push.cpp, pagerank_push.cpp, kcore_push.cpp) driven through the C ABI,
+cc_push.cpp, pagerank_push.cpp, bc_level.cpp, kcore_push.cpp) driven through
+the C ABI,
 against the oracle and the golden vectors: BFS and SSSP distances bit-exact
 (with D-IrGL's infinity), CC labels and k-core membership bit-exact,
-PageRank values within the FP32-atomics tolerance, sanity-check reductions,
+PageRank values within the FP32-atomics tolerance, BC within the FP32
+dependency tolerance and path counts exact, sanity-check reductions,
 split ranges, change bitsets and field accessors."""
 from __future__ import annotations
 
@@ -163,6 +165,40 @@ def test_pagerank_push(gx, golden):
         assert san.max_residual == res.max() and san.min_residual == res.min()
         assert abs(san.sum_value - float(value.astype(np.float64).sum())) < 1e-4
         assert san.residual_over_tolerance == int((res > np.float32(tol)).sum())
+
+
+def test_bc_level(gx, golden, oracle, rmat16):
+    """BC level against the library's FP64 Brandes BC (pinned to the oracle
+    and networkx elsewhere) on the golden graphs (all sources) and RMAT-16
+    (16 seeded sources): FP32 dependencies, 1e-4 relative; path counts of
+    the last source against a BFS path count; Sanity's reductions."""
+    from galois_amd.personality import BC_LENGTH, RANGE_MASTERS, dirgl_bc
+    cases = [(g["name"], *csr_arrays(g)[:2], list(range(g["n"]))) for g in golden["undirected"]]
+    G, row, col, _ = rmat16
+    cases.append(("rmat16", row, col, [int(x) for x in gx.pick_sources(row, 16)]))
+    for name, row, col, srcs in cases:
+        print(f"dirgl bc {name} sources={len(srcs)}", flush=True)
+        P = _part(row, col)
+        bc = dirgl_bc(P, srcs)
+        Gx = gx.Graph.from_csr(row, col, None, symmetric=True)
+        exp = gx.betweenness_centrality(Gx, np.asarray(srcs, np.uint32))[0].cpu().numpy()
+        scale = np.maximum(np.abs(exp), 1.0)
+        assert np.all(np.abs(bc - exp) <= 1e-4 * scale), (name, float(np.max(np.abs(bc - exp) / scale)))
+        # shortest-path counts of the last source: sigma[v] = sum over BFS parents
+        lv = oracle.bfs(row, col, srcs[-1], threads=8).astype(np.int64)
+        sigma = np.zeros(len(lv))
+        sigma[srcs[-1]] = 1.0
+        order = np.argsort(lv, kind="stable")
+        rr = np.asarray(row, np.int64)
+        for u in order:
+            if lv[u] == ORACLE_INF:
+                break
+            for v in col[rr[u]:rr[u + 1]]:
+                if lv[v] == lv[u] + 1:
+                    sigma[v] += sigma[u]
+        np.testing.assert_array_equal(P.read_paths(), sigma, err_msg=name)
+        s, mx, mn = P.bc_check(*P.range(RANGE_MASTERS))
+        assert mx == bc.max() and mn == bc.min() and abs(s - float(bc.astype(np.float64).sum())) <= 1e-5 * max(1.0, s)
 
 
 def test_kcore_push(gx, golden, oracle, rmat16):
