@@ -137,6 +137,12 @@ def test_tuning_knobs():
     assert gx.tune_get("frontier.hub_degree") == -1
     with pytest.raises(GxError):
         gx.tune_set("frontier.hub_degree", 3)  # below the range
+    assert gx.tune_get("pr.blocking") == -1
+    gx.tune_set("pr.blocking", 0)
+    assert gx.tune_get("pr.blocking") == 0
+    gx.tune_set("pr.blocking", -1)
+    with pytest.raises(GxError):
+        gx.tune_set("pr.blocking", 2)  # 0 or 1 only
     with pytest.raises(GxError):
         gx.tune_set("no.such.knob", 1)
     with pytest.raises(GxError):
