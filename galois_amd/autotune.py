@@ -14,7 +14,9 @@ knobs respond to (hub count, gathered working set).
 Knobs and candidates:
   frontier.hub_degree  degree above which a frontier vertex is expanded by the
                        whole grid (BFS, SSSP, CC, k-core, BC)
-  pr.blocking          FP32 PageRank: propagation-blocked sweep (1) or merge-path pull (0)
+  pr.blocking          FP32 PageRank: propagation-blocked sweep (1) or merge-path pull (0);
+                       2 also sends FP64 through the blocked sweep (not a
+                       candidate: it changes FP64 results in the last bit)
   pr.hot_sources       sources of the first PageRank pull pass (0 = one pass)
 """
 from __future__ import annotations

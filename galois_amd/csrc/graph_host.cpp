@@ -46,7 +46,7 @@ struct Knob {
 Knob g_knobs[] = {
     {"frontier.hub_degree", {-1}, 64, int64_t(1) << 30},
     {"pr.hot_sources", {-1}, 0, int64_t(1) << 31},
-    {"pr.blocking", {-1}, 0, 1},
+    {"pr.blocking", {-1}, 0, 2},
 };
 Knob* find_knob(const char* key) {
   if (!key) return nullptr;

@@ -185,6 +185,33 @@ struct PbFin {
   }
 };
 
+// Row finalisation of the propagation-blocked FP64 sweep: x' = base +
+// alpha*sum in FP64, as PrFin<double>.
+struct PbFin64 {
+  const PrState* state;
+  const double* __restrict__ x_old;
+  double* __restrict__ x_new;
+  double* __restrict__ y_new;
+  const uint32_t* __restrict__ outdeg;
+  double base, alpha;
+  __device__ __forceinline__ void load() {
+    base = state->base;
+    alpha = state->alpha;
+  }
+  __device__ __forceinline__ void operator()(int64_t r, double sum, double& resid, double& dang) const {
+    const double xn = base + alpha * sum;
+    resid += fabs(xn - rs::ld_stream(&x_old[r]));
+    __builtin_nontemporal_store(xn, &x_new[r]);
+    const uint32_t d = rs::ld_stream(&outdeg[r]);
+    if (d) {
+      y_new[r] = xn / static_cast<double>(d);
+    } else {
+      y_new[r] = 0.0;
+      dang += xn;
+    }
+  }
+};
+
 // Hot/cold split: per row, the number of in-edges from sources < hot (a
 // prefix: sources are ascending within a row) and the rest.
 __global__ void split_count_kernel(const int32_t* __restrict__ rowoff, const int32_t* __restrict__ col, int64_t n,
@@ -346,10 +373,34 @@ static int64_t hot_sources(int64_t n, size_t real_bytes) {
   return (h <= 0 || h >= n) ? 0 : h;
 }
 
-// FP32 iterations use the propagation-blocked sweep unless the knob
-// "pr.blocking" is 0 or the graph is outside its table limits.
+// Knob "pr.blocking": 1 (default) = propagation-blocked sweep for FP32,
+// 2 = also for FP64 (its sums then depend on the addition order in the last
+// bit), 0 = merge-path pull for both.  Graphs outside the layout's table
+// limits always use the pull.
 static bool use_blocking(const gx_graph* g, size_t real_bytes) {
-  return real_bytes == sizeof(float) && tune_value("pr.blocking", 1) != 0 && pb::supported(g->n, g->m);
+  const int64_t k = tune_value("pr.blocking", 1);
+  const bool on = real_bytes == sizeof(float) ? k != 0 : k == 2;
+  return on && pb::supported(g->n, g->m);
+}
+
+// Phase 1, phase 2 and the sliced-bin finalisation of one blocked sweep.
+template <class Real, class Fin>
+static uint32_t launch_blocked(const PbLayout& L, const Real* y, Real* vals, int64_t n, const Fin& fin,
+                               double* part_resid, double* part_dang, const unsigned* stop, hipStream_t s) {
+  using Acc = typename pb::PbAcc<Real>::T;
+  const pb::Job* jobs = L.jobs.as<pb::Job>();
+  pb::gather_kernel<Real><<<static_cast<unsigned>(L.npieces), pb::kBlock, 0, s>>>(
+      L.pieces.as<pb::Piece>(), L.src16.as<uint16_t>(), y, n, vals, stop);
+  GX_LAUNCHED("pb::gather_kernel");
+  pb::accum_kernel<Real, Fin><<<static_cast<unsigned>(L.ntasks), pb::kBlock, 0, s>>>(
+      L.tasks.as<pb::Task>(), jobs, jobs + L.njobs[0], jobs + L.njobs[0] + L.njobs[1], vals, L.dst16.as<uint16_t>(),
+      L.gacc.as<Acc>(), n, fin, part_resid, part_dang, stop);
+  GX_LAUNCHED("pb::accum_kernel");
+  if (L.nmulti == 0) return 2;
+  pb::multi_kernel<Real, Fin><<<static_cast<unsigned>(L.nmulti), pb::kBlock, 0, s>>>(
+      L.multi.as<pb::Multi>(), L.gacc.as<Acc>(), n, fin, part_resid + L.ntasks, part_dang + L.ntasks, stop);
+  GX_LAUNCHED("pb::multi_kernel");
+  return 3;
 }
 
 // Normalised pull PageRank over the degree-ordered transpose.
@@ -364,6 +415,7 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
   rs::RowPlan P{}, PA{};
   if (blocked) {
     pb::build_layout(g);
+    D.pb.vals.ensure(g->m * sizeof(Real));  // grows once for FP64 contributions
   } else if (hot > 0) {
     build_split(g, hot);
     PA = rs::build_plan(g, D.rowA.as<int32_t>(), D.colA.as<int32_t>(), n, D.hot_edges, D.planA);
@@ -406,35 +458,24 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
     const uint32_t end = std::min(max_iter, it + batch);
     for (; it < end; ++it) {
       const int nxt = cur ^ 1;
-      if constexpr (std::is_same<Real, float>::value) {
-        if (blocked) {
-          const pb::Job* jobs = L.jobs.as<pb::Job>();
-          ktimer.begin();
-          pb::gather_kernel<float><<<static_cast<unsigned>(L.npieces), pb::kBlock, 0, s>>>(
-              L.pieces.as<pb::Piece>(), L.src16.as<uint16_t>(), y[cur], n, L.vals.as<float>(), &state->converged);
-          GX_LAUNCHED("pb::gather_kernel");
+      if (blocked) {
+        ktimer.begin();
+        if constexpr (std::is_same<Real, float>::value) {
           PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, {}, {}};
-          pb::accum_kernel<PbFin><<<static_cast<unsigned>(L.ntasks), pb::kBlock, 0, s>>>(
-              L.tasks.as<pb::Task>(), jobs, jobs + L.njobs[0], jobs + L.njobs[0] + L.njobs[1], L.vals.as<float>(),
-              L.dst16.as<uint16_t>(), L.gacc.as<unsigned long long>(), n, fin, part_resid, part_dang,
-              &state->converged);
-          GX_LAUNCHED("pb::accum_kernel");
-          launches += 2;
-          if (L.nmulti > 0) {
-            pb::multi_kernel<PbFin><<<static_cast<unsigned>(L.nmulti), pb::kBlock, 0, s>>>(
-                L.multi.as<pb::Multi>(), L.gacc.as<unsigned long long>(), n, fin, part_resid + L.ntasks,
-                part_dang + L.ntasks, &state->converged);
-            GX_LAUNCHED("pb::multi_kernel");
-            launches += 1;
-          }
-          ktimer.end();
-          pr_reduce_kernel<<<1, kBlock, 0, s>>>(state, part_resid, part_dang,
-                                                static_cast<int>(L.ntasks + L.nmulti), static_cast<unsigned>(nxt));
-          GX_LAUNCHED("pr_reduce_kernel");
-          launches += 1;
-          cur = nxt;
-          continue;
+          launches += launch_blocked<float>(L, y[cur], L.vals.as<float>(), n, fin, part_resid, part_dang,
+                                            &state->converged, s);
+        } else {
+          PbFin64 fin{state, x[cur], x[nxt], y[nxt], outdeg, 0.0, 0.0};
+          launches += launch_blocked<double>(L, y[cur], L.vals.as<double>(), n, fin, part_resid, part_dang,
+                                             &state->converged, s);
         }
+        ktimer.end();
+        pr_reduce_kernel<<<1, kBlock, 0, s>>>(state, part_resid, part_dang, static_cast<int>(L.ntasks + L.nmulti),
+                                              static_cast<unsigned>(nxt));
+        GX_LAUNCHED("pr_reduce_kernel");
+        launches += 1;
+        cur = nxt;
+        continue;
       }
       PrSrc<Real> src{y[cur]};
       PrFin<Real> fin{state, x[cur], x[nxt], y[nxt], outdeg, part, Real(0), Real(0)};

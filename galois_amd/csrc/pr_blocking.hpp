@@ -19,10 +19,13 @@
 //           vals[p] is added into an LDS accumulator per destination
 //           (4 + 2 bytes read per edge), then the bin's rows are finalised.
 //
-// Row sums are exact: every contribution (0 <= y <= 1 for the normalised
-// iteration) is rounded to a 64-bit fixed-point number with 2^-62 resolution
-// and added with integer atomics, so the sum does not depend on the order of
-// the additions and the result is deterministic.  Slices of a heavy bin add
+// FP32 row sums are exact: every contribution (0 <= y <= 1 for the
+// normalised iteration) is rounded to a 64-bit fixed-point number with 2^-62
+// resolution and added with integer atomics, so the sum does not depend on
+// the order of the additions and the result is deterministic.  The FP64
+// variant streams FP64 contributions and adds them with FP64 LDS atomics
+// (2^-62 fixed point would be coarser than FP64 for small contributions);
+// its sums depend on the addition order in the last bit.  Slices of a heavy bin add
 // their LDS sums into a per-bin global accumulator that multi_kernel
 // finalises (and clears for the next iteration).
 //
@@ -155,21 +158,38 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict_
   }
 }
 
+// Row-sum accumulator per precision: FP32 contributions are summed exactly
+// as 64-bit fixed point (integer atomics, order independent); FP64
+// contributions are summed with FP64 atomics (order dependent in the last
+// bit).
+template <class Real>
+struct PbAcc;
+template <>
+struct PbAcc<float> {
+  using T = unsigned long long;
+  static __device__ __forceinline__ T conv(float v) { return to_fixed(v); }
+};
+template <>
+struct PbAcc<double> {
+  using T = double;
+  static __device__ __forceinline__ T conv(double v) { return v; }
+};
+
 // Adds the entries of one job class into the LDS sums.  Lanes are grouped
 // LPJ to a job; lane `sub` of a group takes entries sub + LPJ*k, k < 8.  A
 // wavefront loads the entries of kU job groups before it adds any of them,
 // so that kU x 8 loads per lane are in flight.
-template <int LPJ>
+template <int LPJ, class Real>
 __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_t j0, uint32_t j1,
-                                           const float* __restrict__ vals, const uint16_t* __restrict__ dst16,
-                                           unsigned long long* acc) {
+                                           const Real* __restrict__ vals, const uint16_t* __restrict__ dst16,
+                                           typename PbAcc<Real>::T* acc) {
   constexpr int kGroups = kWave / LPJ;
-  constexpr int kU = 4;
+  constexpr int kU = sizeof(Real) == 4 ? 4 : 2;
   const unsigned lane = lane_id(), wave = threadIdx.x / kWave;
   const unsigned grp = lane / LPJ, sub = lane % LPJ;
   for (uint32_t jb = j0 + wave * kGroups * kU; jb < j1; jb += kWaves * kGroups * kU) {
     Job J[kU];
-    float v[kU][8];
+    Real v[kU][8];
     uint32_t d[kU][8];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -182,7 +202,7 @@ __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const uint32_t i = sub + LPJ * k;
-        v[u][k] = 0.0f;
+        v[u][k] = Real(0);
         d[u][k] = 0;
         if (i < J[u].len) {
           v[u][k] = rs::ld_stream(&vals[J[u].start + i]);
@@ -193,29 +213,30 @@ __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_
     for (int u = 0; u < kU; ++u)
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (sub + LPJ * k < J[u].len) atomicAdd(&acc[d[u][k]], to_fixed(v[u][k]));
+        if (sub + LPJ * k < J[u].len) atomicAdd(&acc[d[u][k]], PbAcc<Real>::conv(v[u][k]));
   }
 }
 
-// Phase 2: one task.  Fin::operator()(row, fixed-point sum, resid, dang)
+// Phase 2: one task.  Fin::operator()(row, accumulated sum, resid, dang)
 // finalises a row.
-template <class Fin>
+template <class Real, class Fin>
 __global__ void __launch_bounds__(kBlock) accum_kernel(const Task* __restrict__ tasks, const Job* __restrict__ jl,
                                                        const Job* __restrict__ jm, const Job* __restrict__ js,
-                                                       const float* __restrict__ vals,
+                                                       const Real* __restrict__ vals,
                                                        const uint16_t* __restrict__ dst16,
-                                                       unsigned long long* __restrict__ gacc, int64_t n, Fin fin,
+                                                       typename PbAcc<Real>::T* __restrict__ gacc, int64_t n, Fin fin,
                                                        double* __restrict__ part_resid,
                                                        double* __restrict__ part_dang, const unsigned* stop) {
+  using Acc = typename PbAcc<Real>::T;
   if (*stop) return;
-  __shared__ unsigned long long acc[kSpan];
+  __shared__ Acc acc[kSpan];
   __shared__ double red[kWaves];
   const Task T = tasks[blockIdx.x];
-  for (int i = threadIdx.x; i < kSpan; i += kBlock) acc[i] = 0;
+  for (int i = threadIdx.x; i < kSpan; i += kBlock) acc[i] = Acc(0);
   __syncthreads();
-  accum_jobs<kWave>(jl, T.job0[0], T.job1[0], vals, dst16, acc);
-  accum_jobs<8>(jm, T.job0[1], T.job1[1], vals, dst16, acc);
-  accum_jobs<1>(js, T.job0[2], T.job1[2], vals, dst16, acc);
+  accum_jobs<kWave, Real>(jl, T.job0[0], T.job1[0], vals, dst16, acc);
+  accum_jobs<8, Real>(jm, T.job0[1], T.job1[1], vals, dst16, acc);
+  accum_jobs<1, Real>(js, T.job0[2], T.job1[2], vals, dst16, acc);
   __syncthreads();
   const int64_t r0 = static_cast<int64_t>(T.bin) << kShift;
   const int rows = static_cast<int>(n - r0 < kSpan ? n - r0 : kSpan);
@@ -224,9 +245,9 @@ __global__ void __launch_bounds__(kBlock) accum_kernel(const Task* __restrict__ 
     fin.load();
     for (int i = threadIdx.x; i < rows; i += kBlock) fin(r0 + i, acc[i], resid, dang);
   } else {
-    unsigned long long* g = gacc + static_cast<size_t>(T.slot) * kSpan;
+    Acc* g = gacc + static_cast<size_t>(T.slot) * kSpan;
     for (int i = threadIdx.x; i < rows; i += kBlock)
-      if (acc[i]) atomicAdd(&g[i], acc[i]);
+      if (acc[i] != Acc(0)) atomicAdd(&g[i], acc[i]);
   }
   resid = rs::block_sum<kBlock>(resid, red);
   dang = rs::block_sum<kBlock>(dang, red);
@@ -237,23 +258,24 @@ __global__ void __launch_bounds__(kBlock) accum_kernel(const Task* __restrict__ 
 }
 
 // Finalises the sliced bins from their global accumulators and clears them.
-template <class Fin>
+template <class Real, class Fin>
 __global__ void __launch_bounds__(kBlock) multi_kernel(const Multi* __restrict__ multi,
-                                                       unsigned long long* __restrict__ gacc, int64_t n, Fin fin,
-                                                       double* __restrict__ part_resid,
+                                                       typename PbAcc<Real>::T* __restrict__ gacc, int64_t n,
+                                                       Fin fin, double* __restrict__ part_resid,
                                                        double* __restrict__ part_dang, const unsigned* stop) {
+  using Acc = typename PbAcc<Real>::T;
   if (*stop) return;
   __shared__ double red[kWaves];
   const Multi M = multi[blockIdx.x];
   const int64_t r0 = static_cast<int64_t>(M.bin) << kShift;
   const int rows = static_cast<int>(n - r0 < kSpan ? n - r0 : kSpan);
-  unsigned long long* g = gacc + static_cast<size_t>(M.slot) * kSpan;
+  Acc* g = gacc + static_cast<size_t>(M.slot) * kSpan;
   fin.load();
   double resid = 0, dang = 0;
   for (int i = threadIdx.x; i < rows; i += kBlock) {
-    const unsigned long long s = g[i];
-    g[i] = 0;
-    fin(r0 + i, s, resid, dang);
+    const Acc sv = g[i];
+    g[i] = Acc(0);
+    fin(r0 + i, sv, resid, dang);
   }
   resid = rs::block_sum<kBlock>(resid, red);
   dang = rs::block_sum<kBlock>(dang, red);

@@ -106,10 +106,12 @@ void gx_set_profiling(int on);
  *                          sources of the degree order are summed in a pass
  *                          of their own (their ranks stay L2-resident); 0 =
  *                          one pass (the default)
- *   "pr.blocking"          FP32 pull PageRank: 1 (default) = propagation-
- *                          blocked sweep (streamed per-edge contributions,
- *                          exact fixed-point row sums), 0 = merge-path pull
- *                          sweep with Kahan row sums (FP64 always uses it)
+ *   "pr.blocking"          pull PageRank: 1 (default) = propagation-
+ *                          blocked sweep for FP32 (streamed per-edge
+ *                          contributions, exact fixed-point row sums), 2 =
+ *                          also for FP64 (FP64 atomic row sums, order
+ *                          dependent in the last bit), 0 = merge-path pull
+ *                          sweep for both (Kahan row sums in FP32)
  * Unknown keys and out-of-range values are GX_ERR_INVALID. */
 gx_status gx_tune_set(const char* key, int64_t value);
 gx_status gx_tune_get(const char* key, int64_t* value);

@@ -142,7 +142,7 @@ def test_tuning_knobs():
     assert gx.tune_get("pr.blocking") == 0
     gx.tune_set("pr.blocking", -1)
     with pytest.raises(GxError):
-        gx.tune_set("pr.blocking", 2)  # 0 or 1 only
+        gx.tune_set("pr.blocking", 3)  # 0, 1 or 2 only
     with pytest.raises(GxError):
         gx.tune_set("no.such.knob", 1)
     with pytest.raises(GxError):

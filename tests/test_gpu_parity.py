@@ -294,6 +294,15 @@ def test_pagerank_blocked_vs_pull(gx, oracle, scale):
         err = np.abs(x.cpu().numpy().astype(np.float64) - exp)
         assert err.sum() < 1e-6 and it - 2 <= st.iterations < 100, (err.sum(), st.iterations, it)
     np.testing.assert_array_equal(xb.cpu().numpy(), xb2.cpu().numpy())
+    # FP64 through the blocked sweep (knob 2; FP64 atomic row sums): the
+    # oracle's iteration count and 1e-11 relative, as the FP64 pull.
+    gx.tune_set("pr.blocking", 2)
+    try:
+        x64, st64 = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp64")
+    finally:
+        gx.tune_set("pr.blocking", -1)
+    assert st64.iterations == it
+    np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-11, atol=1e-17)
     if scale == 20:
         # init (3) + unpermute (1) + per enqueued iteration (batches of 8):
         # gather, accumulate, sliced bins, reduce
@@ -337,6 +346,13 @@ def test_pagerank_blocked_shapes(gx, oracle):
     a = x1.cpu().numpy()
     assert np.abs(a.astype(np.float64) - exp).sum() < 1e-6
     np.testing.assert_array_equal(a, x2.cpu().numpy())
+    gx.tune_set("pr.blocking", 2)
+    try:
+        x64, st64 = gx.pagerank(G, tol=1e-10, max_iter=200, precision="fp64")
+    finally:
+        gx.tune_set("pr.blocking", -1)
+    assert st64.iterations == it
+    np.testing.assert_allclose(x64.cpu().numpy(), exp, rtol=1e-11, atol=1e-18)
     # tol 1e-10 is below FP32 resolution here: most x values stop changing and
     # FP32's own residual can reach tol before FP64's does; accuracy is the
     # check above.
