@@ -51,6 +51,7 @@ class Stats:
     elapsed_ms: float
     main_kernel_ms: float
     main_kernel_launches: int
+    main_kernel_bytes: int
 
 
 def _stats(st: gx_stats) -> Stats:

@@ -37,6 +37,7 @@ class gx_stats(C.Structure):
         ("main_kernel_ms", C.c_double),
         ("main_kernel_launches", C.c_uint32),
         ("reserved", C.c_uint32),
+        ("main_kernel_bytes", C.c_uint64),
     ]
 
     def as_dict(self) -> dict:

@@ -73,6 +73,15 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStr
                                  rocprim::plus<uint32_t>(), s));
 }
 
+void sort_keys_u64(const uint64_t* in, uint64_t* out, int64_t count, int end_bit, hipStream_t s) {
+  if (count <= 0) return;
+  size_t tmp = 0;
+  GX_HIP(rocprim::radix_sort_keys(nullptr, tmp, in, out, static_cast<size_t>(count), 0, end_bit, s));
+  DevBuf t;
+  t.ensure(tmp);
+  GX_HIP(rocprim::radix_sort_keys(t.p, tmp, in, out, static_cast<size_t>(count), 0, end_bit, s));
+}
+
 static int bits_for(int64_t n) {
   int b = 1;
   while ((int64_t(1) << b) < n) ++b;

@@ -123,9 +123,9 @@ struct PullPlan {
 // work tables of the two phases.
 struct PbLayout {
   bool ready = false;
-  int64_t nbins = 0, npieces = 0, ntasks = 0, nmulti = 0;
+  int64_t nbins = 0, ngroups = 0, npieces = 0, ntasks = 0, nmulti = 0;
   int64_t njobs[3] = {0, 0, 0};  // long, mid, short jobs
-  DevBuf src16, dst16, vals, pieces, jobs, tasks, multi, gacc;
+  DevBuf gsrc16, dst16, vals, pieces, jobs, tasks, multi, gacc;
 };
 
 struct DegreeOrder {
@@ -311,6 +311,8 @@ void build_degree_order(gx_graph* g);  // g->degree_order (relabelled transpose)
 void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipStream_t s);
 // Exclusive prefix sum of `count` uint32 values into uint32 `out` (may alias).
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s);
+// Ascending radix sort of count keys over bits [0, end_bit) (in is not changed).
+void sort_keys_u64(const uint64_t* in, uint64_t* out, int64_t count, int end_bit, hipStream_t s);
 // Device->host copy of a few bytes through the graph's pinned block.
 template <class T> T read_scalar(gx_graph* g, const T* dev) {
   T* h = static_cast<T*>(g->host_pinned(sizeof(T)));

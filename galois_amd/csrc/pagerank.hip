@@ -385,12 +385,13 @@ static bool use_blocking(const gx_graph* g, size_t real_bytes) {
 
 // Phase 1, phase 2 and the sliced-bin finalisation of one blocked sweep.
 template <class Real, class Fin>
-static uint32_t launch_blocked(const PbLayout& L, const Real* y, Real* vals, int64_t n, const Fin& fin,
+static uint32_t launch_blocked(const PbLayout& L, const Real* y, typename pb::PbAcc<Real>::V* vals, int64_t n,
+                               const Fin& fin,
                                double* part_resid, double* part_dang, const unsigned* stop, hipStream_t s) {
   using Acc = typename pb::PbAcc<Real>::T;
   const pb::Job* jobs = L.jobs.as<pb::Job>();
   pb::gather_kernel<Real><<<static_cast<unsigned>(L.npieces), pb::kBlock, 0, s>>>(
-      L.pieces.as<pb::Piece>(), L.src16.as<uint16_t>(), y, n, vals, stop);
+      L.pieces.as<pb::Piece>(), L.gsrc16.as<uint16_t>(), y, n, vals, stop);
   GX_LAUNCHED("pb::gather_kernel");
   pb::accum_kernel<Real, Fin><<<static_cast<unsigned>(L.ntasks), pb::kBlock, 0, s>>>(
       L.tasks.as<pb::Task>(), jobs, jobs + L.njobs[0], jobs + L.njobs[0] + L.njobs[1], vals, L.dst16.as<uint16_t>(),
@@ -415,7 +416,7 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
   rs::RowPlan P{}, PA{};
   if (blocked) {
     pb::build_layout(g);
-    D.pb.vals.ensure(g->m * sizeof(Real));  // grows once for FP64 contributions
+    D.pb.vals.ensure(D.pb.ngroups * sizeof(Real));  // grows once for FP64 contributions
   } else if (hot > 0) {
     build_split(g, hot);
     PA = rs::build_plan(g, D.rowA.as<int32_t>(), D.colA.as<int32_t>(), n, D.hot_edges, D.planA);
@@ -462,7 +463,7 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
         ktimer.begin();
         if constexpr (std::is_same<Real, float>::value) {
           PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, {}, {}};
-          launches += launch_blocked<float>(L, y[cur], L.vals.as<float>(), n, fin, part_resid, part_dang,
+          launches += launch_blocked<float>(L, y[cur], L.vals.as<uint32_t>(), n, fin, part_resid, part_dang,
                                             &state->converged, s);
         } else {
           PbFin64 fin{state, x[cur], x[nxt], y[nxt], outdeg, 0.0, 0.0};
@@ -517,6 +518,17 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
     st->elapsed_ms = ms;
     st->main_kernel_ms = ktimer.total_ms();
     st->main_kernel_launches = ktimer.count();
+    // Algorithmic bytes of one sweep.  Per row, both sweeps read y once (the
+    // blocked sweep stages it in LDS), x_old and the out-degree and write x'
+    // and y': 4R + 4.  Blocked: 2 bytes of destination id per edge; per group
+    // a 2-byte source id, the contribution written and read back; 12 bytes
+    // per phase-2 job.  Pull: 4 bytes of source id per edge and the row
+    // offsets (twice, and the partial sums, with the hot/cold split).
+    const int64_t R = sizeof(Real), m = g->m;
+    const int64_t njobs = L.njobs[0] + L.njobs[1] + L.njobs[2];
+    st->main_kernel_bytes = static_cast<uint64_t>(
+        blocked ? 2 * m + (2 + 2 * R) * L.ngroups + (4 * R + 4) * n + 12 * njobs
+                : 4 * m + (hot > 0 ? 2 : 1) * 4 * (n + 1) + (4 * R + 4) * n + (hot > 0 ? 2 * R * n : 0));
   }
 }
 

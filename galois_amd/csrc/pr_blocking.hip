@@ -1,19 +1,22 @@
 // Layout and work tables of the propagation-blocked PageRank sweep
 // (pr_blocking.hpp), built once per graph from the CSR and the degree order.
 //
-//  1. count_kernel: edges per (chunk of the source, bin of the destination)
-//     key in degree-order ids, one atomic per run of equal keys in a
-//     wavefront.  The edges are read in out-edge (CSR) order, so that the
-//     edges of one source stay together and neighbouring entries of a run
-//     have distinct destinations: phase 2 then adds a wavefront's 64 entries
-//     into mostly different LDS words (destination order would put up to 64
-//     adds on one word) and phase 1 reads one source's y as a broadcast.
-//  2. exclusive scan of the counts: start of every run, chunk-major.
-//  3. scatter_kernel: every run of equal keys in a wavefront reserves its
-//     slots with one atomic and writes src16/dst16.  The order of the edges
-//     inside a run depends on the atomics; the sweep's integer sums do not.
-//  4. host: phase-1 pieces per chunk, phase-2 jobs per bin in three length
-//     classes, tasks (bins, heavy bins sliced), sorted longest first.
+//  1. key_kernel: one 64-bit key per edge, (run, source in chunk, destination
+//     in bin) in degree-order ids, run = chunk of the source * bins + bin of
+//     the destination; a radix sort over the used bits puts the edges in
+//     layout order (deterministic: the graph has no duplicate edges).
+//  2. mark_kernel: dst16 with the group flag, the flag array for the group
+//     numbering, and the edge and group counts per run (one atomic per run of
+//     equal keys in a wavefront).
+//  3. exclusive scans: group number of every edge's group start, start of
+//     every run in edges and in groups; group_kernel writes gsrc16.
+//  4. host: phase-1 pieces per chunk (in groups), phase-2 jobs per bin in
+//     three length classes, tasks (bins, heavy bins sliced), sorted longest
+//     first; job_base_kernel then fills every job's group base.
+//
+// Edges within a run are in (source, destination) order: neighbouring edges
+// of a group have distinct, ascending destinations, so a wavefront's adds
+// mostly hit different LDS words.
 #include <algorithm>
 #include <vector>
 
@@ -39,31 +42,22 @@ __device__ __forceinline__ uint64_t run_heads(bool valid, uint32_t key, unsigned
   return heads;
 }
 
-__global__ void __launch_bounds__(kBuildBlock) count_kernel(const uint32_t* __restrict__ esrc,
-                                                            const int32_t* __restrict__ col,
-                                                            const uint32_t* __restrict__ rank, int64_t m,
-                                                            uint32_t nb, uint32_t* __restrict__ cnt) {
-  const unsigned lane = lane_id();
+__global__ void __launch_bounds__(kBuildBlock) key_kernel(const uint32_t* __restrict__ esrc,
+                                                          const int32_t* __restrict__ col,
+                                                          const uint32_t* __restrict__ rank, int64_t m, uint32_t nb,
+                                                          uint64_t* __restrict__ keys) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kBuildBlock;
-  for (int64_t e0 = blockIdx.x * static_cast<int64_t>(kBuildBlock) + (threadIdx.x & ~(kWave - 1)); e0 < m;
-       e0 += stride) {
-    const int64_t e = e0 + lane;
-    const bool valid = e < m;
-    const unsigned nvalid = static_cast<unsigned>(m - e0 < kWave ? m - e0 : kWave);
-    const uint32_t key =
-        valid ? (rank[esrc[e]] >> kShift) * nb + (rank[col[e]] >> kShift) : 0xffffffffu;
-    unsigned len;
-    const uint64_t heads = run_heads(valid, key, lane, nvalid, len);
-    if ((heads >> lane) & 1ull) atomicAdd(&cnt[key], len);
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(kBuildBlock) + threadIdx.x; e < m; e += stride) {
+    const uint32_t u = rank[esrc[e]], v = rank[col[e]];
+    const uint64_t run = static_cast<uint64_t>(u >> kShift) * nb + (v >> kShift);
+    keys[e] = (run << (2 * kShift)) | (static_cast<uint64_t>(u & (kSpan - 1)) << kShift) | (v & (kSpan - 1));
   }
 }
 
-__global__ void __launch_bounds__(kBuildBlock) scatter_kernel(const uint32_t* __restrict__ esrc,
-                                                              const int32_t* __restrict__ col,
-                                                              const uint32_t* __restrict__ rank, int64_t m,
-                                                              uint32_t nb, uint32_t* __restrict__ cursor,
-                                                              uint16_t* __restrict__ src16,
-                                                              uint16_t* __restrict__ dst16) {
+// Sorted keys -> dst16 (+ group flag), flag[e] (0/1), edges and groups per run.
+__global__ void __launch_bounds__(kBuildBlock) mark_kernel(const uint64_t* __restrict__ keys, int64_t m,
+                                                           uint16_t* __restrict__ dst16, uint32_t* __restrict__ flag,
+                                                           uint32_t* __restrict__ cnt, uint32_t* __restrict__ gcnt) {
   const unsigned lane = lane_id();
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kBuildBlock;
   for (int64_t e0 = blockIdx.x * static_cast<int64_t>(kBuildBlock) + (threadIdx.x & ~(kWave - 1)); e0 < m;
@@ -71,23 +65,40 @@ __global__ void __launch_bounds__(kBuildBlock) scatter_kernel(const uint32_t* __
     const int64_t e = e0 + lane;
     const bool valid = e < m;
     const unsigned nvalid = static_cast<unsigned>(m - e0 < kWave ? m - e0 : kWave);
-    const uint32_t u = valid ? rank[esrc[e]] : 0u;
-    const uint32_t v = valid ? rank[col[e]] : 0u;
-    const uint32_t key = valid ? (u >> kShift) * nb + (v >> kShift) : 0xffffffffu;
-    unsigned len;
-    const uint64_t heads = run_heads(valid, key, lane, nvalid, len);
-    uint32_t base = 0;
-    if ((heads >> lane) & 1ull) base = atomicAdd(&cursor[key], len);
-    // Head of the caller's run: highest head at or below the lane.
-    const uint64_t le = lane == 63 ? ~0ull : ((2ull << lane) - 1);
-    const unsigned h = 63u - static_cast<unsigned>(__builtin_clzll((heads & le) | 1ull));
-    base = __shfl(base, static_cast<int>(h));
+    const uint64_t k = valid ? keys[e] : 0ull;
+    const bool first = valid && (e == 0 || (keys[e - 1] >> kShift) != (k >> kShift));
+    const uint32_t run = valid ? static_cast<uint32_t>(k >> (2 * kShift)) : 0xffffffffu;
     if (valid) {
-      const uint32_t pos = base + (lane - h);
-      src16[pos] = static_cast<uint16_t>(u & (kSpan - 1));
-      dst16[pos] = static_cast<uint16_t>(v & (kSpan - 1));
+      dst16[e] = static_cast<uint16_t>((k & (kSpan - 1)) | (first ? kFirst : 0u));
+      flag[e] = first ? 1u : 0u;
+    }
+    unsigned len;
+    const uint64_t heads = run_heads(valid, run, lane, nvalid, len);
+    const uint64_t f = __ballot(first);
+    if ((heads >> lane) & 1ull) {
+      const uint64_t span = (len >= 64 ? ~0ull : ((uint64_t(1) << len) - 1)) << lane;
+      atomicAdd(&cnt[run], len);
+      atomicAdd(&gcnt[run], static_cast<uint32_t>(__popcll(f & span)));
     }
   }
+}
+
+// gsrc16[group] = source in chunk, for every edge that opens a group.
+__global__ void __launch_bounds__(kBuildBlock) group_kernel(const uint64_t* __restrict__ keys,
+                                                            const uint32_t* __restrict__ flag,
+                                                            const uint32_t* __restrict__ gpos, int64_t m,
+                                                            uint16_t* __restrict__ gsrc16) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBuildBlock;
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(kBuildBlock) + threadIdx.x; e < m; e += stride)
+    if (flag[e]) gsrc16[gpos[e]] = static_cast<uint16_t>((keys[e] >> kShift) & (kSpan - 1));
+}
+
+// gbase of every job: groups opened before its first edge, minus one.
+__global__ void __launch_bounds__(kBuildBlock) job_base_kernel(Job* __restrict__ jobs, int64_t njobs,
+                                                               const uint32_t* __restrict__ gpos) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBuildBlock;
+  for (int64_t j = blockIdx.x * static_cast<int64_t>(kBuildBlock) + threadIdx.x; j < njobs; j += stride)
+    jobs[j].gbase = gpos[jobs[j].start] - 1u;
 }
 
 }  // namespace
@@ -103,36 +114,65 @@ void build_layout(gx_graph* g) {
   const int64_t nb = (n + kSpan - 1) / kSpan;
   const int64_t nkeys = nb * nb;
 
-  std::vector<uint32_t> start(nkeys + 1);
+  std::vector<uint32_t> start(nkeys + 1), gstart(nkeys + 1);
+  DevBuf gpos;  // exclusive scan of the group flags, m + 1 entries
+  int64_t ngroups = 0;
   {
-    DevBuf esrc, cnt, cursor;
-    esrc.ensure(m * 4);
-    edge_sources(g->row, n, m, esrc.as<uint32_t>(), s);
-    const uint32_t* rank = D.new_of_old.as<uint32_t>();
+    DevBuf keys, sorted;
+    keys.ensure(m * 8);
+    {
+      DevBuf esrc;
+      esrc.ensure(m * 4);
+      edge_sources(g->row, n, m, esrc.as<uint32_t>(), s);
+      const uint32_t* rank = D.new_of_old.as<uint32_t>();
+      key_kernel<<<grid_for(m, kBuildBlock, 1 << 16), kBuildBlock, 0, s>>>(
+          esrc.as<uint32_t>(), g->col, rank, m, static_cast<uint32_t>(nb), keys.as<uint64_t>());
+      GX_LAUNCHED("pb key_kernel");
+    }
+    int run_bits = 0;
+    while ((int64_t(1) << run_bits) < nkeys) ++run_bits;
+    sorted.ensure(m * 8);
+    sort_keys_u64(keys.as<uint64_t>(), sorted.as<uint64_t>(), m, 2 * kShift + run_bits, s);
+    keys.release();
+    DevBuf flag, cnt, gcnt, cur, gcur;
+    flag.ensure((m + 1) * 4);
     cnt.ensure((nkeys + 1) * 4);
-    cursor.ensure((nkeys + 1) * 4);
-    GX_HIP(hipMemsetAsync(cnt.p, 0, (nkeys + 1) * 4, s));
-    const unsigned grid = grid_for(m, kBuildBlock, 1 << 16);
-    count_kernel<<<grid, kBuildBlock, 0, s>>>(esrc.as<uint32_t>(), g->col, rank, m, static_cast<uint32_t>(nb),
-                                               cnt.as<uint32_t>());
-    GX_LAUNCHED("pb count_kernel");
-    exclusive_scan_u32(cnt.as<uint32_t>(), cursor.as<uint32_t>(), nkeys + 1, s);
-    GX_HIP(hipStreamSynchronize(s));
-    GX_HIP(hipMemcpy(start.data(), cursor.p, (nkeys + 1) * 4, hipMemcpyDeviceToHost));
-    GX_REQUIRE(start[nkeys] == static_cast<uint32_t>(m), "propagation-blocking layout: edge count mismatch");
-    L.src16.ensure(m * 2);
+    gcnt.ensure((nkeys + 1) * 4);
+    cur.ensure((nkeys + 1) * 4);
+    gcur.ensure((nkeys + 1) * 4);
+    gpos.ensure((m + 1) * 4);
     L.dst16.ensure(m * 2);
-    scatter_kernel<<<grid, kBuildBlock, 0, s>>>(esrc.as<uint32_t>(), g->col, rank, m, static_cast<uint32_t>(nb),
-                                                 cursor.as<uint32_t>(),
-                                                 L.src16.as<uint16_t>(), L.dst16.as<uint16_t>());
-    GX_LAUNCHED("pb scatter_kernel");
+    GX_HIP(hipMemsetAsync(cnt.p, 0, (nkeys + 1) * 4, s));
+    GX_HIP(hipMemsetAsync(gcnt.p, 0, (nkeys + 1) * 4, s));
+    GX_HIP(hipMemsetAsync(flag.as<uint32_t>() + m, 0, 4, s));
+    mark_kernel<<<grid_for(m, kBuildBlock, 1 << 16), kBuildBlock, 0, s>>>(
+        sorted.as<uint64_t>(), m, L.dst16.as<uint16_t>(), flag.as<uint32_t>(), cnt.as<uint32_t>(),
+        gcnt.as<uint32_t>());
+    GX_LAUNCHED("pb mark_kernel");
+    exclusive_scan_u32(cnt.as<uint32_t>(), cur.as<uint32_t>(), nkeys + 1, s);
+    exclusive_scan_u32(gcnt.as<uint32_t>(), gcur.as<uint32_t>(), nkeys + 1, s);
+    exclusive_scan_u32(flag.as<uint32_t>(), gpos.as<uint32_t>(), m + 1, s);
+    GX_HIP(hipStreamSynchronize(s));
+    GX_HIP(hipMemcpy(start.data(), cur.p, (nkeys + 1) * 4, hipMemcpyDeviceToHost));
+    GX_HIP(hipMemcpy(gstart.data(), gcur.p, (nkeys + 1) * 4, hipMemcpyDeviceToHost));
+    uint32_t total = 0;
+    GX_HIP(hipMemcpy(&total, gpos.as<uint32_t>() + m, 4, hipMemcpyDeviceToHost));
+    ngroups = total;
+    GX_REQUIRE(start[nkeys] == static_cast<uint32_t>(m), "propagation-blocking layout: edge count mismatch");
+    GX_REQUIRE(gstart[nkeys] == total && ngroups >= 1 && ngroups <= m,
+               "propagation-blocking layout: group count mismatch");
+    L.gsrc16.ensure(ngroups * 2);
+    group_kernel<<<grid_for(m, kBuildBlock, 1 << 16), kBuildBlock, 0, s>>>(
+        sorted.as<uint64_t>(), flag.as<uint32_t>(), gpos.as<uint32_t>(), m, L.gsrc16.as<uint16_t>());
+    GX_LAUNCHED("pb group_kernel");
     GX_HIP(hipStreamSynchronize(s));
   }
 
-  // Phase-1 pieces: every nonempty chunk in slices of kPieceEntries.
+  // Phase-1 pieces: the groups of every nonempty chunk in slices of
+  // kPieceEntries.
   std::vector<Piece> pieces;
   for (int64_t c = 0; c < nb; ++c) {
-    const uint32_t a = start[c * nb], b = start[(c + 1) * nb];
+    const uint32_t a = gstart[c * nb], b = gstart[(c + 1) * nb];
     for (uint64_t p = a; p < b; p += kPieceEntries)
       pieces.push_back(Piece{static_cast<uint32_t>(c), static_cast<uint32_t>(p),
                              static_cast<uint32_t>(std::min<uint64_t>(b, p + kPieceEntries)), 0u});
@@ -149,7 +189,7 @@ void build_layout(gx_graph* g) {
       while (len > 0) {
         const uint32_t l = std::min<uint32_t>(len, kJob);
         const int cls = l > kMid ? 0 : (l > kShort ? 1 : 2);
-        per[cls][b].push_back(Job{p, l});
+        per[cls][b].push_back(Job{p, l, 0u});
         p += l;
         len -= l;
       }
@@ -237,11 +277,11 @@ void build_layout(gx_graph* g) {
     for (const Multi& mu : multi) fin[mu.bin] += 1;
     for (int64_t b = 0; b < nb; ++b) GX_REQUIRE(fin[b] == 1, "propagation-blocking layout: bin not finalised once");
     for (const Piece& pc : pieces) {
-      GX_REQUIRE(pc.p0 < pc.p1 && pc.p1 <= static_cast<uint64_t>(m) && pc.chunk < nb,
+      GX_REQUIRE(pc.p0 < pc.p1 && pc.p1 <= static_cast<uint64_t>(ngroups) && pc.chunk < nb,
                  "propagation-blocking layout: bad piece");
       pieced += pc.p1 - pc.p0;
     }
-    GX_REQUIRE(covered == static_cast<uint64_t>(m) && pieced == static_cast<uint64_t>(m),
+    GX_REQUIRE(covered == static_cast<uint64_t>(m) && pieced == static_cast<uint64_t>(ngroups),
                "propagation-blocking layout: edges not covered exactly once");
   }
 
@@ -265,10 +305,16 @@ void build_layout(gx_graph* g) {
     L.njobs[cls] = static_cast<int64_t>(jobs[cls].size());
     off += jobs[cls].size();
   }
+  if (nj > 0) {
+    job_base_kernel<<<grid_for(static_cast<int64_t>(nj), kBuildBlock, 1 << 14), kBuildBlock, 0, s>>>(
+        L.jobs.as<Job>(), static_cast<int64_t>(nj), gpos.as<uint32_t>());
+    GX_LAUNCHED("pb job_base_kernel");
+  }
   L.gacc.ensure(std::max<size_t>(multi.size(), 1) * kSpan * sizeof(unsigned long long));
   GX_HIP(hipMemsetAsync(L.gacc.p, 0, std::max<size_t>(multi.size(), 1) * kSpan * sizeof(unsigned long long), s));
-  L.vals.ensure(m * sizeof(float));
+  L.vals.ensure(ngroups * sizeof(float));
   GX_HIP(hipStreamSynchronize(s));
+  L.ngroups = ngroups;
   L.nbins = nb;
   L.npieces = static_cast<int64_t>(pieces.size());
   L.ntasks = static_cast<int64_t>(tv.size());

@@ -8,16 +8,27 @@
 // replaces the random reads with two streaming phases over a static edge
 // layout built once per graph:
 //
-//   layout  edges grouped into runs by (source chunk, destination bin),
-//           chunk-major, kSpan vertices per chunk and per bin; per edge the
-//           in-chunk source (src16) and in-bin destination (dst16).
+//   layout  edges sorted by (source chunk, destination bin, source,
+//           destination), kSpan vertices per chunk and per bin.  A run is
+//           the edges of one (chunk, bin) pair; a group is the edges of one
+//           source inside a run, which all carry the same contribution.  Per
+//           edge: the in-bin destination (dst16, bit 15 set on the first
+//           edge of a group); per group: the in-chunk source (gsrc16) and
+//           the contribution slot vals[group].
 //   phase 1 (gather_kernel) one workgroup per piece of a chunk: the chunk's
-//           kSpan y values are staged in LDS and vals[p] = y[src16[p]] is
-//           streamed out in layout order (2-byte read, 4-byte write per edge).
+//           kSpan y values are staged in LDS (FP32: encoded as fixed-point
+//           words, see encode_fixed) and vals[q] = y[gsrc16[q]] is streamed
+//           out in layout order (2-byte read, 4-byte write per group).
 //   phase 2 (accum_kernel) one workgroup per task (a bin, or a slice of a
-//           heavy bin): the bin's runs are read from every chunk and
-//           vals[p] is added into an LDS accumulator per destination
-//           (4 + 2 bytes read per edge), then the bin's rows are finalised.
+//           heavy bin): the bin's runs are read from every chunk, the group
+//           of each edge is counted from the group flags with wavefront
+//           ballots, and vals[group] is added into an LDS accumulator per
+//           destination (2 bytes per edge and 4 per group read), then the
+//           bin's rows are finalised.
+//
+// On RMAT graphs a hub source has many edges in one bin, so there are far
+// fewer groups than edges (about 0.35 groups per edge at RMAT-26): per edge
+// the sweep moves 2 + 10 * groups / edges bytes instead of 12.
 //
 // FP32 row sums are exact: every contribution (0 <= y <= 1 for the
 // normalised iteration) is rounded to a 64-bit fixed-point number with 2^-62
@@ -29,10 +40,10 @@
 // their LDS sums into a per-bin global accumulator that multi_kernel
 // finalises (and clears for the next iteration).
 //
-// Phase-2 work is described by jobs (start, length) of at most kJob entries
-// in three size classes, so that runs of every length keep the lanes busy:
-// long jobs (65..512 entries) take a wavefront each, mid jobs (9..64) eight
-// lanes each, short jobs (1..8) one lane each.
+// Phase-2 work is described by jobs (start, length, group base) of at most
+// kJob edges of one run, in three size classes, so that runs of every length
+// keep the lanes busy: long jobs (65..512 edges) take a wavefront each, mid
+// jobs (9..64) eight lanes each, short jobs (1..8) one lane each.
 #pragma once
 
 #include <cstdint>
@@ -55,12 +66,16 @@ constexpr int64_t kTaskEntries = int64_t(1) << 19;   // bins above this are slic
 constexpr int64_t kPieceEntries = int64_t(1) << 18;  // phase-1 piece length
 constexpr int64_t kMaxKeys = int64_t(1) << 28;       // chunk x bin table limit
 constexpr uint32_t kSingle = 0xffffffffu;            // task finalises its bin itself
+constexpr uint32_t kFirst = 1u << 15;                // dst16 flag: first edge of a group
 
 struct Piece {
   uint32_t chunk, p0, p1, pad;
 };
+// Edges [start, start + len) of one run.  The group of edge start + i is
+// gbase + (number of group flags among edges start .. start + i), modulo
+// 2^32: gbase is the number of groups that begin before `start`, minus one.
 struct Job {
-  uint32_t start, len;
+  uint32_t start, len, gbase;
 };
 struct Task {
   uint32_t bin, slot;  // slot: global accumulator of a sliced bin, or kSingle
@@ -76,11 +91,11 @@ inline bool supported(int64_t n, int64_t m) {
   return n > 0 && m > 0 && m < (int64_t(1) << 31) && nb * nb <= kMaxKeys;
 }
 
-// y value -> fixed point with 2^-62 resolution, rounded to nearest.  Valid
-// for 0 <= v < 2; the normalised iteration keeps every x (and so every y and
-// every row sum) within [0, 1].
-__device__ __forceinline__ unsigned long long to_fixed(float v) {
-  const uint32_t b = __float_as_uint(v);
+// y value -> fixed point with 2^-62 resolution, rounded to nearest (ties
+// up).  Valid for 0 <= v < 2; the normalised iteration keeps every x (and so
+// every y and every row sum) within [0, 1].
+__host__ __device__ __forceinline__ unsigned long long to_fixed(float v) {
+  const uint32_t b = __builtin_bit_cast(uint32_t, v);
   const int e = static_cast<int>(b >> 23);  // biased exponent (v >= 0)
   if (e == 0) return 0;                     // zero / subnormal: below 2^-63
   const unsigned long long mant = (b & 0x7fffffu) | 0x800000u;
@@ -91,6 +106,26 @@ __device__ __forceinline__ unsigned long long to_fixed(float v) {
   return (mant + (1ull << (r - 1))) >> r;
 }
 
+// to_fixed(v) in 32 bits, as m << s with m < 2^24 in bits 0..23 and s <= 39
+// in bits 24..29: the mantissa with its shift when v >= 2^-39 (exact), else
+// the rounded value itself (s = 0).  Phase 1 encodes once per group, so that
+// phase 2 decodes every edge's contribution with one 64-bit shift instead of
+// the branches of to_fixed.
+__host__ __device__ __forceinline__ uint32_t encode_fixed(float v) {
+  const uint32_t b = __builtin_bit_cast(uint32_t, v);
+  const int e = static_cast<int>(b >> 23);
+  if (e == 0) return 0;
+  const uint32_t mant = (b & 0x7fffffu) | 0x800000u;
+  const int sh = e - 88;
+  if (sh >= 0) return mant | (static_cast<uint32_t>(sh) << 24);
+  if (sh <= -25) return 0;
+  const int r = -sh;
+  return (mant + (1u << (r - 1))) >> r;  // < 2^24
+}
+__host__ __device__ __forceinline__ unsigned long long decode_fixed(uint32_t w) {
+  return static_cast<unsigned long long>(w & 0xffffffu) << (w >> 24);
+}
+
 // Exact fixed-point sum -> unevaluated FP32 pair hi + lo (about 48 bits).
 __device__ __forceinline__ void fixed_to_float2(unsigned long long s, float& hi, float& lo) {
   const float h = __ull2float_rn(s);
@@ -99,18 +134,43 @@ __device__ __forceinline__ void fixed_to_float2(unsigned long long s, float& hi,
   lo = __ll2float_rn(r) * 0x1p-62f;
 }
 
-// Phase 1: vals[p] = y[chunk base + src16[p]] over one piece.
+// Per precision: the accumulator type T, the type V of the per-group
+// contributions phase 1 writes, and the conversions.  FP32 contributions are
+// written as encode_fixed words and summed exactly as 64-bit fixed point
+// (integer atomics, order independent); FP64 contributions are written and
+// summed as FP64 (atomics, order dependent in the last bit).
+template <class Real>
+struct PbAcc;
+template <>
+struct PbAcc<float> {
+  using T = unsigned long long;
+  using V = uint32_t;
+  static __device__ __forceinline__ V encode(float v) { return encode_fixed(v); }
+  static __device__ __forceinline__ T conv(V w) { return decode_fixed(w); }
+};
+template <>
+struct PbAcc<double> {
+  using T = double;
+  using V = double;
+  static __device__ __forceinline__ V encode(double v) { return v; }
+  static __device__ __forceinline__ T conv(V v) { return v; }
+};
+
+// Phase 1: vals[q] = encode(y[chunk base + gsrc16[q]]) over one piece of
+// groups (the chunk's y is encoded once, as it is staged).
 template <class Real>
 __global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict__ pieces,
                                                         const uint16_t* __restrict__ src16,
                                                         const Real* __restrict__ y, int64_t n,
-                                                        Real* __restrict__ vals, const unsigned* stop) {
+                                                        typename PbAcc<Real>::V* __restrict__ vals,
+                                                        const unsigned* stop) {
+  using V = typename PbAcc<Real>::V;
   if (*stop) return;
-  __shared__ Real ys[kSpan];
+  __shared__ V ys[kSpan];
   const Piece P = pieces[blockIdx.x];
   const int64_t base = static_cast<int64_t>(P.chunk) << kShift;
   const int cnt = static_cast<int>(n - base < kSpan ? n - base : kSpan);
-  for (int i = threadIdx.x; i < cnt; i += kBlock) ys[i] = rs::ld_stream(&y[base + i]);
+  for (int i = threadIdx.x; i < cnt; i += kBlock) ys[i] = PbAcc<Real>::encode(rs::ld_stream(&y[base + i]));
   __syncthreads();
   // Body in 4-entry groups: one 8-byte index load and one 16-byte store per
   // lane, contiguous across the wavefront; the unaligned head and tail of the
@@ -131,7 +191,7 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict_
     vals[q] = ys[src16[q]];
   }
   typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-  typedef Real r4 __attribute__((ext_vector_type(4)));
+  typedef V r4 __attribute__((ext_vector_type(4)));
   const u32x2* s4 = reinterpret_cast<const u32x2*>(src16 + a4);
   r4* v4 = reinterpret_cast<r4*>(vals + a4);
   const int64_t groups = (b4 - a4) >> 2;
@@ -158,43 +218,32 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict_
   }
 }
 
-// Row-sum accumulator per precision: FP32 contributions are summed exactly
-// as 64-bit fixed point (integer atomics, order independent); FP64
-// contributions are summed with FP64 atomics (order dependent in the last
-// bit).
-template <class Real>
-struct PbAcc;
-template <>
-struct PbAcc<float> {
-  using T = unsigned long long;
-  static __device__ __forceinline__ T conv(float v) { return to_fixed(v); }
-};
-template <>
-struct PbAcc<double> {
-  using T = double;
-  static __device__ __forceinline__ T conv(double v) { return v; }
-};
-
-// Adds the entries of one job class into the LDS sums.  Lanes are grouped
-// LPJ to a job; lane `sub` of a group takes entries sub + LPJ*k, k < 8.  A
-// wavefront loads the entries of kU job groups before it adds any of them,
-// so that kU x 8 loads per lane are in flight.
+// Adds the edges of one job class into the LDS sums.  Lanes are grouped
+// LPJ to a job; lane `sub` of a group takes edges sub + LPJ*k, k < 8, so for
+// each k the LPJ lanes of a job hold LPJ consecutive edges and a ballot of
+// the group flags, masked to those lanes, counts the groups they open.  A
+// wavefront loads the destinations of kU job groups, derives every edge's
+// group, loads the contributions, and only then adds, so that kU x 8 loads
+// per lane are in flight in each round.
 template <int LPJ, class Real>
 __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_t j0, uint32_t j1,
-                                           const Real* __restrict__ vals, const uint16_t* __restrict__ dst16,
-                                           typename PbAcc<Real>::T* acc) {
+                                           const typename PbAcc<Real>::V* __restrict__ vals,
+                                           const uint16_t* __restrict__ dst16, typename PbAcc<Real>::T* acc) {
+  using V = typename PbAcc<Real>::V;
   constexpr int kGroups = kWave / LPJ;
   constexpr int kU = sizeof(Real) == 4 ? 4 : 2;
   const unsigned lane = lane_id(), wave = threadIdx.x / kWave;
   const unsigned grp = lane / LPJ, sub = lane % LPJ;
+  const uint64_t mine = LPJ == kWave ? ~0ull : ((uint64_t(1) << LPJ) - 1) << (grp * LPJ);
+  const uint64_t upto = mine & ((uint64_t(2) << lane) - 1);  // my job's lanes up to mine (lane 63: all)
   for (uint32_t jb = j0 + wave * kGroups * kU; jb < j1; jb += kWaves * kGroups * kU) {
     Job J[kU];
-    Real v[kU][8];
+    V v[kU][8];
     uint32_t d[kU][8];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const uint32_t j = jb + u * kGroups + grp;
-      J[u] = Job{0, 0};
+      J[u] = Job{0, 0, 0};
       if (j < j1) J[u] = jobs[j];
     }
 #pragma unroll
@@ -202,18 +251,32 @@ __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
         const uint32_t i = sub + LPJ * k;
-        v[u][k] = Real(0);
-        d[u][k] = 0;
-        if (i < J[u].len) {
-          v[u][k] = rs::ld_stream(&vals[J[u].start + i]);
-          d[u][k] = dst16[J[u].start + i];
-        }
+        d[u][k] = i < J[u].len ? rs::ld_stream(&dst16[J[u].start + i]) : 0u;
       }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      uint32_t carry = J[u].gbase;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        const uint32_t own = (d[u][k] & kFirst) >> 15;
+        const uint64_t f = __ballot(own != 0);
+        uint32_t q;
+        if constexpr (LPJ == kWave) {  // flags of the lanes below, plus my own
+          q = __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(f >> 32),
+                                        __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(f), carry + own));
+        } else {
+          q = carry + static_cast<uint32_t>(__popcll(f & upto));
+        }
+        carry += static_cast<uint32_t>(__popcll(f & mine));
+        v[u][k] = V(0);
+        if (sub + LPJ * k < J[u].len) v[u][k] = vals[q];
+      }
+    }
 #pragma unroll
     for (int u = 0; u < kU; ++u)
 #pragma unroll
       for (int k = 0; k < 8; ++k)
-        if (sub + LPJ * k < J[u].len) atomicAdd(&acc[d[u][k]], PbAcc<Real>::conv(v[u][k]));
+        if (sub + LPJ * k < J[u].len) atomicAdd(&acc[d[u][k] & (kSpan - 1)], PbAcc<Real>::conv(v[u][k]));
   }
 }
 
@@ -222,7 +285,7 @@ __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_
 template <class Real, class Fin>
 __global__ void __launch_bounds__(kBlock) accum_kernel(const Task* __restrict__ tasks, const Job* __restrict__ jl,
                                                        const Job* __restrict__ jm, const Job* __restrict__ js,
-                                                       const Real* __restrict__ vals,
+                                                       const typename PbAcc<Real>::V* __restrict__ vals,
                                                        const uint16_t* __restrict__ dst16,
                                                        typename PbAcc<Real>::T* __restrict__ gacc, int64_t n, Fin fin,
                                                        double* __restrict__ part_resid,

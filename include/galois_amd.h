@@ -71,6 +71,9 @@ typedef struct gx_stats {
                                gx_set_profiling(1) is on (else 0)             */
   uint32_t main_kernel_launches; /* launches of the dominant kernel          */
   uint32_t reserved;
+  uint64_t main_kernel_bytes; /* algorithmic HBM bytes per launch of the
+                                 dominant kernel (PageRank pull: per sweep),
+                                 0 where an entry point does not state it     */
 } gx_stats;
 
 /* Host-side CSR as read from / written to a .gr file. */

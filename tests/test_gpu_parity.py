@@ -294,6 +294,11 @@ def test_pagerank_blocked_vs_pull(gx, oracle, scale):
         err = np.abs(x.cpu().numpy().astype(np.float64) - exp)
         assert err.sum() < 1e-6 and it - 2 <= st.iterations < 100, (err.sum(), st.iterations, it)
     np.testing.assert_array_equal(xb.cpu().numpy(), xb2.cpu().numpy())
+    # stated bytes per sweep: blocked 2m + 10g + 20n + 12/job with g groups
+    # (0 < g <= m); pull 4m + 4(n+1) + 20n, more with the hot/cold split
+    m, n = int(row[-1]), len(row) - 1
+    assert 2 * m + 20 * n < stb.main_kernel_bytes, stb.main_kernel_bytes
+    assert stp.main_kernel_bytes >= 4 * m + 24 * n + 4, stp.main_kernel_bytes
     # FP64 through the blocked sweep (knob 2; FP64 atomic row sums): the
     # oracle's iteration count and 1e-11 relative, as the FP64 pull.
     gx.tune_set("pr.blocking", 2)
