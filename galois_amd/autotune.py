@@ -29,7 +29,7 @@ from pathlib import Path
 import numpy as np
 
 CANDIDATES: dict[str, list[int]] = {
-    "frontier.hub_degree": [256, 512, 1024, 2048, 8192, 65536],
+    "frontier.hub_degree": [64, 128, 256, 512, 1024, 4096],
     "pr.blocking": [1, 0],
     "pr.hot_sources": [0, 1 << 17, 1 << 18, 1 << 19, 1 << 20, 1 << 21],
 }

@@ -535,7 +535,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
   double* delta = g->scratch<double>(WS_BC_DELTA, cap);
   uint32_t* order = g->scratch<uint32_t>(WS_Q0, cap);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, cap);
-  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(cap));
   PlainCounters* c = g->scratch<PlainCounters>(WS_COUNTERS, 1);
   PlainCounters* host = static_cast<PlainCounters*>(g->host_pinned(sizeof(PlainCounters)));
   RegionTimer region(s);
@@ -547,7 +547,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
   GX_LAUNCHED("bc_plain_init_kernel");
   launches += 2;
   const unsigned grid = 256 * 8;
-  const uint32_t hub = hub_degree();
+  const uint32_t hub = hub_degree(g->m);
   // In-edges for the path-count pull (the CSR itself for symmetric graphs).
   const int32_t* irow = g->row;
   const int32_t* icol = g->col;
@@ -574,7 +574,9 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
       FwdOp op{g->col, level, order + e, c, nullptr, L + 1, cap - e};
       bc_fwd_expand_kernel<<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(g->row, op,
                                                                                                order + b, cnt, hubq, hub);
-      hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+      // The forward hub list is bounded by the level list's free room
+      // (op.cap), as expand_hubs reads it.
+      launch_hub_offsets<kBlock>(g->row, hubq, &c->hub_count, op.cap, hub_off, s);
       bc_fwd_hubs_kernel<<<grid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
       GX_LAUNCHED("bc forward");
       launches += 3;
@@ -589,7 +591,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
         SigmaOp sop{icol, level, sigma, L - 1};
         bc_expand_kernel<SigmaOp><<<std::min<unsigned>(grid, grid_for(ncnt, kBlock)), kBlock, 0, s>>>(
             irow, sop, order + e, ncnt, hubq, c, cap, hub);
-        hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(irow, hubq, &c->hub_count, cap, hub_off);
+        launch_hub_offsets<kBlock>(irow, hubq, &c->hub_count, cap, hub_off, s);
         bc_hubs_kernel<SigmaOp><<<grid, kBlock, 0, s>>>(irow, sop, hubq, c, cap, hub_off);
         GX_LAUNCHED("bc sigma");
         launches += 3;
@@ -604,7 +606,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
       BwdOp op{g->col, level, sigma, delta, static_cast<uint32_t>(lv + 1)};
       bc_expand_kernel<BwdOp><<<std::min<unsigned>(grid, grid_for(cnt, kBlock)), kBlock, 0, s>>>(
           g->row, op, order + b, cnt, hubq, c, cap, hub);
-      hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+      launch_hub_offsets<kBlock>(g->row, hubq, &c->hub_count, cap, hub_off, s);
       bc_hubs_kernel<BwdOp><<<grid, kBlock, 0, s>>>(g->row, op, hubq, c, cap, hub_off);
       bc_plain_finish_level_kernel<<<grid_for(cnt, kBlock, 4096), kBlock, 0, s>>>(order + b, cnt, sigma, delta, bc);
       GX_LAUNCHED("bc backward");

@@ -14,7 +14,8 @@
 //
 // Design (gfx950)
 //  - Top-down expansion uses the load-balanced workgroup expansion of
-//    frontier.hpp (degree > 4096 goes to a grid-wide hub pass).  Discovery is
+//    frontier.hpp (degree above frontier.hub_degree, 128 by default, goes
+//    to a grid-wide hub pass).  Discovery is
 //    an atomicCAS on the level word, so each vertex enters the next frontier
 //    once.  New vertices are staged in an LDS queue and moved to the global
 //    frontier with one atomic per workgroup flush; per-lane counters are
@@ -193,7 +194,7 @@ __global__ void __launch_bounds__(kBlock) td_hubs_kernel(Bufs B) {
 void enqueue_level(const Bufs& B, hipStream_t s) {
   level_begin_kernel<<<1, 64, 0, s>>>(B.S);
   td_expand_kernel<<<kGrid, kBlock, 0, s>>>(B);
-  hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off);
+  launch_hub_offsets<kBlock>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off, s);
   td_hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
   level_end_kernel<<<1, 64, 0, s>>>(B.S);
   GX_LAUNCHED("bfs level");
@@ -304,7 +305,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   const unsigned cap = static_cast<unsigned>(n);
   Bufs B{g->row, g->col, level, nullptr, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, n + 1), hub_degree()};
+         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m)};
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));
   RegionTimer region(s);
   KernelTimer ktimer(s);
@@ -369,7 +370,7 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   const unsigned cap = static_cast<unsigned>(n);
   Bufs B{g->row, g->col, level, parent, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, n + 1), hub_degree()};
+         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m)};
   const int64_t words = (n + 31) / 32;
   uint32_t* bm[2] = {g->scratch<uint32_t>(WS_BITMAP0, words), g->scratch<uint32_t>(WS_BITMAP1, words)};
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));

@@ -194,7 +194,7 @@ extern "C" gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* stp) {
   uint32_t* parent = g->scratch<uint32_t>(WS_V0, cap);
   uint32_t* rest = g->scratch<uint32_t>(WS_Q1, cap);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, cap);
-  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(cap));
   unsigned* ctr = g->scratch<unsigned>(WS_COUNTERS, 4);  // [0] hubs, [1] overflow, [2] rest, [3] largest
   RegionTimer region(s);
   KernelTimer ktimer(s);
@@ -210,9 +210,9 @@ extern "C" gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* stp) {
     cc_rest_list_kernel<<<vgrid, kBlock, 0, s>>>(g->row, parent, n, &ctr[3], rest, &ctr[2], cap, &ctr[1]);
     HookOp op{g->row, g->col, parent};
     cc_hook_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, rest, &ctr[2], hubq, &ctr[0], cap, &ctr[1],
-                                                 hub_degree());
+                                                 hub_degree(g->m));
     GX_LAUNCHED("cc_hook_kernel");
-    hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &ctr[0], cap, hub_off);
+    launch_hub_offsets<kBlock>(g->row, hubq, &ctr[0], cap, hub_off, s);
     cc_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, &ctr[0], cap, hub_off);
     ktimer.end();
     GX_LAUNCHED("cc_hubs_kernel");

@@ -80,58 +80,93 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
   }
 }
 
-// Step offsets of the hub list for expand_hubs: off[h] = sum over h' < h of
-// ceil(deg(hubq[h']) / BLOCK), off[hubs] = total steps, with hubs =
-// min(*hub_count, cap) (the count runs past cap when the hub queue
-// overflowed; the caller reports that).  One workgroup; `off` holds cap + 1.
-// NT = threads of the calling workgroup (all of them call it).
-template <int BLOCK, int NT>
-__device__ __forceinline__ void hub_offsets_block(const int32_t* __restrict__ row, const uint32_t* __restrict__ hubq,
-                                                  unsigned hub_count, unsigned cap, uint32_t* __restrict__ off) {
+// Step offsets of the hub list for expand_hubs, computed by
+// kHubSlices workgroups: the hub list hubq[0..hubs) (hubs = min(*hub_count,
+// cap); the count runs past cap when the hub queue overflowed, which the
+// caller reports) is cut into kHubSlices slices of 2^hub_chunk_shift(hubs)
+// hubs (the last ones may be empty), and slice k
+// writes off[h] = sum over hubs h' < h of its own slice of ceil(deg(hubq[h'])
+// / BLOCK) for its hubs h, and its total to off[cap + 1 + k].  expand_hubs
+// adds the slice bases; it must be given the same `cap`.  `off` holds
+// hub_off_words(cap) words.
+constexpr int kHubSlices = 256;
+inline int64_t hub_off_words(int64_t cap) { return cap + 1 + kHubSlices; }
+
+// log2 of the slice length: the smallest power of two with
+// kHubSlices << shift >= hubs (a shift, not a division, in the search).
+__device__ __forceinline__ unsigned hub_chunk_shift(unsigned hubs) {
+  unsigned sh = 0;
+  while ((static_cast<uint64_t>(kHubSlices) << sh) < hubs) ++sh;
+  return sh;
+}
+
+constexpr int kHubScanThreads = 256;
+
+template <int BLOCK>
+__global__ void __launch_bounds__(kHubScanThreads) hub_offsets_kernel(const int32_t* __restrict__ row,
+                                                                      const uint32_t* __restrict__ hubq,
+                                                                      const unsigned* __restrict__ hub_count,
+                                                                      unsigned cap, uint32_t* __restrict__ off) {
+  constexpr int NT = kHubScanThreads;
   __shared__ uint32_t lds[NT / kWave + 1];
-  const unsigned hubs = min(hub_count, cap);
+  const unsigned hubs = min(*hub_count, cap);
+  const unsigned sh = hub_chunk_shift(hubs);
+  const unsigned b0 = static_cast<unsigned>(min(static_cast<uint64_t>(blockIdx.x) << sh, static_cast<uint64_t>(hubs)));
+  const unsigned b1 = static_cast<unsigned>(min(static_cast<uint64_t>(b0) + (uint64_t(1) << sh), static_cast<uint64_t>(hubs)));
   uint32_t carry = 0;
-  for (unsigned base = 0; base < hubs; base += NT) {
+  for (unsigned base = b0; base < b1; base += NT) {
     const unsigned i = base + threadIdx.x;
     uint32_t c = 0;
-    if (i < hubs) {
+    if (i < b1) {
       const uint32_t v = hubq[i];
       c = static_cast<uint32_t>((row[v + 1] - row[v] + BLOCK - 1) / BLOCK);
     }
     uint32_t total;
     const uint32_t ex = block_exclusive_scan<NT>(c, lds, &total);
-    if (i < hubs) off[i] = carry + ex;
+    if (i < b1) off[i] = carry + ex;
     carry += total;
   }
-  if (threadIdx.x == 0) off[hubs] = carry;
+  if (threadIdx.x == 0) off[cap + 1 + blockIdx.x] = carry;
 }
 
 template <int BLOCK>
-__global__ void __launch_bounds__(1024) hub_offsets_kernel(const int32_t* __restrict__ row,
-                                                           const uint32_t* __restrict__ hubq,
-                                                           const unsigned* __restrict__ hub_count, unsigned cap,
-                                                           uint32_t* __restrict__ off) {
-  hub_offsets_block<BLOCK, 1024>(row, hubq, *hub_count, cap, off);
+inline void launch_hub_offsets(const int32_t* row, const uint32_t* hubq, const unsigned* hub_count, unsigned cap,
+                               uint32_t* off, hipStream_t s) {
+  hub_offsets_kernel<BLOCK><<<kHubSlices, kHubScanThreads, 0, s>>>(row, hubq, hub_count, cap, off);
 }
 
 // Edges of the hub vertices hubq[0..hubs), cut into BLOCK-edge steps that
-// are spread over every workgroup (step -> hub by binary search over `off`
-// from hub_offsets_kernel).
+// are spread over every workgroup (step -> hub by binary search over the
+// offsets of hub_offsets_kernel plus the scanned slice totals).
 template <int BLOCK, class Op>
 __device__ __forceinline__ void expand_hubs(const int32_t* __restrict__ row, const uint32_t* __restrict__ hubq,
                                             unsigned hub_count, unsigned cap, const uint32_t* __restrict__ off,
                                             Op& op) {
+  static_assert(BLOCK >= kHubSlices, "one slice total per lane");
+  __shared__ uint32_t s_base[kHubSlices];
+  __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+  __shared__ uint32_t s_total;
   const unsigned hubs = min(hub_count, cap);
   if (hubs == 0) return;
-  const uint32_t total = off[hubs];
+  const unsigned sh = hub_chunk_shift(hubs);
+  {
+    const uint32_t t = threadIdx.x < kHubSlices ? off[cap + 1 + threadIdx.x] : 0u;
+    uint32_t total;
+    const uint32_t ex = block_exclusive_scan<BLOCK>(t, s_scan, &total);
+    if (threadIdx.x < kHubSlices) s_base[threadIdx.x] = ex;
+    if (threadIdx.x == 0) s_total = total;
+    __syncthreads();
+  }
+  const uint32_t total = s_total;
   for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
-    unsigned lo = 0, hi = hubs - 1;  // last h with off[h] <= c
+    unsigned lo = 0, hi = hubs - 1;  // last h with off(h) <= c (steps >= 1 per hub: increasing)
     while (lo < hi) {
       const unsigned mid = (lo + hi + 1) >> 1;
-      if (off[mid] <= c) lo = mid; else hi = mid - 1;
+      if (off[mid] + s_base[mid >> sh] <= c) lo = mid; else hi = mid - 1;
     }
     const uint32_t v = hubq[lo];
-    const int64_t e = row[v] + static_cast<int64_t>(c - off[lo]) * BLOCK + threadIdx.x;
+    const uint32_t first = off[lo] + s_base[lo >> sh];
+    const int64_t e = row[v] + static_cast<int64_t>(c - first) * BLOCK + threadIdx.x;
     const bool valid = e < row[v + 1];
     op(valid, v, valid ? static_cast<int32_t>(e) : 0);
     op.after_step();

@@ -153,7 +153,7 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   const unsigned cap = static_cast<unsigned>(std::max<int64_t>(n, 1));
   uint32_t* q[2] = {g->scratch<uint32_t>(WS_Q0, cap), g->scratch<uint32_t>(WS_Q1, cap)};
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, cap);
-  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(cap));
   KcoreCounters* c = g->scratch<KcoreCounters>(WS_COUNTERS, 1);
   KcoreCounters* host = static_cast<KcoreCounters*>(g->host_pinned(sizeof(KcoreCounters)));
   RegionTimer region(s);
@@ -171,12 +171,12 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
     items += count;
     const int out = in ^ 1;
     kcore_round_begin_kernel<<<1, 64, 0, s>>>(c, out);
-    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap, hub_degree()};
+    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap, hub_degree(g->m)};
     ktimer.begin();
     kcore_peel_kernel<<<std::min<unsigned>(256 * 8, grid_for(count, kBlock)), kBlock, 0, s>>>(g->row, op, q[in], in,
                                                                                               hubq);
     GX_LAUNCHED("kcore_peel_kernel");
-    hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+    launch_hub_offsets<kBlock>(g->row, hubq, &c->hub_count, cap, hub_off, s);
     kcore_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
     ktimer.end();
     GX_LAUNCHED("kcore_hubs_kernel");

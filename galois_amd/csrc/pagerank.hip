@@ -834,7 +834,7 @@ extern "C" gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol, uint3
   uint32_t* active = g->scratch<uint32_t>(WS_Q0, n);
   uint32_t* pushers = g->scratch<uint32_t>(WS_Q1, n);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, n);
-  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, n + 1);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n));
   float* residual = g->scratch<float>(WS_V0, n);
   float* delta = g->scratch<float>(WS_V1, n);
   uint32_t* stamp = g->scratch<uint32_t>(WS_V2, n);
@@ -858,8 +858,8 @@ extern "C" gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol, uint3
         active, c, 0, pr, residual, delta, g->row, alpha, tol, pushers, 1, cap);
     push_set_kernel<<<1, 64, 0, s>>>(c, 0, 0u);
     PushOp op{g->col, delta, residual, stamp, active, c, nullptr, tol, it, 0, cap};
-    push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq, hub_degree());
-    hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, &c->hub_count, cap, hub_off);
+    push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq, hub_degree(g->m));
+    launch_hub_offsets<kBlock>(g->row, hubq, &c->hub_count, cap, hub_off, s);
     push_hubs_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
     ktimer.end();
     GX_LAUNCHED("push_hubs_kernel");

@@ -377,7 +377,7 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   Real* carry = g->scratch<Real>(WS_V2, 3 * P.ntiles);
   uint32_t* hubq = g->scratch<uint32_t>(WS_Q2, n);
   unsigned* hubc = g->scratch<unsigned>(WS_Q3, 4);
-  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, n + 1);
+  uint32_t* hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n));
   AdState* state = g->scratch<AdState>(WS_COUNTERS, 1);
   AdState* host = static_cast<AdState*>(g->host_pinned(sizeof(AdState)));
   RegionTimer region(s);
@@ -409,8 +409,8 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   zero_u32_kernel<<<1, 64, 0, s>>>(hubc, 2);
   GradOp<Real> op{g->col, x, lam_f, pl, inv_s, grad_w, static_cast<Real>(alpha)};
   grad_edges_kernel<Real><<<grid_for(n, kBlock, 256 * 16), kBlock, 0, s>>>(g->row, op, static_cast<unsigned>(n), hubq,
-                                                                          hubc, hubc + 1, hub_degree());
-  hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(g->row, hubq, hubc, static_cast<unsigned>(n), hub_off);
+                                                                          hubc, hubc + 1, hub_degree(g->m));
+  launch_hub_offsets<kBlock>(g->row, hubq, hubc, static_cast<unsigned>(n), hub_off, s);
   grad_hubs_kernel<Real><<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hubc, static_cast<unsigned>(n), hub_off);
   grad_t_kernel<Real><<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(lam_f, n, static_cast<Real>(1.0 - alpha), grad_t);
   GX_LAUNCHED("adjoint gradients");

@@ -634,11 +634,10 @@ void run_edges(gx_part* p, uint32_t begin, uint32_t end, const Op& op) {
   PartCounters* c = p->counters.as<PartCounters>();
   const unsigned cap = static_cast<unsigned>(p->n);
   const int32_t* row = p->row.as<int32_t>();
-  range_expand_kernel<Op><<<grid_of(count), kBlock, 0, p->stream>>>(row, begin, count, hub_degree(),
+  range_expand_kernel<Op><<<grid_of(count), kBlock, 0, p->stream>>>(row, begin, count, hub_degree(p->m),
                                                                      p->hubq.as<uint32_t>(), c, cap, op);
   GX_LAUNCHED("range_expand_kernel");
-  hub_offsets_kernel<kBlock><<<1, 1024, 0, p->stream>>>(row, p->hubq.as<uint32_t>(), &c->hub_count, cap,
-                                                       p->hub_off.as<uint32_t>());
+  launch_hub_offsets<kBlock>(row, p->hubq.as<uint32_t>(), &c->hub_count, cap, p->hub_off.as<uint32_t>(), p->stream);
   GX_LAUNCHED("hub_offsets_kernel");
   range_hubs_kernel<Op><<<kMaxGrid, kBlock, 0, p->stream>>>(row, p->hubq.as<uint32_t>(), c, cap,
                                                            p->hub_off.as<uint32_t>(), op);
@@ -721,7 +720,7 @@ extern "C" gx_status gx_part_load(gx_part* p, uint64_t n, uint64_t m, const uint
     GX_HIP(hipMemset(p->bits[k].p, 0, p->words() * 8));
   }
   p->hubq.ensure(n * 4);
-  p->hub_off.ensure((n + 1) * 4);
+  p->hub_off.ensure(hub_off_words(n) * 4);
   p->push.ensure(n * 4);
   p->bq.ensure(n * 4);
   p->loaded = true;

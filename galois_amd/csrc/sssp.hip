@@ -314,7 +314,7 @@ __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
 void enqueue_step(const Bufs& B, hipStream_t s) {
   step_kernel<<<1, 64, 0, s>>>(B.S);
   relax_kernel<<<kGrid, kBlock, 0, s>>>(B);
-  hub_offsets_kernel<kBlock><<<1, 1024, 0, s>>>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off);
+  launch_hub_offsets<kBlock>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off, s);
   hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
   far_split_kernel<<<kGrid, kBlock, 0, s>>>(B);
   GX_LAUNCHED("sssp step");
@@ -348,8 +348,8 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.S = g->scratch<SsspState>(WS_COUNTERS, 1);
   B.shift = shift;
   B.cap = cap;
-  B.hub_off = g->scratch<uint32_t>(WS_HUBOFF, cap + 1);
-  B.hub = hub_degree();
+  B.hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(cap));
+  B.hub = hub_degree(g->m);
   SsspState* host = static_cast<SsspState*>(g->host_pinned(sizeof(SsspState)));
 
   RegionTimer region(s);
