@@ -222,30 +222,32 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict_
 // LPJ to a job; lane `sub` of a group takes edges sub + LPJ*k, k < 8, so for
 // each k the LPJ lanes of a job hold LPJ consecutive edges and a ballot of
 // the group flags, masked to those lanes, counts the groups they open.  A
-// wavefront loads the destinations of kU job groups, derives every edge's
-// group, loads the contributions, and only then adds, so that kU x 8 loads
-// per lane are in flight in each round.
+// wavefront works in rounds of kU job groups, software-pipelined: while a
+// round loads its contributions and adds them, the next round's destination
+// ids and the job entries of the round after that are already in flight, so
+// each round waits for one load latency instead of three (job entry ->
+// destination ids -> contribution).
 template <int LPJ, class Real>
 __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_t j0, uint32_t j1,
                                            const typename PbAcc<Real>::V* __restrict__ vals,
                                            const uint16_t* __restrict__ dst16, typename PbAcc<Real>::T* acc) {
   using V = typename PbAcc<Real>::V;
   constexpr int kGroups = kWave / LPJ;
-  constexpr int kU = sizeof(Real) == 4 ? 4 : 2;
+  constexpr int kU = sizeof(Real) == 4 ? 2 : 1;
+  constexpr uint32_t kStride = kWaves * kGroups * kU;
   const unsigned lane = lane_id(), wave = threadIdx.x / kWave;
   const unsigned grp = lane / LPJ, sub = lane % LPJ;
   const uint64_t mine = LPJ == kWave ? ~0ull : ((uint64_t(1) << LPJ) - 1) << (grp * LPJ);
   const uint64_t upto = mine & ((uint64_t(2) << lane) - 1);  // my job's lanes up to mine (lane 63: all)
-  for (uint32_t jb = j0 + wave * kGroups * kU; jb < j1; jb += kWaves * kGroups * kU) {
-    Job J[kU];
-    V v[kU][8];
-    uint32_t d[kU][8];
+  auto load_jobs = [&](uint32_t jb, Job (&J)[kU]) {
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const uint32_t j = jb + u * kGroups + grp;
       J[u] = Job{0, 0, 0};
       if (j < j1) J[u] = jobs[j];
     }
+  };
+  auto load_dst = [&](const Job (&J)[kU], uint32_t (&d)[kU][8]) {
 #pragma unroll
     for (int u = 0; u < kU; ++u)
 #pragma unroll
@@ -253,6 +255,16 @@ __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_
         const uint32_t i = sub + LPJ * k;
         d[u][k] = i < J[u].len ? rs::ld_stream(&dst16[J[u].start + i]) : 0u;
       }
+  };
+  uint32_t jb = j0 + wave * kGroups * kU;
+  if (jb >= j1) return;  // wavefront-uniform
+  Job J[kU], Jn[kU];
+  uint32_t d[kU][8];
+  load_jobs(jb, J);
+  load_dst(J, d);
+  load_jobs(jb + kStride, Jn);
+  for (;;) {
+    V v[kU][8];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       uint32_t carry = J[u].gbase;
@@ -272,11 +284,27 @@ __device__ __forceinline__ void accum_jobs(const Job* __restrict__ jobs, uint32_
         if (sub + LPJ * k < J[u].len) v[u][k] = vals[q];
       }
     }
+    const bool more = jb + kStride < j1;  // wavefront-uniform
+    uint32_t dn[kU][8];
+    Job Jnn[kU];
+    if (more) {
+      load_dst(Jn, dn);
+      load_jobs(jb + 2 * kStride, Jnn);
+    }
 #pragma unroll
     for (int u = 0; u < kU; ++u)
 #pragma unroll
       for (int k = 0; k < 8; ++k)
         if (sub + LPJ * k < J[u].len) atomicAdd(&acc[d[u][k] & (kSpan - 1)], PbAcc<Real>::conv(v[u][k]));
+    if (!more) break;
+    jb += kStride;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      J[u] = Jn[u];
+      Jn[u] = Jnn[u];
+#pragma unroll
+      for (int k = 0; k < 8; ++k) d[u][k] = dn[u][k];
+    }
   }
 }
 
