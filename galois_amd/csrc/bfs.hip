@@ -72,7 +72,8 @@ struct Bufs {
   BfsState* S;
   unsigned cap;
   uint32_t* hub_off;
-  uint32_t hub;  // frontier.hub_degree
+  uint32_t hub;   // frontier.hub_degree
+  uint32_t* vis;  // visited bitmap (top-down BFS only; null for the DO-BFS)
 };
 
 __global__ void bfs_init_kernel(uint32_t* __restrict__ level, uint32_t* __restrict__ parent, int64_t n,
@@ -85,8 +86,10 @@ __global__ void bfs_init_kernel(uint32_t* __restrict__ level, uint32_t* __restri
   }
 }
 
-__global__ void bfs_reset_kernel(BfsState* S, uint32_t* q0, uint32_t src, uint32_t* level, uint32_t* parent) {
+__global__ void bfs_reset_kernel(BfsState* S, uint32_t* q0, uint32_t src, uint32_t* level, uint32_t* parent,
+                                 uint32_t* vis) {
   if (threadIdx.x == 0) {
+    if (vis) vis[src >> 5] |= 1u << (src & 31);
     S->level_out = level;
     S->parent_out = parent;
     S->count[0] = 1;
@@ -143,13 +146,19 @@ struct TdOp {
   LdsQueue<kQCap>* sq;
   uint32_t nl;
   unsigned long long scout;  // this lane's degree sum of discovered vertices
+  uint32_t* vis;
+  // With a visited bitmap (n/8 bytes, mostly cache-resident) an edge to an
+  // already discovered vertex is dropped without the random read of its
+  // level word; a bit set late only sends the edge on to the level check.
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     bool won = false;
     uint32_t v = 0;
     if (valid) {
       v = static_cast<uint32_t>(col[e]);
-      if (level[v] == kInf && atomicCAS(&level[v], kInf, nl) == kInf) {
+      const bool seen = vis && ((vis[v >> 5] >> (v & 31)) & 1u);
+      if (!seen && level[v] == kInf && atomicCAS(&level[v], kInf, nl) == kInf) {
         won = true;
+        if (vis) atomicOr(&vis[v >> 5], 1u << (v & 31));
         if (parent) parent[v] = u;
         scout += static_cast<unsigned long long>(row[v + 1] - row[v]);
       }
@@ -162,7 +171,7 @@ struct TdOp {
 __device__ __forceinline__ TdOp make_td(const Bufs& B, LdsQueue<kQCap>* sq) {
   BfsState* S = B.S;
   const unsigned nq = S->q ^ 1;
-  return TdOp{B.row, B.col, S->level_out, S->parent_out, B.q[nq], &S->count[nq], B.cap, S, sq, S->level, 0ull};
+  return TdOp{B.row, B.col, S->level_out, S->parent_out, B.q[nq], &S->count[nq], B.cap, S, sq, S->level, 0ull, B.vis};
 }
 
 __device__ __forceinline__ void td_finish(TdOp& op) {
@@ -303,18 +312,27 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   const int64_t n = g->n;
   hipStream_t s = g->stream;
   const unsigned cap = static_cast<unsigned>(n);
+  // Visited bitmap from 2^25 vertices on: below that the 4-byte level array
+  // (<= 128 MB) stays cache-resident and the bitmap's clear and atomics cost
+  // more than they save (RMAT-18 0.455 -> 0.52 ms, RMAT-22 2.03 -> 2.21 ms
+  // with it; RMAT-26 33.9 -> 24.7 ms).
+  const int64_t words = (n + 31) / 32;
+  const bool use_vis = tune_value("bfs.visited_bitmap", n >= (int64_t(1) << 25) ? 1 : 0) != 0;
+  uint32_t* vis = use_vis ? g->scratch<uint32_t>(WS_BITMAP0, words) : nullptr;
   Bufs B{g->row, g->col, level, nullptr, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m)};
+         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m), vis};
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));
   RegionTimer region(s);
   KernelTimer ktimer(s);
   bfs_init_kernel<<<sweep_grid(n), kBlock, 0, s>>>(level, nullptr, n, src);
-  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src, level, nullptr);
+  if (vis) fill_kernel<uint32_t><<<sweep_grid(words), kBlock, 0, s>>>(vis, 0u, words);
+  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src, level, nullptr, B.vis);
   GX_LAUNCHED("bfs init");
   uint64_t key = 0;
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.q[0],
-                        (const void*)B.q[1], (const void*)B.hubq, (const void*)B.S, (const void*)B.hub_off})
+                        (const void*)B.q[1], (const void*)B.hubq, (const void*)B.S, (const void*)B.hub_off,
+                        (const void*)B.vis})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
   key = mix_key(mix_key(key, cap), B.hub);
   uint32_t replays = 0;
@@ -334,7 +352,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = host->level - 1;  // levels with a non-empty frontier beyond the source
-    st->launches = 2 + replays * kLevelsPerGraph * 5;
+    st->launches = (vis ? 3 : 2) + replays * kLevelsPerGraph * 5;
     st->edges_visited = host->edges + static_cast<uint64_t>(0);
     st->items = host->items;
     st->elapsed_ms = ms;
@@ -370,7 +388,7 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   const unsigned cap = static_cast<unsigned>(n);
   Bufs B{g->row, g->col, level, parent, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m)};
+         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m), nullptr};
   const int64_t words = (n + 31) / 32;
   uint32_t* bm[2] = {g->scratch<uint32_t>(WS_BITMAP0, words), g->scratch<uint32_t>(WS_BITMAP1, words)};
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));
@@ -383,7 +401,7 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   KernelTimer ktimer(s);
   uint32_t launches = 2;
   bfs_init_kernel<<<sweep_grid(n), kBlock, 0, s>>>(level, parent, n, src);
-  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src, level, parent);
+  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src, level, parent, nullptr);
   GX_LAUNCHED("bfs init");
   int32_t rsrc[2];
   read_bytes(g, rsrc, g->row + src, sizeof(rsrc));

@@ -112,11 +112,14 @@ void gx_set_profiling(int on);
  *                          of their own (their ranks stay L2-resident); 0 =
  *                          one pass (the default)
  *   "pr.blocking"          pull PageRank: 1 (default) = propagation-
- *                          blocked sweep for FP32 (streamed per-edge
+ *                          blocked sweep for FP32 (streamed per-group
  *                          contributions, exact fixed-point row sums), 2 =
  *                          also for FP64 (FP64 atomic row sums, order
  *                          dependent in the last bit), 0 = merge-path pull
  *                          sweep for both (Kahan row sums in FP32)
+ *   "bfs.visited_bitmap"   gx_bfs: 1 = test a visited bitmap before the
+ *                          level word of every edge's target, 0 = not; the
+ *                          default uses it from 2^25 vertices on
  * Unknown keys and out-of-range values are GX_ERR_INVALID. */
 gx_status gx_tune_set(const char* key, int64_t value);
 gx_status gx_tune_get(const char* key, int64_t* value);

@@ -364,6 +364,24 @@ def test_pagerank_blocked_shapes(gx, oracle):
     assert st1.iterations <= 200 and (st1.residual < 1e-10 or st1.iterations == 200)
 
 
+@pytest.mark.parametrize("vis", [0, 1])
+def test_bfs_visited_bitmap_knob(gx, oracle, rmat16, vis):
+    """Top-down BFS with and without the visited bitmap (used by default from
+    2^25 vertices on): levels bit-exact vs the oracle either way."""
+    G, row, col, _ = rmat16
+    say(f"rmat16 bfs visited_bitmap={vis}")
+    src = int(np.argmax(np.diff(row)))
+    exp = oracle.bfs(row, col, src)
+    gx.tune_set("bfs.visited_bitmap", vis)
+    try:
+        a, st = gx.bfs(G, src)
+        b, _ = gx.bfs(G, 7)
+    finally:
+        gx.tune_set("bfs.visited_bitmap", -1)
+    np.testing.assert_array_equal(a.cpu().numpy().view(np.uint32), exp)
+    np.testing.assert_array_equal(b.cpu().numpy().view(np.uint32), oracle.bfs(row, col, 7))
+
+
 @pytest.mark.parametrize("hub", [64, 100000])
 def test_hub_degree_knob(gx, oracle, rmat16, hub):
     """Results do not depend on the grid-wide hub threshold."""
