@@ -73,6 +73,31 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStr
                                  rocprim::plus<uint32_t>(), s));
 }
 
+__global__ void max_degree_kernel(const int32_t* __restrict__ row, int64_t n, unsigned* __restrict__ out) {
+  unsigned mx = 0;
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    mx = max(mx, static_cast<unsigned>(row[v + 1] - row[v]));
+  mx = wave_max(mx);
+  if (lane_id() == 0 && mx) atomicMax(out, mx);
+}
+
+int64_t max_degree(gx_graph* g) {
+  if (g->max_deg >= 0) return g->max_deg;
+  unsigned mx = 0;
+  if (g->n > 0) {
+    DevBuf d;
+    d.ensure(sizeof(unsigned));
+    GX_HIP(hipMemsetAsync(d.p, 0, sizeof(unsigned), g->stream));
+    max_degree_kernel<<<grid_for(g->n, 256, 4096), 256, 0, g->stream>>>(g->row, g->n, d.as<unsigned>());
+    GX_LAUNCHED("max_degree_kernel");
+    GX_HIP(hipMemcpyAsync(&mx, d.p, sizeof(unsigned), hipMemcpyDeviceToHost, g->stream));
+    GX_HIP(hipStreamSynchronize(g->stream));
+  }
+  g->max_deg = mx;
+  return g->max_deg;
+}
+
 void sort_keys_u64(const uint64_t* in, uint64_t* out, int64_t count, int end_bit, hipStream_t s) {
   if (count <= 0) return;
   size_t tmp = 0;

@@ -179,6 +179,7 @@ struct gx_graph {
   // Derived per-vertex data, built on demand.
   gx::DevBuf outdeg_buf;  // uint32 out-degree (for transpose-based PR)
   bool has_outdeg = false;
+  int64_t max_deg = -1;  // largest out-degree, computed on demand (gx::max_degree)
   gx::PullPlan pull_plan;            // degree-ordered transpose (PageRank pull)
   gx::PullPlan csc_plan, csr_plan;   // plain transpose / CSR (weighted PageRank and adjoint)
   gx::DegreeOrder degree_order;
@@ -318,6 +319,8 @@ void build_degree_order(gx_graph* g);  // g->degree_order (relabelled transpose)
 void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipStream_t s);
 // Exclusive prefix sum of `count` uint32 values into uint32 `out` (may alias).
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s);
+// Largest out-degree of the graph (one reduction, then cached).
+int64_t max_degree(gx_graph* g);
 // Ascending radix sort of count keys over bits [0, end_bit) (in is not changed).
 void sort_keys_u64(const uint64_t* in, uint64_t* out, int64_t count, int end_bit, hipStream_t s);
 // Device->host copy of a few bytes through the graph's pinned block.
