@@ -22,8 +22,9 @@
 //    reduced once per workgroup.
 //  - gx_bfs drives its levels on the device: each level is a fixed kernel
 //    sequence (begin, expand, hubs, end) that reads the frontier size from a
-//    device state word; 8 levels are captured in one HIP graph and replayed
-//    until the frontier is empty.
+//    device state word; 8 levels are captured in one HIP graph (the end of a
+//    level and the begin of the next in one launch) and replayed until the
+//    frontier is empty.
 //  - Bottom-up steps test a 1-bit-per-vertex frontier bitmap (n/8 bytes, L2
 //    resident for the graphs here) and stop at the first hit.
 #include <algorithm>
@@ -113,8 +114,7 @@ __global__ void bfs_reset_kernel(BfsState* S, uint32_t* q0, uint32_t src, uint32
   }
 }
 
-__global__ void level_begin_kernel(BfsState* S) {
-  if (threadIdx.x != 0) return;
+__device__ __forceinline__ void level_begin(BfsState* S) {
   if (S->done || S->overflow || S->count[S->q] == 0) {
     S->active = 0;
     S->done = 1;
@@ -127,12 +127,28 @@ __global__ void level_begin_kernel(BfsState* S) {
   S->scout = 0;
 }
 
-__global__ void level_end_kernel(BfsState* S) {
-  if (threadIdx.x != 0 || !S->active) return;
+__device__ __forceinline__ void level_end(BfsState* S) {
+  if (!S->active) return;
   S->q ^= 1;
   S->items += S->count[S->q];
   S->edges += S->scout;
   S->active = 0;
+}
+
+__global__ void level_begin_kernel(BfsState* S) {
+  if (threadIdx.x == 0) level_begin(S);
+}
+
+__global__ void level_end_kernel(BfsState* S) {
+  if (threadIdx.x == 0) level_end(S);
+}
+
+// End of one level and begin of the next in one launch (inside a captured
+// batch of levels).
+__global__ void level_turn_kernel(BfsState* S) {
+  if (threadIdx.x != 0) return;
+  level_end(S);
+  level_begin(S);
 }
 
 // Plain pointers only: the output arrays come from the device state and the
@@ -205,17 +221,36 @@ __global__ void __launch_bounds__(kBlock) td_hubs_kernel(Bufs B) {
   td_finish(op);
 }
 
-// `hubs` false: no vertex exceeds the hub threshold, so the hub list stays
-// empty and its two launches are left out of the level.
-void enqueue_level(const Bufs& B, bool hubs, hipStream_t s) {
-  level_begin_kernel<<<1, 64, 0, s>>>(B.S);
+// Expansion of one level.  `hubs` false: no vertex exceeds the hub
+// threshold, so the hub list stays empty and its two launches are left out.
+void enqueue_expand(const Bufs& B, bool hubs, hipStream_t s) {
   td_expand_kernel<<<hubs ? kGrid : kGridNoHubs, kBlock, 0, s>>>(B);
   if (hubs) {
     launch_hub_offsets<kBlock>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off, s);
     td_hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
   }
+}
+
+// One level with its begin and end (a level launched on its own).
+void enqueue_level(const Bufs& B, bool hubs, hipStream_t s) {
+  level_begin_kernel<<<1, 64, 0, s>>>(B.S);
+  enqueue_expand(B, hubs, s);
   level_end_kernel<<<1, 64, 0, s>>>(B.S);
   GX_LAUNCHED("bfs level");
+}
+
+// `levels` consecutive levels with the end of each and the begin of the
+// next in one launch.
+void enqueue_levels(const Bufs& B, bool hubs, int levels, hipStream_t s) {
+  level_begin_kernel<<<1, 64, 0, s>>>(B.S);
+  for (int i = 0; i < levels; ++i) {
+    enqueue_expand(B, hubs, s);
+    if (i + 1 < levels)
+      level_turn_kernel<<<1, 64, 0, s>>>(B.S);
+    else
+      level_end_kernel<<<1, 64, 0, s>>>(B.S);
+  }
+  GX_LAUNCHED("bfs levels");
 }
 
 // ------------------------------------------------------------ bottom-up
@@ -349,7 +384,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   for (;;) {
     ktimer.begin();
     g->replay(g->bfs_batch, key, [&](hipStream_t cs) {
-      for (int i = 0; i < kLevelsPerGraph; ++i) enqueue_level(B, hubs, cs);
+      enqueue_levels(B, hubs, kLevelsPerGraph, cs);
     });
     ktimer.end();
     ++replays;
@@ -362,7 +397,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = host->level - 1;  // levels with a non-empty frontier beyond the source
-    st->launches = (vis ? 3 : 2) + replays * kLevelsPerGraph * (hubs ? 5 : 3);
+    st->launches = (vis ? 3 : 2) + replays * (kLevelsPerGraph * (hubs ? 4 : 2) + 1);
     st->edges_visited = host->edges + static_cast<uint64_t>(0);
     st->items = host->items;
     st->elapsed_ms = ms;
