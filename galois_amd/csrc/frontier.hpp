@@ -135,9 +135,12 @@ inline void launch_hub_offsets(const int32_t* row, const uint32_t* hubq, const u
   hub_offsets_kernel<BLOCK><<<kHubSlices, kHubScanThreads, 0, s>>>(row, hubq, hub_count, cap, off);
 }
 
-// Edges of the hub vertices hubq[0..hubs), cut into BLOCK-edge steps that
-// are spread over every workgroup (step -> hub by binary search over the
-// offsets of hub_offsets_kernel plus the scanned slice totals).
+// Edges of the hub vertices hubq[0..hubs), cut into BLOCK-edge steps (a
+// hub's steps are consecutive).  Each workgroup takes a contiguous range of
+// steps: one binary search over the hub offsets (those of hub_offsets_kernel
+// plus the scanned slice totals) finds the hub of its first step, and the
+// following steps advance to the next hub when they pass its first step, so
+// a step costs no search.
 template <int BLOCK, class Op>
 __device__ __forceinline__ void expand_hubs(const int32_t* __restrict__ row, const uint32_t* __restrict__ hubq,
                                             unsigned hub_count, unsigned cap, const uint32_t* __restrict__ off,
@@ -158,16 +161,32 @@ __device__ __forceinline__ void expand_hubs(const int32_t* __restrict__ row, con
     __syncthreads();
   }
   const uint32_t total = s_total;
-  for (uint32_t c = blockIdx.x; c < total; c += gridDim.x) {
-    unsigned lo = 0, hi = hubs - 1;  // last h with off(h) <= c (steps >= 1 per hub: increasing)
-    while (lo < hi) {
-      const unsigned mid = (lo + hi + 1) >> 1;
-      if (off[mid] + s_base[mid >> sh] <= c) lo = mid; else hi = mid - 1;
+  const uint32_t per = (total + gridDim.x - 1) / gridDim.x;
+  const uint32_t c0 = static_cast<uint32_t>(min(static_cast<uint64_t>(blockIdx.x) * per, static_cast<uint64_t>(total)));
+  const uint32_t c1 = static_cast<uint32_t>(min(static_cast<uint64_t>(c0) + per, static_cast<uint64_t>(total)));
+  if (c0 >= c1) return;
+  auto first_step = [&](unsigned h) { return off[h] + s_base[h >> sh]; };
+  unsigned lo = 0, hi = hubs - 1;  // last h with first_step(h) <= c0 (steps >= 1 per hub: increasing)
+  while (lo < hi) {
+    const unsigned mid = (lo + hi + 1) >> 1;
+    if (first_step(mid) <= c0) lo = mid; else hi = mid - 1;
+  }
+  unsigned h = lo;
+  uint32_t first = first_step(h);
+  uint32_t next = h + 1 < hubs ? first_step(h + 1) : total;
+  uint32_t v = hubq[h];
+  int64_t rb = row[v], re = row[v + 1];
+  for (uint32_t c = c0; c < c1; ++c) {
+    if (c >= next) {  // workgroup-uniform; the next hub has >= 1 step
+      ++h;
+      first = next;
+      next = h + 1 < hubs ? first_step(h + 1) : total;
+      v = hubq[h];
+      rb = row[v];
+      re = row[v + 1];
     }
-    const uint32_t v = hubq[lo];
-    const uint32_t first = off[lo] + s_base[lo >> sh];
-    const int64_t e = row[v] + static_cast<int64_t>(c - first) * BLOCK + threadIdx.x;
-    const bool valid = e < row[v + 1];
+    const int64_t e = rb + static_cast<int64_t>(c - first) * BLOCK + threadIdx.x;
+    const bool valid = e < re;
     op(valid, v, valid ? static_cast<int32_t>(e) : 0);
     op.after_step();
   }
