@@ -229,14 +229,10 @@ bool profiling_enabled();
 // Current value of a tuning knob (gx_tune_set), `dflt` when unset.
 int64_t tune_value(const char* key, int64_t dflt);
 // Degree above which a frontier vertex is expanded by the whole grid.
-// Frontier vertices of higher degree are expanded by the whole grid.  The
-// default follows the graph size: on graphs up to 2^28 edges the frontiers
-// of the power-law levels are too short to fill the grid with workgroup
-// batches, so vertices above degree 128 are spread; on larger graphs the
-// batches fill it and the batched path is cheaper up to degree 1024.
-inline uint32_t hub_degree(int64_t m) {
-  return static_cast<uint32_t>(tune_value("frontier.hub_degree", m > (int64_t(1) << 28) ? 1024 : 128));
-}
+// Frontier vertices of higher degree are expanded by the whole grid (default
+// 128: a workgroup batch of 256 frontier vertices then holds at most 32K
+// edges, so short power-law frontiers still spread over the grid).
+inline uint32_t hub_degree(int64_t /*m*/) { return static_cast<uint32_t>(tune_value("frontier.hub_degree", 128)); }
 
 // Accumulates the device time of the launches of one kernel with HIP events
 // (enabled by gx_set_profiling).  Events are recorded on the graph's stream,

@@ -104,8 +104,7 @@ void gx_set_profiling(int on);
  * (galois_amd/autotune.py) or by hand; -1 restores the built-in default.
  *   "frontier.hub_degree"  frontier vertices with more edges than this are
  *                          expanded by the whole grid (BFS, SSSP, CC, k-core,
- *                          BC, push PageRank); default 128 on graphs
- *                          of up to 2^28 edges, 1024 above; range
+ *                          BC, push PageRank); default 128, range
  *                          [64, 2^30]
  *   "pr.hot_sources"       pull PageRank: in-edges from the first this-many
  *                          sources of the degree order are summed in a pass
