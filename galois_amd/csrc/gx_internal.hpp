@@ -116,9 +116,11 @@ struct PullPlan {
 // vector is a few contiguous megabytes.  trow/tcol are the transpose in new
 // ids (tcol padded by kEdgePad), outdeg is in new order.
 // Edge layout of the propagation-blocked PageRank sweep (pr_blocking.hip),
-// built once per graph from the degree-ordered transpose.  Edges are grouped
-// into runs by (source chunk, destination bin), chunk-major; src16/dst16 are
-// the in-chunk source and in-bin destination of every edge, vals the per-edge
+// built once per graph from the CSR in degree-order ids.  Edges are sorted
+// into runs by (source chunk, destination bin), chunk-major, and by (source,
+// destination) inside a run; a group is the edges of one source in a run.
+// dst16 is the in-bin destination of every edge (bit 15: first edge of a
+// group), gsrc16 the in-chunk source of every group, vals the per-group
 // contributions of the current iteration.  pieces/jobs/tasks/multi are the
 // work tables of the two phases.
 struct PbLayout {

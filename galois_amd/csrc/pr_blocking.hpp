@@ -27,7 +27,7 @@
 //           bin's rows are finalised.
 //
 // On RMAT graphs a hub source has many edges in one bin, so there are far
-// fewer groups than edges (about 0.35 groups per edge at RMAT-26): per edge
+// fewer groups than edges (about 0.33 groups per edge at RMAT-26): per edge
 // the sweep moves 2 + 10 * groups / edges bytes instead of 12.
 //
 // FP32 row sums are exact: every contribution (0 <= y <= 1 for the
