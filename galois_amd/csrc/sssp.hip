@@ -24,13 +24,15 @@
 //    have meanwhile moved nearer; that split then moves nothing and the next
 //    one uses the exact minimum of the items it kept.
 //  - The loop is driven on the device: every step is the same fixed kernel
-//    sequence (step, relax with the stale-item filter fused in as the
-//    expansion's admission test, hub layout, hubs, far-split) whose kernels
-//    read the step's mode from a device state word and return at once when
-//    it is not theirs; the step kernel also closes the previous step.  Steps
-//    are captured once into a HIP graph (kStepsPerGraph steps and a closing
-//    kernel) and replayed until the state says done, so the host
-//    synchronises once per graph replay instead of once per round.
+//    sequence (step; work, which is either the relax, with the stale-item
+//    filter fused in as the expansion's admission test, or the far split, as
+//    the step's mode says; hub layout; hubs) whose kernels read the step's
+//    mode from a device state word and return at once when it is not
+//    theirs; the step kernel also closes the previous step.  On graphs
+//    without hubs the two hub launches are left out.  Steps are captured
+//    once into a HIP graph (kStepsPerGraph steps and a closing kernel) and
+//    replayed until the state says done, so the host synchronises once per
+//    graph replay instead of once per round.
 #include <algorithm>
 
 #include "frontier.hpp"
@@ -251,12 +253,8 @@ __device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn
 }
 
 // Relax step: expand the near items that pass the stale-item filter.
-__global__ void __launch_bounds__(kBlock) relax_kernel(Bufs B) {
+__device__ __forceinline__ void relax_step(const Bufs& B, LdsQueue<kQCap>& qn, LdsQueue<kQCap>& qf) {
   SsspState* S = B.S;
-  if (S->mode != kRelax) return;
-  __shared__ LdsQueue<kQCap> qn, qf;
-  lq_init(qn);
-  lq_init(qf);
   RelaxOp op = make_relax(B, &qn, &qf);
   const unsigned q = S->q;
   expand_queue<kBlock, RelaxOp, true>(B.row, B.nearq[q], S->near_count[q], B.hub, B.hubq, &S->hub_count, B.cap,
@@ -278,14 +276,10 @@ __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
 // Advance step: far items of the new bucket go to the near queue, later
 // ones are kept (one copy per vertex, their buckets fold into the running far
 // minimum), settled ones are dropped.
-__global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
+__device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& qn, LdsQueue<kQCap>& qk) {
   SsspState* S = B.S;
-  if (S->mode != kAdvance) return;
   const uint32_t* __restrict__ dist = S->dist_out;
   const uint32_t next = S->bucket;
-  __shared__ LdsQueue<kQCap> qn, qk;
-  lq_init(qn);
-  lq_init(qk);
   const unsigned fs = S->far_sel, q = S->q;
   const unsigned cnt = S->far_count[fs];
   const uint32_t* farq = B.farq[fs];
@@ -316,17 +310,29 @@ __global__ void __launch_bounds__(kBlock) far_split_kernel(Bufs B) {
   if (lane_id() == 0 && kept_min != 0xffffffffu) atomicMin(&S->far_min, kept_min);
 }
 
+// The work of a step: a relax or an advance (the step's mode picks one; the
+// two never share a step, so one launch serves both).
+__global__ void __launch_bounds__(kBlock) work_kernel(Bufs B) {
+  const unsigned mode = B.S->mode;
+  if (mode != kRelax && mode != kAdvance) return;
+  __shared__ LdsQueue<kQCap> q0, q1;
+  lq_init(q0);
+  lq_init(q1);
+  if (mode == kRelax)
+    relax_step(B, q0, q1);
+  else
+    far_split_step(B, q0, q1);
+}
+
 // `hubs` false: no vertex exceeds the hub threshold, so the hub list stays
 // empty and its two launches are left out of the step.
 void enqueue_step(const Bufs& B, bool hubs, hipStream_t s) {
-  const unsigned grid = hubs ? kGrid : kGridNoHubs;
   step_kernel<<<1, 64, 0, s>>>(B.S);
-  relax_kernel<<<grid, kBlock, 0, s>>>(B);
+  work_kernel<<<hubs ? kGrid : kGridNoHubs, kBlock, 0, s>>>(B);
   if (hubs) {
     launch_hub_offsets<kBlock>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off, s);
     hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
   }
-  far_split_kernel<<<grid, kBlock, 0, s>>>(B);
   GX_LAUNCHED("sssp step");
 }
 
@@ -395,7 +401,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = host->steps;
-    st->launches = 2 + replays * (kStepsPerGraph * (hubs ? 5 : 3) + 1);
+    st->launches = 2 + replays * (kStepsPerGraph * (hubs ? 4 : 2) + 1);
     st->edges_visited = host->edges;
     st->items = host->items;
     st->elapsed_ms = ms;
