@@ -547,7 +547,7 @@ extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
   GX_LAUNCHED("bc_plain_init_kernel");
   launches += 2;
   const unsigned grid = 256 * 8;
-  const uint32_t hub = hub_degree(g->m);
+  const uint32_t hub = hub_degree();
   // In-edges for the path-count pull (the CSR itself for symmetric graphs).
   const int32_t* irow = g->row;
   const int32_t* icol = g->col;

@@ -365,7 +365,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   uint32_t* vis = use_vis ? g->scratch<uint32_t>(WS_BITMAP0, words) : nullptr;
   Bufs B{g->row, g->col, level, nullptr, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m), vis};
+         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(), vis};
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));
   RegionTimer region(s);
@@ -433,7 +433,7 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   const unsigned cap = static_cast<unsigned>(n);
   Bufs B{g->row, g->col, level, parent, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,
-         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(g->m), nullptr};
+         g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(n)), hub_degree(), nullptr};
   const int64_t words = (n + 31) / 32;
   uint32_t* bm[2] = {g->scratch<uint32_t>(WS_BITMAP0, words), g->scratch<uint32_t>(WS_BITMAP1, words)};
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);

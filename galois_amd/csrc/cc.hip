@@ -210,7 +210,7 @@ extern "C" gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* stp) {
     cc_rest_list_kernel<<<vgrid, kBlock, 0, s>>>(g->row, parent, n, &ctr[3], rest, &ctr[2], cap, &ctr[1]);
     HookOp op{g->row, g->col, parent};
     cc_hook_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, rest, &ctr[2], hubq, &ctr[0], cap, &ctr[1],
-                                                 hub_degree(g->m));
+                                                 hub_degree());
     GX_LAUNCHED("cc_hook_kernel");
     launch_hub_offsets<kBlock>(g->row, hubq, &ctr[0], cap, hub_off, s);
     cc_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, &ctr[0], cap, hub_off);

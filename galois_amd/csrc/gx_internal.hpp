@@ -234,7 +234,7 @@ int64_t tune_value(const char* key, int64_t dflt);
 // Frontier vertices of higher degree are expanded by the whole grid (default
 // 128: a workgroup batch of 256 frontier vertices then holds at most 32K
 // edges, so short power-law frontiers still spread over the grid).
-inline uint32_t hub_degree(int64_t /*m*/) { return static_cast<uint32_t>(tune_value("frontier.hub_degree", 128)); }
+inline uint32_t hub_degree() { return static_cast<uint32_t>(tune_value("frontier.hub_degree", 128)); }
 
 // Accumulates the device time of the launches of one kernel with HIP events
 // (enabled by gx_set_profiling).  Events are recorded on the graph's stream,

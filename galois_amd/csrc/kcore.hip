@@ -171,7 +171,7 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
     items += count;
     const int out = in ^ 1;
     kcore_round_begin_kernel<<<1, 64, 0, s>>>(c, out);
-    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap, hub_degree(g->m)};
+    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap, hub_degree()};
     ktimer.begin();
     kcore_peel_kernel<<<std::min<unsigned>(256 * 8, grid_for(count, kBlock)), kBlock, 0, s>>>(g->row, op, q[in], in,
                                                                                               hubq);

@@ -858,7 +858,7 @@ extern "C" gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol, uint3
         active, c, 0, pr, residual, delta, g->row, alpha, tol, pushers, 1, cap);
     push_set_kernel<<<1, 64, 0, s>>>(c, 0, 0u);
     PushOp op{g->col, delta, residual, stamp, active, c, nullptr, tol, it, 0, cap};
-    push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq, hub_degree(g->m));
+    push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq, hub_degree());
     launch_hub_offsets<kBlock>(g->row, hubq, &c->hub_count, cap, hub_off, s);
     push_hubs_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
     ktimer.end();

@@ -409,7 +409,7 @@ void run_adjoint(gx_graph* g, const float* w, const Real* x, const float* c, dou
   zero_u32_kernel<<<1, 64, 0, s>>>(hubc, 2);
   GradOp<Real> op{g->col, x, lam_f, pl, inv_s, grad_w, static_cast<Real>(alpha)};
   grad_edges_kernel<Real><<<grid_for(n, kBlock, 256 * 16), kBlock, 0, s>>>(g->row, op, static_cast<unsigned>(n), hubq,
-                                                                          hubc, hubc + 1, hub_degree(g->m));
+                                                                          hubc, hubc + 1, hub_degree());
   launch_hub_offsets<kBlock>(g->row, hubq, hubc, static_cast<unsigned>(n), hub_off, s);
   grad_hubs_kernel<Real><<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hubc, static_cast<unsigned>(n), hub_off);
   grad_t_kernel<Real><<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(lam_f, n, static_cast<Real>(1.0 - alpha), grad_t);

@@ -634,7 +634,7 @@ void run_edges(gx_part* p, uint32_t begin, uint32_t end, const Op& op) {
   PartCounters* c = p->counters.as<PartCounters>();
   const unsigned cap = static_cast<unsigned>(p->n);
   const int32_t* row = p->row.as<int32_t>();
-  range_expand_kernel<Op><<<grid_of(count), kBlock, 0, p->stream>>>(row, begin, count, hub_degree(p->m),
+  range_expand_kernel<Op><<<grid_of(count), kBlock, 0, p->stream>>>(row, begin, count, hub_degree(),
                                                                      p->hubq.as<uint32_t>(), c, cap, op);
   GX_LAUNCHED("range_expand_kernel");
   launch_hub_offsets<kBlock>(row, p->hubq.as<uint32_t>(), &c->hub_count, cap, p->hub_off.as<uint32_t>(), p->stream);

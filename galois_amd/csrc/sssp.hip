@@ -365,7 +365,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.shift = shift;
   B.cap = cap;
   B.hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(cap));
-  B.hub = hub_degree(g->m);
+  B.hub = hub_degree();
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
   SsspState* host = static_cast<SsspState*>(g->host_pinned(sizeof(SsspState)));
 
