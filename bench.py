@@ -21,7 +21,9 @@ Also reported, one line of JSON in total:
                 same graph);
   kernels       GTEPS and %HBM of every kernel on the other workloads (BFS on
                 symmetric RMAT-18; SSSP/CC/k-core on symmetric RMAT-22 and the
-                2048x2048 road-like grid; batched vs one-source-per-pass BC;
+                2048x2048 road-like grid; batched vs one-source-per-pass BC,
+                on RMAT-22 and on 64 uneven graphs with 4 and with 64
+                sources per graph;
                 adjoint vs forward PageRank; the D-IrGL personality loops;
                 BFS/SSSP/CC/k-core on symmetric RMAT-26), plus the
                 FP32-vs-FP64 PageRank error.
@@ -209,7 +211,32 @@ def kernel_table(gx, quick: bool) -> dict:
         "plain_ms_per_source": round(per_src_plain, 3),
         "speedup_batched_vs_plain": round(per_src_plain * len(srcs) / stb.elapsed_ms, 2),
         "max_rel_err_vs_oracle_8_sources": float(np.max(np.abs(got - exp) / np.maximum(np.abs(exp), 1.0)))}
-    del U, bat, pl, sub
+    # The same 64 graphs with 64 seeded sources each (4096 sources): every
+    # 64-source batch then lies in one graph, so all its lanes work at every
+    # vertex it reaches (with 4 sources per graph a batch spans 16 graphs and
+    # at most 4 of its 64 lanes are live at a vertex).  One source per pass is
+    # timed on 16 of them; 8 are checked against the oracle.
+    srcs64 = []
+    for i in range(ngraphs):
+        lo, hi = int(offs[i]), int(offs[i + 1])
+        srcs64.extend((gx.pick_sources(urow[lo:hi + 1] - urow[lo], 64, seed=SEED + 7000 + i) + lo).tolist())
+    srcs64 = np.array(srcs64, np.uint32)
+    gx.betweenness_centrality_batched(U, srcs64[:64])
+    bat64, stb64 = gx.betweenness_centrality_batched(U, srcs64)
+    pl64, stp64 = gx.betweenness_centrality(U, srcs64[::len(srcs64) // 16][:16])
+    per_src_plain64 = stp64.elapsed_ms / 16
+    sub64, _ = gx.betweenness_centrality_batched(U, srcs64[:8])
+    exp64 = np.empty(U.n, np.float64)
+    assert lib.oracle_bc(U.n, urow.ctypes.data, ucol.ctypes.data, srcs64[:8].ctypes.data, 8, exp64.ctypes.data,
+                         int(os.environ.get("OMP_NUM_THREADS", "8"))) == 0
+    got64 = sub64.cpu().numpy()
+    out["bc_batched_uneven64_64_per_graph"] = {
+        "graphs": ngraphs, "sources": int(len(srcs64)), "ms": round(stb64.elapsed_ms, 3),
+        "TEPS_G": round(len(srcs64) * U.m / ngraphs / stb64.elapsed_ms / 1e6, 3),
+        "plain_ms_per_source": round(per_src_plain64, 3),
+        "speedup_batched_vs_plain": round(per_src_plain64 * len(srcs64) / stb64.elapsed_ms, 2),
+        "max_rel_err_vs_oracle_8_sources": float(np.max(np.abs(got64 - exp64) / np.maximum(np.abs(exp64), 1.0)))}
+    del U, bat, pl, sub, bat64, pl64, sub64
     # Config 5: differentiable PageRank on directed weighted RMAT-22 (FP32
     # lognormal weights, row-normalised), loss sum(c * x), c ~ N(0,1).
     scale5 = 16 if quick else 22
