@@ -270,6 +270,76 @@ __device__ __forceinline__ uint64_t group_mask(const BatchDims& D, int g) {
   return g == D.groups - 1 ? D.last_mask : ~0ull;
 }
 
+// Vertices a 64-source group can reach: [vlo, vhi), the span of its
+// sources' connected components (the whole graph when components are not
+// known), and the chunks of those vertices [clo, clo + (next istart -
+// istart)).  istart is the group's first work item in the group-major item
+// order; entry `groups` holds the total.
+struct GroupRange {
+  int64_t vlo, vhi, clo, istart;
+};
+
+// Group ranges staged in LDS by every workgroup of a sweep.
+struct GroupLds {
+  int64_t istart[65], clo[65];
+};
+
+__device__ __forceinline__ void load_ranges(const GroupRange* __restrict__ R, int groups, GroupLds& L) {
+  for (int i = threadIdx.x; i <= groups; i += blockDim.x) {
+    L.istart[i] = R[i].istart;
+    L.clo[i] = R[i].clo;
+  }
+  __syncthreads();
+}
+
+// Group of work item `it`: the last g with istart[g] <= it.
+__device__ __forceinline__ int item_group(const GroupLds& L, int groups, int64_t it) {
+  int lo = 0, hi = groups - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (L.istart[mid] <= it) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+// One thread: the ranges of every group from its sources.  comp_lo/comp_hi:
+// per vertex the smallest id of its component and per component label one
+// past its largest id (null: whole graph).  first: first chunk of every
+// vertex (n + 1 entries).
+__global__ void bcb_ranges_kernel(const uint32_t* __restrict__ src, int count, int groups, int64_t n,
+                                  const uint32_t* __restrict__ comp_lo, const uint32_t* __restrict__ comp_hi,
+                                  const uint32_t* __restrict__ first, GroupRange* __restrict__ R) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t acc = 0;
+  for (int g = 0; g < groups; ++g) {
+    int64_t lo = 0, hi = n;
+    if (comp_lo) {
+      lo = n;
+      hi = 0;
+      const int e = min(count, (g + 1) * 64);
+      for (int i = g * 64; i < e; ++i) {
+        const uint32_t l = comp_lo[src[i]];
+        lo = min(lo, static_cast<int64_t>(l));
+        hi = max(hi, static_cast<int64_t>(comp_hi[l]));
+      }
+    }
+    R[g].vlo = lo;
+    R[g].vhi = hi;
+    R[g].clo = first[lo];
+    R[g].istart = acc;
+    acc += static_cast<int64_t>(first[hi]) - first[lo];
+  }
+  R[groups].vlo = R[groups].vhi = R[groups].clo = 0;
+  R[groups].istart = acc;
+}
+
+// comp_hi[label] = 1 + largest vertex with that component label.
+__global__ void bcb_comp_hi_kernel(const uint32_t* __restrict__ label, int64_t n, uint32_t* __restrict__ comp_hi) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    atomicMax(&comp_hi[label[v]], static_cast<uint32_t>(v + 1));
+}
+
 // Level-0 masks, visited masks and path counts of the sources.
 __global__ void bcb_seed_kernel(const uint32_t* __restrict__ src, int count, BatchDims D, uint64_t* __restrict__ F0,
                                 uint64_t* __restrict__ vis, float* __restrict__ sigma) {
@@ -286,15 +356,17 @@ __global__ void bcb_seed_kernel(const uint32_t* __restrict__ src, int count, Bat
 __global__ void __launch_bounds__(kBlock) bcb_forward_kernel(
     const int32_t* __restrict__ trow, const int32_t* __restrict__ tcol, ChunkTable T, BatchDims D,
     const uint64_t* __restrict__ FL, uint64_t* __restrict__ Fn, const uint64_t* __restrict__ vis,
-    float* __restrict__ sigma, unsigned long long* __restrict__ newcount) {
+    float* __restrict__ sigma, unsigned long long* __restrict__ newcount, const GroupRange* __restrict__ R) {
   const unsigned lane = lane_id();
   const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
-  const int64_t items = T.count * D.groups;
+  __shared__ GroupLds GL;
+  load_ranges(R, D.groups, GL);
+  const int64_t items = GL.istart[D.groups];
   unsigned long long found = 0;
   for (int64_t it = wave; it < items; it += nwaves) {
-    const int g = static_cast<int>(it / T.count);
-    const int64_t c = it - static_cast<int64_t>(g) * T.count;
+    const int g = item_group(GL, D.groups, it);
+    const int64_t c = GL.clo[g] + (it - GL.istart[g]);
     const uint32_t v = T.vtx[c];
     const int64_t goff = static_cast<int64_t>(g) * D.n;
     const uint64_t need = group_mask(D, g) & ~vis[goff + v];
@@ -360,14 +432,16 @@ __global__ void __launch_bounds__(kBlock) bcb_visit_kernel(const uint64_t* __res
 __global__ void __launch_bounds__(kBlock) bcb_backward_kernel(
     const int32_t* __restrict__ row, const int32_t* __restrict__ col, ChunkTable T, BatchDims D,
     const uint64_t* __restrict__ FL, const uint64_t* __restrict__ FL1, const float* __restrict__ sigma,
-    double* __restrict__ delta, double* __restrict__ bcpart) {
+    double* __restrict__ delta, double* __restrict__ bcpart, const GroupRange* __restrict__ R) {
   const unsigned lane = lane_id();
   const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
-  const int64_t items = T.count * D.groups;
+  __shared__ GroupLds GL;
+  load_ranges(R, D.groups, GL);
+  const int64_t items = GL.istart[D.groups];
   for (int64_t it = wave; it < items; it += nwaves) {
-    const int g = static_cast<int>(it / T.count);
-    const int64_t c = it - static_cast<int64_t>(g) * T.count;
+    const int g = item_group(GL, D.groups, it);
+    const int64_t c = GL.clo[g] + (it - GL.istart[g]);
     const uint32_t v = T.vtx[c];
     const int64_t goff = static_cast<int64_t>(g) * D.n;
     const uint64_t mine = FL[goff + v];
@@ -417,13 +491,15 @@ __global__ void __launch_bounds__(kBlock) bcb_multi_finish_kernel(const uint32_t
                                                                   const uint64_t* __restrict__ FL,
                                                                   const float* __restrict__ sigma,
                                                                   double* __restrict__ delta,
-                                                                  double* __restrict__ bcpart) {
+                                                                  double* __restrict__ bcpart,
+                                                                  const GroupRange* __restrict__ R) {
   const unsigned lane = lane_id();
   const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
   for (int64_t it = wave; it < nmulti * D.groups; it += nwaves) {
     const int g = static_cast<int>(it / nmulti);
     const uint32_t v = multi[it - static_cast<int64_t>(g) * nmulti];
+    if (v < R[g].vlo || v >= R[g].vhi) continue;
     const int64_t goff = static_cast<int64_t>(g) * D.n;
     const uint64_t mine = FL[goff + v];
     if (mine == 0) continue;
@@ -457,14 +533,15 @@ void fill_async(T* p, T v, int64_t n, hipStream_t s) {
 
 // Chunk table of a CSR, cached in `buf`.
 struct CachedChunks {
-  DevBuf buf;
+  DevBuf buf, first;  // first: first chunk of every vertex (n + 1)
   ChunkTable t{};
   bool ready = false;
 };
 
 void build_chunks(const int32_t* row, int64_t n, int64_t m, hipStream_t s, gx_graph* g, CachedChunks& cc) {
   if (cc.ready) return;
-  DevBuf cnt, first, flag, pos;
+  DevBuf cnt, flag, pos;
+  DevBuf& first = cc.first;
   cnt.ensure((n + 1) * 4);
   first.ensure((n + 1) * 4);
   flag.ensure((n + 1) * 4);
@@ -507,6 +584,10 @@ void build_chunks(const int32_t* row, int64_t n, int64_t m, hipStream_t s, gx_gr
 struct BcCache {
   CachedChunks out, in;
   std::vector<DevBuf> levels;  // frontier masks per level
+  // Symmetric graphs: component label (smallest vertex id) per vertex and,
+  // per label, one past the component's largest vertex id.
+  DevBuf comp_lo, comp_hi;
+  bool comps = false;
 };
 
 }  // namespace
@@ -656,6 +737,23 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
     tin = cache.in.t;
   }
   const ChunkTable tout = cache.out.t;
+  const CachedChunks& cin = (g->flags & GX_GRAPH_SYMMETRIC) ? cache.out : cache.in;
+  // Component bounds (symmetric graphs): a group only sweeps the span of its
+  // sources' components, which matters when the graph is a union of many
+  // (a batch of uneven graphs); one giant component leaves the whole graph.
+  if ((g->flags & GX_GRAPH_SYMMETRIC) && !cache.comps && n > 0) {
+    cache.comp_lo.ensure(n * 4);
+    cache.comp_hi.ensure(n * 4);
+    if (gx_cc(g, cache.comp_lo.as<uint32_t>(), nullptr) != GX_OK) fail(GX_ERR_HIP, "BC: component labels failed");
+    fill_async<uint32_t>(cache.comp_hi.as<uint32_t>(), 0u, n, s);
+    bcb_comp_hi_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(cache.comp_lo.as<uint32_t>(), n,
+                                                                     cache.comp_hi.as<uint32_t>());
+    GX_LAUNCHED("bcb_comp_hi_kernel");
+    GX_HIP(hipStreamSynchronize(s));
+    cache.comps = true;
+  }
+  const uint32_t* comp_lo = cache.comps ? cache.comp_lo.as<uint32_t>() : nullptr;
+  const uint32_t* comp_hi = cache.comps ? cache.comp_hi.as<uint32_t>() : nullptr;
   // Groups per launch bounded by memory: sigma 256 B + delta 512 B + two
   // masks per vertex per group (plus the level masks).
   const int64_t per_group = n * (64 * 4 + 64 * 8 + 8 + 8);
@@ -672,6 +770,11 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
   uint32_t* dsrc = g->scratch<uint32_t>(WS_Q3, std::max<uint32_t>(count, 1));
   if (count > 0) GX_HIP(hipMemcpyAsync(dsrc, sources, count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
   const unsigned wgrid = 256 * 32;  // workgroups of the persistent-style sweeps
+  // Group ranges over the in-edge chunks (forward) and the out-edge chunks
+  // (backward); 65 entries each.
+  GroupRange* Rin = g->scratch<GroupRange>(WS_V5, 2 * 65);
+  GroupRange* Rout = Rin + 65;
+  std::vector<GroupRange> hr(65);
   for (uint32_t first = 0; first < count; first += 64u * max_groups) {
     const uint32_t cnt = std::min<uint32_t>(count - first, 64u * max_groups);
     BatchDims D;
@@ -684,11 +787,26 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
     double* delta = g->scratch<double>(WS_BC_DELTA, gn * 64);
     uint64_t* vis = g->scratch<uint64_t>(WS_BC_MASK0, gn);
     double* bcpart = g->scratch<double>(WS_BC_MASK1, gn);
-    fill_async<float>(sigma, 0.f, gn * 64, s);
-    fill_async<double>(delta, 0.0, gn * 64, s);
+    bcb_ranges_kernel<<<1, 64, 0, s>>>(dsrc + first, static_cast<int>(cnt), D.groups, n, comp_lo, comp_hi,
+                                         cin.first.as<uint32_t>(), Rin);
+    bcb_ranges_kernel<<<1, 64, 0, s>>>(dsrc + first, static_cast<int>(cnt), D.groups, n, comp_lo, comp_hi,
+                                         cache.out.first.as<uint32_t>(), Rout);
+    GX_LAUNCHED("bcb_ranges_kernel");
+    GX_HIP(hipMemcpyAsync(hr.data(), Rout, (D.groups + 1) * sizeof(GroupRange), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    launches += 2;
+    // Path counts and dependencies are read only at reached vertices, so only
+    // each group's span is cleared.
+    for (int gi = 0; gi < D.groups; ++gi) {
+      GX_REQUIRE(hr[gi].vlo >= 0 && hr[gi].vlo <= hr[gi].vhi && hr[gi].vhi <= n, "BC: bad group range");
+      const int64_t off = (static_cast<int64_t>(gi) * n + hr[gi].vlo) * 64;
+      fill_async<float>(sigma + off, 0.f, (hr[gi].vhi - hr[gi].vlo) * 64, s);
+      fill_async<double>(delta + off, 0.0, (hr[gi].vhi - hr[gi].vlo) * 64, s);
+      launches += 2;
+    }
     fill_async<uint64_t>(vis, 0ull, gn, s);
     fill_async<double>(bcpart, 0.0, gn, s);
-    launches += 4;
+    launches += 2;
     auto level_buf = [&](size_t L) -> uint64_t* {
       while (cache.levels.size() <= L) cache.levels.emplace_back();
       cache.levels[L].ensure(gn * sizeof(uint64_t));
@@ -708,7 +826,7 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
       FL = cache.levels[L].as<uint64_t>();
       fill_async<uint64_t>(Fn, 0ull, gn, s);
       fill_async<unsigned long long>(ctr, 0ull, 2, s);
-      bcb_forward_kernel<<<wgrid, kBlock, 0, s>>>(trow, tcol, tin, D, FL, Fn, vis, sigma, ctr);
+      bcb_forward_kernel<<<wgrid, kBlock, 0, s>>>(trow, tcol, tin, D, FL, Fn, vis, sigma, ctr, Rin);
       GX_LAUNCHED("bcb_forward_kernel");
       bcb_visit_kernel<<<grid_for(gn, kBlock, 8192), kBlock, 0, s>>>(Fn, vis, gn, ctr + 1);
       GX_LAUNCHED("bcb_visit_kernel");
@@ -733,10 +851,11 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
       for (int64_t lv = static_cast<int64_t>(deepest) - 1; lv >= 1; --lv) {
         const uint64_t* FL = cache.levels[lv].as<uint64_t>();
         const uint64_t* FL1 = cache.levels[lv + 1].as<uint64_t>();
-        bcb_backward_kernel<<<wgrid, kBlock, 0, cs>>>(g->row, g->col, tout, D, FL, FL1, sigma, delta, bcpart);
+        bcb_backward_kernel<<<wgrid, kBlock, 0, cs>>>(g->row, g->col, tout, D, FL, FL1, sigma, delta, bcpart,
+                                                       Rout);
         if (tout.nmulti > 0)
           bcb_multi_finish_kernel<<<grid_for(tout.nmulti * D.groups * 64, kBlock, 8192), kBlock, 0, cs>>>(
-              tout.multi, tout.nmulti, D, FL, sigma, delta, bcpart);
+              tout.multi, tout.nmulti, D, FL, sigma, delta, bcpart, Rout);
       }
       hipError_t cap_err = hipStreamEndCapture(cs, &graph);
       GX_HIP(cap_err);
