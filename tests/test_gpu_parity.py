@@ -633,6 +633,40 @@ def test_batched_bc_over_union_of_uneven_graphs(gx, oracle):
     _bc_close(bc.cpu().numpy(), exp)
 
 
+def test_batched_bc_scattered_components(gx, oracle):
+    """Batched BC sweeps each 64-source group over the id span of its sources'
+    components.  Here the union's vertex ids are shuffled, so every span also
+    holds vertices of other components (and groups mix graphs): results must
+    still equal the oracle's, and a second call (reused level buffers) too."""
+    say("batched bc, scattered components")
+    rng = np.random.default_rng(5)
+    parts = [gx.Graph.rmat(int(rng.integers(7, 11)), int(rng.integers(4, 13)), seed=300 + i, symmetric=True)
+             for i in range(10)]
+    U, _ = gx.Graph.union(parts)
+    del parts
+    row, col, _ = U.to_host()
+    n = len(row) - 1
+    perm = rng.permutation(n)  # old id -> new id
+    src_old = np.repeat(np.arange(n), np.diff(row))
+    s_new, d_new = perm[src_old], perm[col]
+    order = np.lexsort((d_new, s_new))
+    s_new, d_new = s_new[order], d_new[order]
+    prow = np.zeros(n + 1, np.int64)
+    np.add.at(prow, s_new + 1, 1)
+    prow = np.cumsum(prow).astype(np.int32)
+    pcol = d_new.astype(np.int32)
+    G = gx.Graph.from_csr(prow, pcol, None, symmetric=True)
+    srcs = gx.pick_sources(prow, 150, seed=11)
+    exp = oracle.bc(prow, pcol, srcs)
+    for _ in range(2):
+        bc, _ = gx.betweenness_centrality_batched(G, srcs)
+        _bc_close(bc.cpu().numpy(), exp)
+    # a group whose sources sit in one component, after groups that did not
+    one = srcs[np.argsort(prow[srcs + 1] - prow[srcs])][:3]
+    bc1, _ = gx.betweenness_centrality_batched(G, one)
+    _bc_close(bc1.cpu().numpy(), oracle.bc(prow, pcol, one))
+
+
 def test_autotuner_picks_and_caches(gx, tmp_path):
     from galois_amd import autotune
     say("autotune rmat14")
