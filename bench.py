@@ -415,6 +415,10 @@ def main() -> int:
                     help="set a tuning knob before running (gx_tune_set), e.g. pr.hot_sources=0")
     ap.add_argument("--autotune", action="store_true",
                     help="run galois_amd.autotune on the headline graph before the timed steps")
+    ap.add_argument("--frontier-pass", action="store_true",
+                    help="workload for rocprofv3 kernel traces of the frontier kernels: k-core (k=10), SSSP "
+                         "(delta 64) and top-down BFS on symmetric weighted RMAT at --scale, each twice; prints no "
+                         "JSON")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="workload for rocprofv3 --pmc runs: 3 pull sweeps on the headline graph, then a 1 GiB "
                          "device copy (the calibration reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
@@ -432,6 +436,18 @@ def main() -> int:
     for kv in args.tune:
         key, val = kv.split("=", 1)
         gx.tune_set(key, int(val))
+
+    if args.frontier_pass:
+        G = gx.Graph.rmat(args.scale, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
+        row, _, _ = G.to_host()
+        src = int(np.argmax(np.diff(row)))
+        del row
+        for name, fn in (("kcore", lambda: gx.kcore(G, 10)), ("sssp", lambda: gx.sssp(G, 0, 6)),
+                         ("bfs", lambda: gx.bfs(G, src))):
+            fn()
+            st = fn()[-1]
+            log(f"frontier pass: {name} {st.elapsed_ms:.2f} ms, {st.iterations} rounds")
+        return 0
 
     log(f"generating directed RMAT-{args.scale}")
     G = gx.Graph.rmat(args.scale, 16, symmetric=False, seed=SEED)
