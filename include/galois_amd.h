@@ -304,7 +304,11 @@ gx_status gx_pagerank_adjoint(gx_graph* g, const float* weights, const void* x, 
 gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count,
                 double* bc, gx_stats* st);
 /* Same result, all sources batched: 64 sources per wavefront (one lane per
- * source), level loops captured in a HIP graph.  sources: HOST array. */
+ * source), level loops captured in a HIP graph.  On a symmetric graph each
+ * group of 64 sources sweeps only the vertex span of its sources' connected
+ * components (labels computed on the first call and kept with the graph), so
+ * a union of many graphs (gx_graph_union) costs each group its own graphs.
+ * sources: HOST array. */
 gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_t count,
                         double* bc, gx_stats* st);
 
