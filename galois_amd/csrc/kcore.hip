@@ -32,6 +32,11 @@ struct KcoreCounters {
   unsigned count[2];
   unsigned hub_count;
   unsigned overflow;
+  unsigned in;      // dead frontier read this round (q[in]); the next is q[in ^ 1]
+  unsigned active;  // this round has a frontier
+  unsigned done;
+  unsigned rounds;
+  unsigned long long items;
   unsigned long long alive;
   unsigned long long edges;
 };
@@ -41,12 +46,19 @@ __global__ void kcore_reset_kernel(KcoreCounters* c) {
     c->count[0] = c->count[1] = 0;
     c->hub_count = 0;
     c->overflow = 0;
+    c->in = 0;
+    c->active = 0;
+    c->done = 0;
+    c->rounds = 0;
+    c->items = 0;
     c->alive = 0;
     c->edges = 0;
   }
 }
 
 constexpr int kQCap = 2048;
+constexpr int kRoundsPerGraph = 4;
+constexpr int kRoundsFirst = 2;  // first replay: many graphs need one round (plus the empty one)
 
 // degree[v] = out-degree; vertices below k form the first dead frontier
 // (staged in LDS, one global append per workgroup flush).
@@ -71,6 +83,8 @@ __global__ void __launch_bounds__(kBlock) kcore_init_kernel(const int32_t* __res
   lq_flush(sq, q, &c->count[0], cap, &c->overflow, true);
 }
 
+// Plain pointers, the output queue picked from the device state in the
+// kernel (the kernel arguments are never modified or copied).
 struct PeelOp {
   const int32_t* col;
   uint32_t* degree;
@@ -78,7 +92,7 @@ struct PeelOp {
   KcoreCounters* c;
   LdsQueue<kQCap>* sq;
   uint32_t k;
-  int out;
+  unsigned out;
   unsigned cap;
   uint32_t hub;  // frontier.hub_degree
   __device__ __forceinline__ void operator()(bool valid, uint32_t, int32_t e) {
@@ -96,33 +110,81 @@ struct PeelOp {
   __device__ __forceinline__ void after_step() { flush(false); }
 };
 
-__global__ void __launch_bounds__(kBlock) kcore_peel_kernel(const int32_t* __restrict__ row, PeelOp op,
-                                                            const uint32_t* __restrict__ qin, int in,
-                                                            uint32_t* __restrict__ hubq) {
+struct KcoreBufs {
+  const int32_t* row;
+  const int32_t* col;
+  uint32_t* degree;
+  uint32_t* q0;
+  uint32_t* q1;
+  uint32_t* hubq;
+  uint32_t* hub_off;
+  KcoreCounters* c;
+  uint32_t k;
+  unsigned cap;
+  uint32_t hub;
+};
+
+__device__ __forceinline__ PeelOp make_peel(const KcoreBufs& B, LdsQueue<kQCap>* sq) {
+  const unsigned out = B.c->in ^ 1u;
+  return PeelOp{B.col, B.degree, out ? B.q1 : B.q0, B.c, sq, B.k, out, B.cap, B.hub};
+}
+
+__global__ void __launch_bounds__(kBlock) kcore_peel_kernel(KcoreBufs B) {
+  KcoreCounters* c = B.c;
+  if (!c->active) return;
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
-  op.sq = &sq;
-  const unsigned cnt = op.c->count[in];
-  expand_queue<kBlock>(row, qin, cnt, op.hub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  PeelOp op = make_peel(B, &sq);
+  const unsigned in = c->in;
+  expand_queue<kBlock>(B.row, in ? B.q1 : B.q0, c->count[in], B.hub, B.hubq, &c->hub_count, B.cap, &c->overflow,
+                       op);
   op.flush(true);
 }
 
-__global__ void __launch_bounds__(kBlock) kcore_hubs_kernel(const int32_t* __restrict__ row, PeelOp op,
-                                                            const uint32_t* __restrict__ hubq,
-                                                            const uint32_t* __restrict__ hub_off) {
-  if (op.c->hub_count == 0) return;
+__global__ void __launch_bounds__(kBlock) kcore_hubs_kernel(KcoreBufs B) {
+  KcoreCounters* c = B.c;
+  if (!c->active || c->hub_count == 0) return;
   __shared__ LdsQueue<kQCap> sq;
   lq_init(sq);
-  op.sq = &sq;
-  expand_hubs<kBlock>(row, hubq, op.c->hub_count, op.cap, hub_off, op);
+  PeelOp op = make_peel(B, &sq);
+  expand_hubs<kBlock>(B.row, B.hubq, c->hub_count, B.cap, B.hub_off, op);
   op.flush(true);
 }
 
-__global__ void kcore_round_begin_kernel(KcoreCounters* c, int out) {
-  if (threadIdx.x == 0) {
-    c->count[out] = 0;
-    c->hub_count = 0;
+// Round bracket, driven by the device state: a round runs while the dead
+// frontier is nonempty; the end swaps the frontiers.
+__device__ __forceinline__ void round_begin(KcoreCounters* c) {
+  if (c->done || c->overflow || c->count[c->in] == 0) {
+    c->active = 0;
+    c->done = 1;
+    return;
   }
+  c->active = 1;
+  c->count[c->in ^ 1] = 0;
+  c->hub_count = 0;
+  c->items += c->count[c->in];
+  c->rounds += 1;
+}
+
+__device__ __forceinline__ void round_end(KcoreCounters* c) {
+  if (!c->active) return;
+  c->in ^= 1;
+  c->active = 0;
+}
+
+__global__ void kcore_round_begin_kernel(KcoreCounters* c) {
+  if (threadIdx.x == 0) round_begin(c);
+}
+
+__global__ void kcore_round_end_kernel(KcoreCounters* c) {
+  if (threadIdx.x == 0) round_end(c);
+}
+
+// End of one round and begin of the next in one launch.
+__global__ void kcore_round_turn_kernel(KcoreCounters* c) {
+  if (threadIdx.x != 0) return;
+  round_end(c);
+  round_begin(c);
 }
 
 __global__ void __launch_bounds__(kBlock) kcore_finish_kernel(const uint32_t* __restrict__ degree,
@@ -158,36 +220,54 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   KcoreCounters* host = static_cast<KcoreCounters*>(g->host_pinned(sizeof(KcoreCounters)));
   RegionTimer region(s);
   KernelTimer ktimer(s);
-  uint32_t launches = 2, rounds = 0;
-  uint64_t edges = 0, items = 0;
+  uint32_t launches = 2;
   kcore_reset_kernel<<<1, 64, 0, s>>>(c);
   kcore_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(g->row, degree, n, k, q[0], c, cap);
   GX_LAUNCHED("kcore_init_kernel");
-  GX_HIP(hipMemcpyAsync(host, c, sizeof(KcoreCounters), hipMemcpyDeviceToHost, s));
-  GX_HIP(hipStreamSynchronize(s));
-  unsigned count = host->count[0];
-  int in = 0;
-  while (count > 0) {
-    items += count;
-    const int out = in ^ 1;
-    kcore_round_begin_kernel<<<1, 64, 0, s>>>(c, out);
-    PeelOp op{g->col, degree, q[out], c, nullptr, k, out, cap, hub_degree()};
+  // Rounds are driven on the device: rounds (begin, peel, hub layout, hubs,
+  // end) captured in HIP graphs, kRoundsFirst for the first replay and
+  // kRoundsPerGraph for the next ones, replayed until the state says done,
+  // so the host synchronises once per replay.
+  KcoreBufs B{g->row, g->col, degree, q[0], q[1], hubq, hub_off, c, k, cap, hub_degree()};
+  // No vertex above the hub threshold: the hub list stays empty and its two
+  // launches are left out of each round.
+  const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
+  uint64_t key = 0;
+  for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.degree, (const void*)B.q0,
+                        (const void*)B.q1, (const void*)B.hubq, (const void*)B.hub_off, (const void*)B.c})
+    key = mix_key(key, reinterpret_cast<uintptr_t>(p));
+  key = mix_key(mix_key(mix_key(mix_key(key, k), cap), B.hub), hubs ? 1u : 0u);
+  uint32_t replays = 0, slots = 0;
+  for (;;) {
+    const int nr = replays == 0 ? kRoundsFirst : kRoundsPerGraph;
     ktimer.begin();
-    kcore_peel_kernel<<<std::min<unsigned>(256 * 8, grid_for(count, kBlock)), kBlock, 0, s>>>(g->row, op, q[in], in,
-                                                                                              hubq);
-    GX_LAUNCHED("kcore_peel_kernel");
-    launch_hub_offsets<kBlock>(g->row, hubq, &c->hub_count, cap, hub_off, s);
-    kcore_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
+    g->replay(replays == 0 ? g->kcore_head : g->kcore_batch, key, [&](hipStream_t cs) {
+      kcore_round_begin_kernel<<<1, 64, 0, cs>>>(c);
+      for (int i = 0; i < nr; ++i) {
+        kcore_peel_kernel<<<256 * 8, kBlock, 0, cs>>>(B);
+        if (hubs) {
+          launch_hub_offsets<kBlock>(B.row, B.hubq, &c->hub_count, cap, B.hub_off, cs);
+          kcore_hubs_kernel<<<256 * 8, kBlock, 0, cs>>>(B);
+        }
+        if (i + 1 < nr)
+          kcore_round_turn_kernel<<<1, 64, 0, cs>>>(c);
+        else
+          kcore_round_end_kernel<<<1, 64, 0, cs>>>(c);
+      }
+      GX_LAUNCHED("kcore rounds");
+    });
     ktimer.end();
-    GX_LAUNCHED("kcore_hubs_kernel");
-    launches += 3;
-    ++rounds;
+    ++replays;
+    slots += static_cast<uint32_t>(nr);
     GX_HIP(hipMemcpyAsync(host, c, sizeof(KcoreCounters), hipMemcpyDeviceToHost, s));
     GX_HIP(hipStreamSynchronize(s));
     if (host->overflow) fail(GX_ERR_CAPACITY, "k-core worklist overflow");
-    count = host->count[out];
-    in = out;
+    if (host->done) break;
   }
+  launches += slots * (hubs ? 4 : 2) + replays;
+  const uint32_t rounds = host->rounds;
+  const uint64_t items = host->items;
+  uint64_t edges = 0;
   kcore_finish_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(degree, alive, n, k, c);
   GX_LAUNCHED("kcore_finish_kernel");
   launches += 1;
