@@ -22,9 +22,9 @@
 //    reduced once per workgroup.
 //  - gx_bfs drives its levels on the device: each level is a fixed kernel
 //    sequence (begin, expand, hubs, end) that reads the frontier size from a
-//    device state word; 8 levels are captured in one HIP graph (the end of a
-//    level and the begin of the next in one launch) and replayed until the
-//    frontier is empty.
+//    device state word; levels are captured in HIP graphs (7 for the first
+//    replay, 8 for the next; the end of a level and the begin of the next in
+//    one launch) and replayed until the frontier is empty.
 //  - Bottom-up steps test a 1-bit-per-vertex frontier bitmap (n/8 bytes, L2
 //    resident for the graphs here) and stop at the first hit.
 #include <algorithm>
@@ -48,6 +48,7 @@ constexpr int kGrid = 256 * 8;
 // 21.7 ms, BFS 2.03 -> 1.57 ms), while RMAT frontiers need the full grid.
 constexpr int kGridNoHubs = 512;
 constexpr int kLevelsPerGraph = 8;
+constexpr int kLevelsFirst = 7;  // first replay: power-law graphs need 5-6 levels plus the empty one
 
 struct BfsState {
   unsigned count[2];      // frontier sizes (double buffered by q)
@@ -380,14 +381,16 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
                         (const void*)B.vis})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
   key = mix_key(mix_key(mix_key(key, cap), B.hub), hubs ? 1u : 0u);
-  uint32_t replays = 0;
+  uint32_t replays = 0, slots = 0;
   for (;;) {
+    const int nl = replays == 0 ? kLevelsFirst : kLevelsPerGraph;
     ktimer.begin();
-    g->replay(g->bfs_batch, key, [&](hipStream_t cs) {
-      enqueue_levels(B, hubs, kLevelsPerGraph, cs);
+    g->replay(replays == 0 ? g->bfs_head : g->bfs_batch, key, [&](hipStream_t cs) {
+      enqueue_levels(B, hubs, nl, cs);
     });
     ktimer.end();
     ++replays;
+    slots += static_cast<uint32_t>(nl);
     GX_HIP(hipMemcpyAsync(host, B.S, sizeof(BfsState), hipMemcpyDeviceToHost, s));
     GX_HIP(hipStreamSynchronize(s));
     if (host->overflow) fail(GX_ERR_CAPACITY, "BFS frontier queue overflow");
@@ -397,7 +400,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = host->level - 1;  // levels with a non-empty frontier beyond the source
-    st->launches = (vis ? 3 : 2) + replays * (kLevelsPerGraph * (hubs ? 4 : 2) + 1);
+    st->launches = (vis ? 3 : 2) + slots * (hubs ? 4 : 2) + replays;
     st->edges_visited = host->edges + static_cast<uint64_t>(0);
     st->items = host->items;
     st->elapsed_ms = ms;

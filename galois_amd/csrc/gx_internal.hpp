@@ -188,7 +188,7 @@ struct gx_graph {
   std::shared_ptr<void> bc_cache;  // batched-BC tables (bc.hip)
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // non-null stream for HIP graph capture
-  gx::CapturedBatch sssp_batch, bfs_batch, kcore_batch, kcore_head;
+  gx::CapturedBatch sssp_batch, bfs_batch, bfs_head, kcore_batch, kcore_head;
   gx::DevBuf ws[gx::WS_NUM_SLOTS];
   void* pinned = nullptr;  // small pinned host block for counters
   size_t pinned_bytes = 0;
