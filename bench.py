@@ -10,12 +10,13 @@ One step = one full PageRank solve; value = GTEPS = edges x iterations / time.
 
 Also reported, one line of JSON in total:
   roofline      the dominant kernel (one sweep: phase 1 + phase 2 + sliced-bin
-                finalisation) against the 8 TB/s HBM peak, with the
-                algorithmic bytes per sweep the library reports for the sweep
-                it ran (blocked: 2m + 10g + 20n + 12 per job, g = (source,
-                destination bin) groups), timed with HIP events on the
-                library's stream during the timed steps; measured HBM traffic
-                from the committed rocprofv3 summary;
+                finalisation) against the 8 TB/s HBM peak, with the pull
+                iteration's algorithmic bytes 4m + 16n + 4 per sweep, timed
+                with HIP events on the library's stream during the timed
+                steps; beside them the bytes of the sweep actually run as the
+                library states them (blocked: 2m + 10g + 20n + 12 per job,
+                g = (source, destination bin) groups) and measured HBM
+                traffic from the committed rocprofv3 summary;
   cpu_baseline  the CPU restatement (oracle, FP64 pull) timed on this host's
                 cores on a bounded sample (a few PageRank iterations on the
                 same graph);
@@ -489,7 +490,7 @@ def main() -> int:
         iters += st.iterations
         sweeps += st.main_kernel_launches
         sweep_ms_total += st.main_kernel_ms
-        algo_bytes = st.main_kernel_bytes
+        design_bytes = st.main_kernel_bytes
     sync()
     dt = time.perf_counter() - t0
     gx.set_profiling(False)
@@ -498,14 +499,18 @@ def main() -> int:
     log(f"pagerank fp32: {ms_per_step:.1f} ms/step, {iters / args.steps:.0f} iterations, {value:.2f} GTEPS")
 
     # Roofline of the dominant kernel: one FP32 sweep, bracketed by HIP events
-    # per iteration on the graph's stream, against the algorithmic bytes the
-    # library states for the sweep it ran (gx_stats.main_kernel_bytes):
+    # per iteration on the graph's stream.  Algorithmic bytes per sweep are
+    # the pull iteration's minimum for element size 4: 4m source ids + 4(n+1)
+    # row offsets + 4n ranks gathered (each once) + 4n ranks written + 4n
+    # out-degrees = 4m + 16n + 4.  The bytes of the sweep actually run, as the
+    # library states them (gx_stats.main_kernel_bytes), are reported beside:
     #  blocked (default): phase 1 reads a 2-byte source id per group and y
     #    (4n), writes one 4-byte contribution per group; phase 2 reads a
     #    2-byte destination id per edge, the contributions, its job table
     #    (12 bytes per job), x_old and out-degrees (8n), writes x' and y' (8n).
     #  pull: 4m column ids + 4(n+1) row offsets + 4n y (read once) + 4n x_old +
     #    4n out-degrees + 8n (x', y' written).
+    algo_bytes = 4 * m + 4 * (n + 1) + 12 * n
     blocked = gx.tune_get("pr.blocking") != 0
     sweep = "blocked" if blocked else "pull"
     sweep_ms = sweep_ms_total / max(sweeps, 1)
@@ -515,10 +520,12 @@ def main() -> int:
     else:
         kname = ("pull sweep: rs::sweep_kernel<float,HOT,PrSrc,PrFin> + rs::span_finalize_kernel")
     achieved = algo_bytes / (sweep_ms * 1e-3) / 1e9
+    design_rate = design_bytes / (sweep_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved / (HBM_PEAK / 1e9), 4), "traffic": measured_traffic(args.scale, sweep),
                 "kernel": kname, "bytes_per_launch": algo_bytes, "avg_launch_ms": round(sweep_ms, 4),
-                "launches": sweeps}
+                "launches": sweeps, "sweep_bytes_per_launch": design_bytes,
+                "sweep_bytes_frac": round(design_rate / (HBM_PEAK / 1e9), 4)}
 
     # FP64 run and the FP32-vs-FP64 error.
     x64, st64 = gx.pagerank(G, alpha=0.85, tol=1e-9, max_iter=100, precision="fp64")
