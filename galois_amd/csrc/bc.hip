@@ -333,11 +333,28 @@ __global__ void bcb_ranges_kernel(const uint32_t* __restrict__ src, int count, i
   R[groups].istart = acc;
 }
 
-// comp_hi[label] = 1 + largest vertex with that component label.
+// comp_hi[label] = 1 + largest vertex with that component label.  Vertices
+// are visited from the highest id down and an atomic is issued only when it
+// would raise the stored value, so a giant component settles its maximum in
+// its first wavefronts and the rest only read it (one word would otherwise
+// take an atomic per vertex).  A wavefront whose live lanes share one label
+// reduces first.
 __global__ void bcb_comp_hi_kernel(const uint32_t* __restrict__ label, int64_t n, uint32_t* __restrict__ comp_hi) {
-  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
-       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    atomicMax(&comp_hi[label[v]], static_cast<uint32_t>(v + 1));
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (int64_t b = blockIdx.x * static_cast<int64_t>(blockDim.x) + (threadIdx.x & ~(kWave - 1)); b < n; b += stride) {
+    const int64_t i = b + lane_id();
+    const bool live = i < n;
+    const int64_t v = n - 1 - i;  // highest ids first
+    const uint32_t l = live ? label[v] : 0xffffffffu;
+    const uint32_t l0 = __shfl(l, 0);  // lane 0 is live whenever any lane is
+    const uint64_t same = __ballot(live && l == l0), alive = __ballot(live);
+    if (same == alive) {
+      const uint32_t hi = wave_max(live ? static_cast<uint32_t>(v + 1) : 0u);
+      if (lane_id() == 0 && comp_hi[l0] < hi) atomicMax(&comp_hi[l0], hi);
+    } else if (live && comp_hi[l] < static_cast<uint32_t>(v + 1)) {
+      atomicMax(&comp_hi[l], static_cast<uint32_t>(v + 1));
+    }
+  }
 }
 
 // Level-0 masks, visited masks and path counts of the sources.
