@@ -47,6 +47,7 @@ Knob g_knobs[] = {
     {"frontier.hub_degree", {-1}, 64, int64_t(1) << 30},
     {"pr.hot_sources", {-1}, 0, int64_t(1) << 31},
     {"pr.blocking", {-1}, 0, 2},
+    {"pr.task_edges", {-1}, int64_t(1) << 12, int64_t(1) << 30},
     {"bfs.visited_bitmap", {-1}, 0, 1},
 };
 Knob* find_knob(const char* key) {

@@ -207,9 +207,10 @@ void build_layout(gx_graph* g) {
     first[cls][nb] = static_cast<uint32_t>(jobs[cls].size());
   }
 
-  // Tasks: a bin with at most kTaskEntries edges is one task; a heavier bin
-  // is cut into slices of about kTaskEntries (long, then mid, then short
-  // jobs) that share a global accumulator slot.
+  // Tasks: a bin with at most `task_entries` edges (knob "pr.task_edges")
+  // is one task; a heavier bin is cut into slices of about that many (long,
+  // then mid, then short jobs) that share a global accumulator slot.
+  const uint64_t task_entries = static_cast<uint64_t>(tune_value("pr.task_edges", kTaskEntries));
   std::vector<std::pair<uint64_t, Task>> tasks;
   std::vector<Multi> multi;
   for (int64_t b = 0; b < nb; ++b) {
@@ -218,7 +219,7 @@ void build_layout(gx_graph* g) {
       for (uint32_t j = first[cls][b]; j < first[cls][b + 1]; ++j) entries += jobs[cls][j].len;
     Task t{};
     t.bin = static_cast<uint32_t>(b);
-    if (entries <= static_cast<uint64_t>(kTaskEntries)) {
+    if (entries <= task_entries) {
       t.slot = kSingle;
       for (int cls = 0; cls < 3; ++cls) {
         t.job0[cls] = first[cls][b];
@@ -241,7 +242,7 @@ void build_layout(gx_graph* g) {
       for (uint32_t j = first[cls][b]; j < first[cls][b + 1]; ++j) {
         t.job1[cls] = j + 1;
         acc += jobs[cls][j].len;
-        if (acc >= static_cast<uint64_t>(kTaskEntries)) {
+        if (acc >= task_entries) {
           tasks.push_back({acc, t});
           acc = 0;
           reset(cls, j + 1);

@@ -116,6 +116,11 @@ void gx_set_profiling(int on);
  *                          also for FP64 (FP64 atomic row sums, order
  *                          dependent in the last bit), 0 = merge-path pull
  *                          sweep for both (Kahan row sums in FP32)
+ *   "pr.task_edges"        propagation-blocked sweep: a destination bin
+ *                          with more in-edges than this is summed by
+ *                          several workgroups (slices) that meet in a
+ *                          global accumulator; read when a graph's layout
+ *                          is built; default 2^19, range [2^12, 2^30]
  *   "bfs.visited_bitmap"   gx_bfs: 1 = test a visited bitmap before the
  *                          level word of every edge's target, 0 = not; the
  *                          default uses it from 2^25 vertices on

@@ -143,6 +143,11 @@ def test_tuning_knobs():
     gx.tune_set("pr.blocking", -1)
     with pytest.raises(GxError):
         gx.tune_set("pr.blocking", 3)  # 0, 1 or 2 only
+    gx.tune_set("pr.task_edges", 1 << 20)
+    assert gx.tune_get("pr.task_edges") == 1 << 20
+    gx.tune_set("pr.task_edges", -1)
+    with pytest.raises(GxError):
+        gx.tune_set("pr.task_edges", 1000)  # below 2^12
     with pytest.raises(GxError):
         gx.tune_set("no.such.knob", 1)
     with pytest.raises(GxError):
