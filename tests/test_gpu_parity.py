@@ -315,22 +315,24 @@ def test_pagerank_blocked_vs_pull(gx, oracle, scale):
         assert stb.launches == 4 + 4 * enq, (stb.launches, stb.iterations)
 
 
-@pytest.mark.parametrize("task_edges", [1 << 12, 1 << 30])
-def test_pagerank_task_edges_knob(gx, task_edges):
+@pytest.mark.parametrize("knobs", [{"pr.task_edges": 1 << 12}, {"pr.task_edges": 1 << 30}])
+def test_pagerank_task_knobs(gx, knobs):
     """The task size of the blocked sweep only decides how a bin's in-edges
     are split over workgroups (2^12: every bin of RMAT-18 in dozens of slices
     meeting in the global accumulator; 2^30: no bin sliced).  The fixed-point
     row sums are exact, so the FP32 ranks and the iteration count are
     bit-identical to the default's (2^19)."""
-    say(f"rmat18 pagerank task_edges={task_edges}")
+    say(f"rmat18 pagerank {knobs}")
     G = gx.Graph.rmat(18, 16, symmetric=False)
     ref, st_ref = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
-    gx.tune_set("pr.task_edges", task_edges)
     try:
-        G2 = gx.Graph.rmat(18, 16, symmetric=False)  # its layout is built under the knob
+        for k, v in knobs.items():
+            gx.tune_set(k, v)
+        G2 = gx.Graph.rmat(18, 16, symmetric=False)  # its layout is built under the knobs
         x, st = gx.pagerank(G2, tol=1e-9, max_iter=100, precision="fp32")
     finally:
-        gx.tune_set("pr.task_edges", -1)
+        for k in knobs:
+            gx.tune_set(k, -1)
     assert st.iterations == st_ref.iterations
     np.testing.assert_array_equal(x.cpu().numpy(), ref.cpu().numpy())
 
