@@ -120,7 +120,9 @@ struct PullPlan {
 // into runs by (source chunk, destination bin), chunk-major, and by (source,
 // destination) inside a run; a group is the edges of one source in a run.
 // dst16 is the in-bin destination of every edge (bit 15: first edge of a
-// group), gsrc16 the in-chunk source of every group, vals the per-group
+// group), stored with the runs in bin-major order (the runs of one bin from
+// every chunk are adjacent); gsrc16 is the in-chunk source of every group,
+// in run-major order, and vals the per-group
 // contributions of the current iteration.  pieces/jobs/tasks/multi are the
 // work tables of the two phases.
 struct PbLayout {

@@ -9,10 +9,14 @@
 //     numbering, and the edge and group counts per run (one atomic per run of
 //     equal keys in a wavefront).
 //  3. exclusive scans: group number of every edge's group start, start of
-//     every run in edges and in groups; group_kernel writes gsrc16.
+//     every run in edges and in groups; bin_major_kernel moves dst16 into
+//     bin-major run order (the runs of one bin, from every chunk, in one
+//     stretch: phase 2 reads a bin's runs as one stream); group_kernel
+//     writes gsrc16 (groups stay in run-major order, as phase 1 writes them).
 //  4. host: phase-1 pieces per chunk (in groups), phase-2 jobs per bin in
-//     three length classes, tasks (bins, heavy bins sliced), sorted longest
-//     first; job_base_kernel then fills every job's group base.
+//     three length classes (start in bin-major dst16), tasks (bins, heavy
+//     bins sliced), sorted longest first; job_base_kernel then fills every
+//     job's group base from its run-major first edge.
 //
 // Edges within a run are in (source, destination) order: neighbouring edges
 // of a group have distinct, ascending destinations, so a wavefront's adds
@@ -93,12 +97,27 @@ __global__ void __launch_bounds__(kBuildBlock) group_kernel(const uint64_t* __re
     if (flag[e]) gsrc16[gpos[e]] = static_cast<uint16_t>((keys[e] >> kShift) & (kSpan - 1));
 }
 
-// gbase of every job: groups opened before its first edge, minus one.
+// dst16 from run-major (chunk, bin) order into bin-major (bin, chunk)
+// order: edge e of run r moves from cstart[r] + i to bstart[r] + i.
+__global__ void __launch_bounds__(kBuildBlock) bin_major_kernel(const uint64_t* __restrict__ keys, int64_t m,
+                                                                const uint32_t* __restrict__ cstart,
+                                                                const uint32_t* __restrict__ bstart,
+                                                                const uint16_t* __restrict__ in,
+                                                                uint16_t* __restrict__ out) {
+  const int64_t stride = static_cast<int64_t>(gridDim.x) * kBuildBlock;
+  for (int64_t e = blockIdx.x * static_cast<int64_t>(kBuildBlock) + threadIdx.x; e < m; e += stride) {
+    const uint32_t r = static_cast<uint32_t>(keys[e] >> (2 * kShift));
+    out[bstart[r] + (static_cast<uint32_t>(e) - cstart[r])] = in[e];
+  }
+}
+
+// gbase of every job: groups opened before its first edge, minus one.  The
+// gbase field holds the job's first edge in run-major order on entry.
 __global__ void __launch_bounds__(kBuildBlock) job_base_kernel(Job* __restrict__ jobs, int64_t njobs,
                                                                const uint32_t* __restrict__ gpos) {
   const int64_t stride = static_cast<int64_t>(gridDim.x) * kBuildBlock;
   for (int64_t j = blockIdx.x * static_cast<int64_t>(kBuildBlock) + threadIdx.x; j < njobs; j += stride)
-    jobs[j].gbase = gpos[jobs[j].start] - 1u;
+    jobs[j].gbase = gpos[jobs[j].gbase] - 1u;
 }
 
 }  // namespace
@@ -114,7 +133,7 @@ void build_layout(gx_graph* g) {
   const int64_t nb = (n + kSpan - 1) / kSpan;
   const int64_t nkeys = nb * nb;
 
-  std::vector<uint32_t> start(nkeys + 1), gstart(nkeys + 1);
+  std::vector<uint32_t> start(nkeys + 1), gstart(nkeys + 1), bstart;
   DevBuf gpos;  // exclusive scan of the group flags, m + 1 entries
   int64_t ngroups = 0;
   {
@@ -141,12 +160,13 @@ void build_layout(gx_graph* g) {
     cur.ensure((nkeys + 1) * 4);
     gcur.ensure((nkeys + 1) * 4);
     gpos.ensure((m + 1) * 4);
-    L.dst16.ensure(m * 2);
+    DevBuf d16;  // run-major dst16; L.dst16 gets it bin-major below
+    d16.ensure(m * 2);
     GX_HIP(hipMemsetAsync(cnt.p, 0, (nkeys + 1) * 4, s));
     GX_HIP(hipMemsetAsync(gcnt.p, 0, (nkeys + 1) * 4, s));
     GX_HIP(hipMemsetAsync(flag.as<uint32_t>() + m, 0, 4, s));
     mark_kernel<<<grid_for(m, kBuildBlock, 1 << 16), kBuildBlock, 0, s>>>(
-        sorted.as<uint64_t>(), m, L.dst16.as<uint16_t>(), flag.as<uint32_t>(), cnt.as<uint32_t>(),
+        sorted.as<uint64_t>(), m, d16.as<uint16_t>(), flag.as<uint32_t>(), cnt.as<uint32_t>(),
         gcnt.as<uint32_t>());
     GX_LAUNCHED("pb mark_kernel");
     exclusive_scan_u32(cnt.as<uint32_t>(), cur.as<uint32_t>(), nkeys + 1, s);
@@ -161,6 +181,31 @@ void build_layout(gx_graph* g) {
     GX_REQUIRE(start[nkeys] == static_cast<uint32_t>(m), "propagation-blocking layout: edge count mismatch");
     GX_REQUIRE(gstart[nkeys] == total && ngroups >= 1 && ngroups <= m,
                "propagation-blocking layout: group count mismatch");
+    // Bin-major run offsets: phase 2 reads the runs of one bin, from every
+    // chunk, as one contiguous stretch of dst16.
+    bstart.resize(nkeys + 1);
+    {
+      uint32_t at = 0;
+      for (int64_t b = 0; b < nb; ++b)
+        for (int64_t c = 0; c < nb; ++c) {
+          const int64_t key = c * nb + b;
+          bstart[key] = at;
+          at += start[key + 1] - start[key];
+        }
+      bstart[nkeys] = at;
+      GX_REQUIRE(at == static_cast<uint32_t>(m), "propagation-blocking layout: bin-major offsets");
+    }
+    {
+      DevBuf bst;
+      bst.ensure(nkeys * 4);
+      GX_HIP(hipMemcpy(bst.p, bstart.data(), nkeys * 4, hipMemcpyHostToDevice));
+      L.dst16.ensure(m * 2);
+      bin_major_kernel<<<grid_for(m, kBuildBlock, 1 << 16), kBuildBlock, 0, s>>>(
+          sorted.as<uint64_t>(), m, cur.as<uint32_t>(), bst.as<uint32_t>(), d16.as<uint16_t>(),
+          L.dst16.as<uint16_t>());
+      GX_LAUNCHED("pb bin_major_kernel");
+      GX_HIP(hipStreamSynchronize(s));
+    }
     L.gsrc16.ensure(ngroups * 2);
     group_kernel<<<grid_for(m, kBuildBlock, 1 << 16), kBuildBlock, 0, s>>>(
         sorted.as<uint64_t>(), flag.as<uint32_t>(), gpos.as<uint32_t>(), m, L.gsrc16.as<uint16_t>());
@@ -189,7 +234,7 @@ void build_layout(gx_graph* g) {
       while (len > 0) {
         const uint32_t l = std::min<uint32_t>(len, kJob);
         const int cls = l > kMid ? 0 : (l > kShort ? 1 : 2);
-        per[cls][b].push_back(Job{p, l, 0u});
+        per[cls][b].push_back(Job{bstart[key] + (p - start[key]), l, p});  // gbase: set on the device
         p += l;
         len -= l;
       }

@@ -13,8 +13,10 @@
 //           the edges of one (chunk, bin) pair; a group is the edges of one
 //           source inside a run, which all carry the same contribution.  Per
 //           edge: the in-bin destination (dst16, bit 15 set on the first
-//           edge of a group); per group: the in-chunk source (gsrc16) and
-//           the contribution slot vals[group].
+//           edge of a group; stored bin-major, so that a bin's runs from
+//           every chunk form one stretch); per group, in run-major order:
+//           the in-chunk source (gsrc16) and the contribution slot
+//           vals[group].
 //   phase 1 (gather_kernel) one workgroup per piece of a chunk: the chunk's
 //           kSpan y values are staged in LDS (FP32: encoded as fixed-point
 //           words, see encode_fixed) and vals[q] = y[gsrc16[q]] is streamed
@@ -71,9 +73,10 @@ constexpr uint32_t kFirst = 1u << 15;                // dst16 flag: first edge o
 struct Piece {
   uint32_t chunk, p0, p1, pad;
 };
-// Edges [start, start + len) of one run.  The group of edge start + i is
-// gbase + (number of group flags among edges start .. start + i), modulo
-// 2^32: gbase is the number of groups that begin before `start`, minus one.
+// Edges [start, start + len) of one run, positions in the bin-major dst16.
+// The group of its i-th edge is gbase + (number of group flags among its
+// edges 0 .. i), modulo 2^32: gbase is the number of groups that begin
+// before the job's first edge in run-major order, minus one.
 struct Job {
   uint32_t start, len, gbase;
 };
