@@ -11,7 +11,8 @@
 //  3. exclusive scans: group number of every edge's group start, start of
 //     every run in edges and in groups; bin_major_kernel moves dst16 into
 //     bin-major run order (the runs of one bin, from every chunk, in one
-//     stretch: phase 2 reads a bin's runs as one stream); group_kernel
+//     stretch, long runs first, then mid, then short: each of phase 2's
+//     three passes over a bin reads one contiguous part); group_kernel
 //     writes gsrc16 (groups stay in run-major order, as phase 1 writes them).
 //  4. host: phase-1 pieces per chunk (in groups), phase-2 jobs per bin in
 //     three length classes (start in bin-major dst16), tasks (bins, heavy
@@ -182,16 +183,22 @@ void build_layout(gx_graph* g) {
     GX_REQUIRE(gstart[nkeys] == total && ngroups >= 1 && ngroups <= m,
                "propagation-blocking layout: group count mismatch");
     // Bin-major run offsets: phase 2 reads the runs of one bin, from every
-    // chunk, as one contiguous stretch of dst16.
+    // chunk, as one stretch of dst16, and inside it the runs whose first job
+    // is long, then mid, then short, so that each of its three passes over
+    // the bin (one per job class) reads a contiguous part.
     bstart.resize(nkeys + 1);
     {
       uint32_t at = 0;
       for (int64_t b = 0; b < nb; ++b)
-        for (int64_t c = 0; c < nb; ++c) {
-          const int64_t key = c * nb + b;
-          bstart[key] = at;
-          at += start[key + 1] - start[key];
-        }
+        for (int cls = 0; cls < 3; ++cls)
+          for (int64_t c = 0; c < nb; ++c) {
+            const int64_t key = c * nb + b;
+            const uint32_t len = start[key + 1] - start[key];
+            const uint32_t l = std::min<uint32_t>(len, kJob);
+            if ((l > kMid ? 0 : (l > kShort ? 1 : 2)) != cls) continue;
+            bstart[key] = at;
+            at += len;
+          }
       bstart[nkeys] = at;
       GX_REQUIRE(at == static_cast<uint32_t>(m), "propagation-blocking layout: bin-major offsets");
     }

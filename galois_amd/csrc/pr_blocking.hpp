@@ -14,7 +14,8 @@
 //           source inside a run, which all carry the same contribution.  Per
 //           edge: the in-bin destination (dst16, bit 15 set on the first
 //           edge of a group; stored bin-major, so that a bin's runs from
-//           every chunk form one stretch); per group, in run-major order:
+//           every chunk form one stretch, ordered by the length class of
+//           their first job); per group, in run-major order:
 //           the in-chunk source (gsrc16) and the contribution slot
 //           vals[group].
 //   phase 1 (gather_kernel) one workgroup per piece of a chunk: the chunk's
