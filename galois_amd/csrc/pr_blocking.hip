@@ -186,21 +186,34 @@ void build_layout(gx_graph* g) {
     // chunk, as one stretch of dst16, and inside it the runs whose first job
     // is long, then mid, then short, so that each of its three passes over
     // the bin (one per job class) reads a contiguous part.
+    // Two passes in run order (the table is 4 bytes x bins^2): edges per
+    // (bin, class), their exclusive scan, then a cursor per (bin, class) that
+    // the chunks, in ascending order, advance.
     bstart.resize(nkeys + 1);
     {
+      auto cls_of = [](uint32_t len) {
+        const uint32_t l = std::min<uint32_t>(len, kJob);
+        return l > kMid ? 0 : (l > kShort ? 1 : 2);
+      };
+      std::vector<uint32_t> cursor(nb * 3, 0);
+      for (int64_t key = 0; key < nkeys; ++key) {
+        const uint32_t len = start[key + 1] - start[key];
+        cursor[(key % nb) * 3 + cls_of(len)] += len;
+      }
       uint32_t at = 0;
-      for (int64_t b = 0; b < nb; ++b)
-        for (int cls = 0; cls < 3; ++cls)
-          for (int64_t c = 0; c < nb; ++c) {
-            const int64_t key = c * nb + b;
-            const uint32_t len = start[key + 1] - start[key];
-            const uint32_t l = std::min<uint32_t>(len, kJob);
-            if ((l > kMid ? 0 : (l > kShort ? 1 : 2)) != cls) continue;
-            bstart[key] = at;
-            at += len;
-          }
-      bstart[nkeys] = at;
+      for (uint32_t& v : cursor) {
+        const uint32_t t = v;
+        v = at;
+        at += t;
+      }
       GX_REQUIRE(at == static_cast<uint32_t>(m), "propagation-blocking layout: bin-major offsets");
+      for (int64_t key = 0; key < nkeys; ++key) {
+        const uint32_t len = start[key + 1] - start[key];
+        uint32_t& p = cursor[(key % nb) * 3 + cls_of(len)];
+        bstart[key] = p;
+        p += len;
+      }
+      bstart[nkeys] = at;
     }
     {
       DevBuf bst;
