@@ -120,7 +120,8 @@ void gx_set_profiling(int on);
  *                          with more in-edges than this is summed by
  *                          several workgroups (slices) that meet in a
  *                          global accumulator; read when a graph's layout
- *                          is built; default 2^19, range [2^12, 2^30]
+ *                          is built; default m/1024 within [2^19, 2^20],
+ *                          range [2^12, 2^30]
  *   "bfs.visited_bitmap"   gx_bfs: 1 = test a visited bitmap before the
  *                          level word of every edge's target, 0 = not; the
  *                          default uses it from 2^25 vertices on
