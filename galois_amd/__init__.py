@@ -25,7 +25,8 @@ from ._lib import (GX_BC_INFINITY, GX_DIST_INFINITY, GX_GRAPH_FWEIGHTED, GX_GRAP
 __all__ = [
     "Graph", "GxError", "Stats", "bfs", "bfs_do", "sssp", "connected_components", "connectedComponents",
     "kcore", "pagerank", "pagerank_ref", "pagerank_push", "pagerank_weighted", "pagerank_adjoint",
-    "betweenness_centrality", "betweenness_centrality_batched", "pick_sources", "read_gr", "write_gr",
+    "betweenness_centrality", "betweenness_centrality_batched", "pick_sources", "degree_power_law", "read_gr",
+    "write_gr",
     "GX_DIST_INFINITY", "GX_BC_INFINITY", "set_profiling", "differentiable_pagerank",
 ]
 
@@ -247,6 +248,16 @@ def pick_sources(row, count: int, seed: int = KRAND_SEED) -> np.ndarray:
     return out[:count]
 
 
+def degree_power_law(row) -> bool:
+    """The reference's AutoAlgo degree test (Utils.h:37-57) on a host CSR row
+    array: average degree >= 10 and, over 1000 picked vertices, mean / 1.25 >
+    median (gx_degree_power_law)."""
+    row = np.ascontiguousarray(row, dtype=np.int32)
+    out = C.c_int()
+    check(lib().gx_degree_power_law(len(row) - 1, int(row[-1]), row.ctypes.data, C.byref(out)))
+    return bool(out.value)
+
+
 # -------------------------------------------------------------- algorithms
 def _empty(n, dtype):
     torch = _torch()
@@ -305,7 +316,8 @@ def kcore(g: Graph, k: int):
 
 
 def pagerank(g: Graph, alpha: float = 0.85, tol: float = 1e-9, max_iter: int = 100, precision: str = "fp32"):
-    """Normalised pull PageRank; precision 'fp32' (Kahan) or 'fp64'. Returns (pr, Stats)."""
+    """Normalised pull PageRank; precision 'fp32' (exact fixed-point row sums in the default blocked sweep,
+    Kahan sums with pr.blocking = 0) or 'fp64'. Returns (pr, Stats)."""
     torch = _torch()
     fp64 = precision == "fp64"
     pr = _empty(g.n, torch.float64 if fp64 else torch.float32)

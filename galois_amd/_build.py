@@ -68,7 +68,8 @@ BIN = PKG / "bin"
 # Lonestar app entry points: source stem -> executable name of the reference.
 APP_NAMES = {"bfs": "bfs", "sssp": "sssp", "connected_components": "connected-components", "kcore": "k-core",
              "pagerank_pull": "pagerank-pull", "betweennesscentrality": "betweennesscentrality",
-             "graph_convert": "graph-convert"}
+             "graph_convert": "graph-convert", "pagerank_push": "pagerank-push",
+             "bfs_directionopt": "bfs-directionopt"}
 
 
 def _build_app(src: Path) -> Path:

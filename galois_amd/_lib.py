@@ -98,6 +98,7 @@ SIGNATURES: dict[str, tuple] = {
     "gx_gen_grid": (INT, [U32, U32, U32, U64, U32, U32, PG]),
     "gx_graph_union": (INT, [P, U32, PG, P]),
     "gx_pick_sources": (INT, [I64, P, U32, U32, P]),
+    "gx_degree_power_law": (INT, [I64, I64, P, C.POINTER(INT)]),
     "gx_bfs": (INT, [P, U32, P, PST]),
     "gx_bfs_do": (INT, [P, U32, U32, U32, P, P, PST]),
     "gx_sssp_delta": (INT, [P, U32, U32, P, PST]),

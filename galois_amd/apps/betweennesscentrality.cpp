@@ -89,6 +89,8 @@ int main(int argc, char** argv) {
       std::cout << line;
     }
   }
+  write_node_values(cli.str("o"), b,
+                    [](FILE* f, double v) { return std::fprintf(f, "%.9f\n", static_cast<float>(v)) > 0; });
   check(gx_graph_free(g), "gx_graph_free");
   total.stop();
   exec.report();

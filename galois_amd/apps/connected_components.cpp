@@ -38,8 +38,11 @@ int main(int argc, char** argv) {
   check(gx_cc(g, label.p, nullptr), "gx_cc");
   hip_check(hipDeviceSynchronize(), "cc");
   exec.stop();
+  const std::vector<uint32_t> lab = label.host();
+  // "node label" lines (lonestar/analytics/gpu/connected-components/
+  // support.cu:16-36 without its per-edge debug lines).
+  write_node_values(cli.str("o"), lab, [](FILE* f, uint32_t v) { return std::fprintf(f, "%u\n", v) > 0; });
   if (!cli.on("noverify")) {
-    const std::vector<uint32_t> lab = label.host();
     size_t reps = 0;
     std::unordered_map<uint32_t, int64_t> members;  // non-representative members per component
     for (int64_t v = 0; v < hg.n(); ++v) {

@@ -40,6 +40,12 @@ int main(int argc, char** argv) {
   hip_check(hipDeviceSynchronize(), "kcore");
   exec.stop();
   if (!cli.on("noverify")) print("Number of nodes in the ", k, "-core is ", alive, "\n");
+  // "node flag" lines: 1 for members of the k-core (the flag field of
+  // lonestar/analytics/distributed/k-core/kcore_push.cpp:69).
+  if (!cli.str("o").empty()) {
+    const std::vector<uint32_t> deg = degree.host();
+    write_node_values(cli.str("o"), deg, [k](FILE* f, uint32_t d) { return std::fprintf(f, "%d\n", d >= k) > 0; });
+  }
   check(gx_graph_free(g), "gx_graph_free");
   total.stop();
   exec.report();

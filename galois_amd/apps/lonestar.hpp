@@ -65,6 +65,7 @@ class Cli {
     value("t", "1", "Number of threads (default value 1)");
     value("statFile", "", "ouput file to print stats to (default value empty)");
     flag("symmetricGraph", "Specify that the input graph is symmetric");
+    value("o", "", "Write the per-node result to this file, one 'node value' line per node (- = stdout)");
   }
   void flag(const std::string& name, const std::string& desc) { opts_[name] = Opt{"false", desc, true, false}; }
   void value(const std::string& name, const std::string& def, const std::string& desc) {
@@ -259,28 +260,70 @@ struct DeviceArray {
   }
 };
 
-// Source picker and power-law test of lonestar/liblonestar/include/Lonestar/
-// Utils.h:12-57 (mt19937 seeded 27491095, rejecting degree-0 vertices;
-// power law when the sampled mean / 1.25 exceeds the sampled median).
+// isApproximateDegreeDistributionPowerLaw of lonestar/liblonestar/include/
+// Lonestar/Utils.h:37-57 through the C ABI (gx_degree_power_law).
 inline bool approximately_power_law(const HostGraph& hg) {
-  const int64_t n = hg.n(), m = hg.m();
-  if (n == 0) return false;
-  const uint32_t avg = static_cast<uint32_t>(m / n);
-  if (avg < 10) return false;
-  uint32_t samples = 1000;
-  if (samples > n) samples = static_cast<uint32_t>(n);
-  std::vector<uint32_t> pick(samples);
-  check(gx_pick_sources(n, hg.csr.row, samples, 27491095u, pick.data()), "gx_pick_sources");
-  std::vector<uint32_t> deg(samples);
-  uint32_t total = 0;
-  for (uint32_t i = 0; i < samples; ++i) {
-    deg[i] = static_cast<uint32_t>(hg.csr.row[pick[i] + 1] - hg.csr.row[pick[i]]);
-    total += deg[i];
+  int power_law = 0;
+  check(gx_degree_power_law(hg.n(), hg.m(), hg.csr.row, &power_law), "gx_degree_power_law");
+  return power_law != 0;
+}
+
+// Per-node output of the LonestarGPU apps (the skelapp -o option,
+// libgpu/src/skelapp/skel.cu:75,105; format of lonestar/analytics/gpu/bfs/
+// support.cu:27-47): one "node value" line per node, read by
+// scripts/result_checker.py.  `fmt` prints one value.
+template <class T, class F>
+void write_node_values(const std::string& path, const std::vector<T>& values, F fmt) {
+  if (path.empty()) return;
+  FILE* f = path == "-" ? stdout : std::fopen(path.c_str(), "w");
+  if (!f) die("cannot open output file " + path);
+  for (size_t i = 0; i < values.size(); ++i) {
+    if (std::fprintf(f, "%zu ", i) < 0 || !fmt(f, values[i])) die("write to " + path + " failed");
   }
-  std::sort(deg.begin(), deg.end());
-  const double mean = static_cast<double>(total) / samples;
-  const double median = deg[samples / 2];
-  return mean / 1.25 > median;
+  if (f != stdout && std::fclose(f) != 0) die("write to " + path + " failed");
+  if (f == stdout) std::fflush(stdout);
+}
+
+// Distances and levels: unreached vertices are printed as UINT32_MAX / 4
+// (1073741823), the value of the distributed apps' output, as
+// lonestar/analytics/gpu/bfs/support.cu:38-45 and sssp/support.cu:45-52 do.
+inline void write_distances(const std::string& path, const std::vector<uint32_t>& d) {
+  write_node_values(path, d, [](FILE* f, uint32_t v) {
+    return std::fprintf(f, "%u\n", v == GX_DIST_INFINITY ? 1073741823u : v) > 0;
+  });
+}
+
+// The 20 largest ranks, ties to the lower id (TopPair ordering and printTop
+// of lonestar/analytics/cpu/pagerank/PageRank-constants.h:60-105).
+inline void print_top(const std::vector<float>& x, unsigned topn = 20) {
+  struct Key {
+    float value;
+    uint32_t id;
+    bool operator<(const Key& b) const { return value == b.value ? id > b.id : value < b.value; }
+  };
+  std::map<Key, uint32_t> top;
+  for (uint32_t v = 0; v < x.size(); ++v) {
+    const Key k{x[v], v};
+    if (top.size() < topn) {
+      top.emplace(k, v);
+      continue;
+    }
+    if (top.begin()->first < k) {
+      top.erase(top.begin());
+      top.emplace(k, v);
+    }
+  }
+  std::cout << "Rank PageRank Id\n";
+  int rank = 1;
+  for (auto it = top.rbegin(); it != top.rend(); ++it, ++rank)
+    std::cout << rank << ": " << it->first.value << " " << it->first.id << "\n";
+}
+
+// Ranks per node, "%d %f"-style lines as lonestar/analytics/gpu/pagerank/
+// support.cu:95-97, with 9 significant digits (the %f of the reference
+// prints normalised ranks as 0.000000).
+inline void write_ranks(const std::string& path, const std::vector<float>& x) {
+  write_node_values(path, x, [](FILE* f, float v) { return std::fprintf(f, "%.9g\n", v) > 0; });
 }
 
 }  // namespace lonestar

@@ -6,8 +6,6 @@
 // bit-exact with the reference for the same in-edge order; -algo=Topo is the
 // Jacobi topological pull).  The input is the transposed graph, as in the
 // reference.
-#include <map>
-
 #include "lonestar.hpp"
 
 using namespace lonestar;
@@ -69,30 +67,8 @@ int main(int argc, char** argv) {
   info("Max rank is ", mx);
   info("Min rank is ", mn);
   info("Sum is ", sum);
-  if (!cli.on("noverify")) {
-    // Top 20 by value, ties to the lower id (TopPair ordering).
-    struct Key {
-      float value;
-      uint32_t id;
-      bool operator<(const Key& b) const { return value == b.value ? id > b.id : value < b.value; }
-    };
-    std::map<Key, uint32_t> top;
-    for (uint32_t v = 0; v < x.size(); ++v) {
-      const Key k{x[v], v};
-      if (top.size() < 20) {
-        top.emplace(k, v);
-        continue;
-      }
-      if (top.begin()->first < k) {
-        top.erase(top.begin());
-        top.emplace(k, v);
-      }
-    }
-    std::cout << "Rank PageRank Id\n";
-    int rank = 1;
-    for (auto it = top.rbegin(); it != top.rend(); ++it, ++rank)
-      std::cout << rank << ": " << it->first.value << " " << it->first.id << "\n";
-  }
+  write_ranks(cli.str("o"), x);
+  if (!cli.on("noverify")) print_top(x);
   check(gx_graph_free(g), "gx_graph_free");
   check(gx_graph_free(gt), "gx_graph_free");
   total.stop();

@@ -99,6 +99,7 @@ int main(int argc, char** argv) {
   exec.stop();
   const std::vector<uint32_t> d = dist.host();
   std::cout << "Node " << report << " has distance " << d[report] << "\n";
+  write_distances(cli.str("o"), d);
   uint64_t maxd = 0, sum = 0, visited = 0;
   for (uint32_t x : d)
     if (x != GX_DIST_INFINITY) {

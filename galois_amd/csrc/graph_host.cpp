@@ -249,3 +249,34 @@ extern "C" gx_status gx_pick_sources(int64_t n, const int32_t* row, uint32_t cou
   }
   GX_API_END
 }
+
+// isApproximateDegreeDistributionPowerLaw (lonestar/liblonestar/include/
+// Lonestar/Utils.h:37-57): average degree m/n (integer) below 10 is not a
+// power law; otherwise the degrees of 1000 vertices (n if fewer) drawn by
+// the source picker (seed 27491095, degree-0 vertices rejected) are sorted,
+// and the graph is a power law when their mean / 1.25 exceeds their median
+// (the element at samples/2).  The sum is a uint32 as in the reference.
+extern "C" gx_status gx_degree_power_law(int64_t n, int64_t m, const int32_t* row, int* power_law) {
+  GX_API_BEGIN
+  check_ptr(row, "row");
+  check_ptr(power_law, "power_law");
+  GX_REQUIRE(n > 0, "graph has no vertices");
+  GX_REQUIRE(m >= 0 && row[n] == m, "row[n] differs from m");
+  *power_law = 0;
+  const uint32_t average = static_cast<uint32_t>(m / n);
+  if (average < 10) return GX_OK;
+  const uint32_t samples = n < 1000 ? static_cast<uint32_t>(n) : 1000u;
+  std::vector<uint32_t> pick(samples), deg(samples);
+  const gx_status st = gx_pick_sources(n, row, samples, 27491095u, pick.data());
+  if (st != GX_OK) return st;
+  uint32_t total = 0;
+  for (uint32_t i = 0; i < samples; ++i) {
+    deg[i] = static_cast<uint32_t>(row[pick[i] + 1] - row[pick[i]]);
+    total += deg[i];
+  }
+  std::sort(deg.begin(), deg.end());
+  const double mean = static_cast<double>(total) / samples;
+  const double median = deg[samples / 2];
+  *power_law = mean / 1.25 > median ? 1 : 0;
+  GX_API_END
+}

@@ -1,4 +1,4 @@
-// PageRank: normalised pull (FP32-Kahan / FP64), the reference's own
+// PageRank: normalised pull (FP32 / FP64), the reference's own
 // unnormalised pull variants, and the synchronous residual push.
 //
 // Reference

@@ -202,6 +202,16 @@ gx_status gx_graph_union(gx_graph* const* parts, uint32_t count,
 gx_status gx_pick_sources(int64_t n, const int32_t* row, uint32_t count,
                           uint32_t seed, uint32_t* out);
 
+/* Host-only degree-distribution test that the apps' AutoAlgo choice uses
+ * (isApproximateDegreeDistributionPowerLaw, lonestar/liblonestar/include/
+ * Lonestar/Utils.h:37-57, called by SSSP.cpp:410-419, bfsDirectionOpt.cpp:
+ * 469-478 and BetweennessCentrality.cpp:110-122): *power_law = 1 when the
+ * integer average degree m/n is at least 10 and, over the degrees of 1000
+ * vertices drawn by gx_pick_sources (seed 27491095), mean / 1.25 > median.
+ * row is the host CSR row array (n+1, row[n] == m). */
+gx_status gx_degree_power_law(int64_t n, int64_t m, const int32_t* row,
+                              int* power_law);
+
 /* ------------------------------------------------------------ algorithms
  * Outputs are DEVICE pointers owned by the caller. */
 
@@ -248,7 +258,15 @@ gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t* alive,
  * Semantics (normalised): x0 = 1/n; x' = (1-alpha)/n + alpha*(sum_{u->v}
  * x[u]/outdeg(u) + D/n), D = sum of x over dangling (outdeg 0) vertices;
  * stop when ||x'-x||_1 < tol (FP64 reduction) or after max_iter iterations.
- * precision: 0 = FP32 with Kahan-compensated row sums, 1 = FP64.
+ * precision: 0 = FP32 ranks, 1 = FP64 ranks.  FP32 runs by default
+ * through the propagation-blocked sweep, whose row sums are exact (every
+ * contribution rounded to 64-bit fixed point with resolution 2^-62 and
+ * summed with integer atomics, so the sum is order independent and more
+ * accurate than a compensated FP32 sum), and x' = base + alpha*sum is
+ * evaluated in FP32 with error-free transformations; with the knob
+ * "pr.blocking" = 0 it runs the merge-path pull with Kahan-compensated FP32
+ * row sums.  In both, the residual and the dangling mass D are reduced in
+ * FP64 (more accurate than an FP32 reduction and read once per iteration).
  * pr: device float[n] (precision 0) or double[n] (precision 1).
  * Replaces computePRTopological (PageRank-pull.cpp:196-251) with the
  * normalised, dangling-aware semantics. */

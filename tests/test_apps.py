@@ -13,7 +13,8 @@ import pytest
 
 ROOT = Path(__file__).resolve().parents[1]
 BIN = ROOT / "galois_amd" / "bin"
-APPS = ["bfs", "sssp", "connected-components", "k-core", "pagerank-pull", "betweennesscentrality"]
+APPS = ["bfs", "sssp", "connected-components", "k-core", "pagerank-pull", "betweennesscentrality",
+        "pagerank-push", "bfs-directionopt"]
 INF = 2147483646
 
 
@@ -160,3 +161,101 @@ def test_bc_app(inputs, oracle):
     got = np.array([vals[v] for v in range(len(exp))])
     np.testing.assert_allclose(got, exp.astype(np.float32), rtol=1e-6, atol=1e-6)
     assert line_value(r.stdout, "Max BC is ") and "INFO: Running level BC" in r.stdout
+
+
+def node_values(path, kind=int):
+    """The -o file: one 'node value' line per node, in node order."""
+    lines = Path(path).read_text().splitlines()
+    ids = [int(ln.split()[0]) for ln in lines]
+    assert ids == list(range(len(lines)))
+    return np.array([kind(ln.split()[1]) for ln in lines])
+
+
+@pytest.mark.gpu
+def test_node_output_files(inputs, oracle):
+    """-o writes the LonestarGPU per-node output (node value lines; unreached
+    distances as 1073741823) that scripts/result_checker.py compares."""
+    d, (row, col, w), (drow, dcol) = inputs
+    src = int(np.argmax(np.diff(row)))
+    assert run("bfs", f"-startNode={src}", "-o", d / "bfs.out", d / "rmat12.sgr").returncode == 0
+    lv = oracle.bfs(row, col, src).astype(np.int64)
+    np.testing.assert_array_equal(node_values(d / "bfs.out"), np.where(lv == INF, 1073741823, lv))
+    assert run("sssp", "-delta=6", "-startNode=3", f"-o={d / 'sssp.out'}", d / "rmat12.sgr").returncode == 0
+    dist = oracle.sssp(row, col, w, 3, 6).astype(np.int64)
+    np.testing.assert_array_equal(node_values(d / "sssp.out"), np.where(dist == INF, 1073741823, dist))
+    assert run("connected-components", "-symmetricGraph", "-o", d / "cc.out", d / "rmat12.sgr").returncode == 0
+    np.testing.assert_array_equal(node_values(d / "cc.out"), oracle.cc(row, col))
+    assert run("k-core", "-kcore=10", "-symmetricGraph", "-o", d / "kc.out", d / "rmat12.sgr").returncode == 0
+    np.testing.assert_array_equal(node_values(d / "kc.out"), (oracle.kcore(row, col, 10) >= 10).astype(int))
+    assert run("betweennesscentrality", "-numOfSources=4", "-o", d / "bc.out", d / "rmat12.sgr").returncode == 0
+    np.testing.assert_allclose(node_values(d / "bc.out", float),
+                               oracle.bc(row, col, np.arange(4, dtype=np.uint32)), rtol=1e-6, atol=1e-6)
+    trow, tcol, _ = oracle.transpose(drow, dcol)
+    write_gr(d / "rmat12o.tgr", trow, tcol)
+    r = run("pagerank-pull", "-transposedGraph", "-tolerance=1e-4", "-o", d / "pr.out", d / "rmat12o.tgr")
+    assert r.returncode == 0, r.stderr
+    x, _ = oracle.pagerank_residual(drow, dcol, tol=1e-4)
+    np.testing.assert_allclose(node_values(d / "pr.out", float), x, rtol=1e-7)
+    r = run("bfs", "-o", "-", d / "rmat12.sgr")  # "-" is stdout
+    assert r.returncode == 0 and "0 0" in r.stdout.splitlines()
+
+
+@pytest.mark.gpu
+def test_pagerank_push_app(inputs, oracle):
+    d, _, (drow, dcol) = inputs
+    write_gr(d / "rmat12d.gr", drow, dcol)
+    r = run("pagerank-push", "-tolerance=1e-7", "-o", d / "push.out", d / "rmat12d.gr")
+    assert r.returncode == 0, r.stderr
+    assert "tolerance:1e-07, maxIterations:1000" in r.stdout and "Running Edge Async push version," in r.stdout
+    got = node_values(d / "push.out", float)
+    exp, _ = oracle.pagerank_push(drow, dcol, tol=1e-7)
+    np.testing.assert_allclose(got, exp, atol=2e-5)  # FP32 atomics: order-dependent last bits
+    lines = r.stdout.splitlines()
+    top = [ln.split() for ln in lines[lines.index("Rank PageRank Id") + 1:][:20]]
+    assert [int(t[2]) for t in top[:5]] == list(np.argsort(-exp, kind="stable")[:5])
+    r = run("pagerank-push", "-algo=Sync", "-tolerance=1e-3", d / "rmat12d.gr")
+    assert r.returncode == 0 and "Running Edge Sync push version," in r.stdout
+
+
+@pytest.mark.gpu
+def test_bfs_directionopt_app(inputs, oracle):
+    d, (row, col, w), _ = inputs
+    src = int(np.argmax(np.diff(row)))
+    r = run("bfs-directionopt", f"-startNode={src}", "-reportNode=9", "-numRuns=2", "-o", d / "par.out",
+            d / "rmat12.sgr")
+    assert r.returncode == 0, r.stderr
+    lv = oracle.bfs(row, col, src)
+    par = node_values(d / "par.out")
+    assert par[src] == 0  # the reference's source keeps its initial 0
+    for v in range(len(lv)):
+        if v == src:
+            continue
+        if lv[v] == INF:
+            assert par[v] == INF
+        else:
+            p = par[v]
+            assert lv[p] == lv[v] - 1 and v in col[row[p]:row[p + 1]]
+    assert line_value(r.stdout, "Node 9 has parent ") == str(par[9])
+    assert "parent[0] : " in r.stdout and "BFS::go run 1 called" in r.stdout
+
+
+@pytest.mark.gpu
+def test_autoalgo_choice(inputs, gx, tmp_path):
+    """The Auto branches of sssp (SSSP.cpp:410-419), bfs-directionopt
+    (bfsDirectionOpt.cpp:469-478) and BC (BetweennessCentrality.cpp:110-122)
+    choose by the degree test: the power-law RMAT input picks deltaStep /
+    SyncDO / Async, the road-like grid (average degree below 10) picks
+    deltaStepBarrier / Async / Level."""
+    d, (row, col, w), _ = inputs
+    G = gx.Graph.grid(64, 50000, 10000, 7, 1, 1000)
+    grow, gcol, gw = G.to_host()
+    write_gr(tmp_path / "grid.sgr", grow, gcol, gw)
+    assert gx.degree_power_law(row) and not gx.degree_power_law(grow)
+    cases = [("sssp", ["-delta=6"], "deltaStep", "deltaStepBarrier"),
+             ("bfs-directionopt", [], "SyncDO", "Async"),
+             ("betweennesscentrality", ["-numOfSources=2"], "Async", "Level")]
+    for app, extra, pl, other in cases:
+        for path, exp in ((d / "rmat12.sgr", pl), (tmp_path / "grid.sgr", other)):
+            r = run(app, *extra, path)
+            assert r.returncode == 0, r.stderr
+            assert f"INFO: Choosing {exp} algorithm" in r.stdout, (app, path)

@@ -119,6 +119,58 @@ int main(){ std::mt19937 rng(27491095u); std::uniform_int_distribution<uint32_t>
     assert list(got) == expect
 
 
+def _power_law_restated(row):
+    """The AutoAlgo degree test of Utils.h:37-57 restated from its text, on
+    the vertices the source picker draws (pinned above)."""
+    import galois_amd as gx
+    n, m = len(row) - 1, int(row[-1])
+    if m // n < 10:
+        return False
+    k = min(1000, n)
+    deg = np.sort(np.diff(row)[gx.pick_sources(row, k)].astype(np.uint64))
+    total = int(deg.sum()) & 0xFFFFFFFF  # uint32 sum, as the reference
+    return total / k / 1.25 > float(deg[k // 2])
+
+
+def test_degree_power_law_choice():
+    """gx_degree_power_law picks the reference's AutoAlgo branch: a graph of
+    average degree below 10 never counts; a regular graph (mean = median) does
+    not; a skewed one does; and random degree sequences agree with the
+    restatement, including sequences near the mean / 1.25 = median edge."""
+    import galois_amd as gx
+    from galois_amd import _lib
+
+    def row_of(deg):
+        return np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+
+    assert not gx.degree_power_law(row_of(np.full(2000, 9)))
+    assert not gx.degree_power_law(row_of(np.full(2000, 12)))
+    assert gx.degree_power_law(row_of(np.r_[np.full(1800, 2), np.full(200, 200)]))
+    assert not gx.degree_power_law(row_of(np.r_[np.full(1800, 2), np.full(200, 20)]))  # average 3
+    # two vertices, both drawn: sorted degrees (10, 30), median = element 1
+    # = 30 > 20 / 1.25 (element 0 as the median would say power law)
+    assert list(gx.pick_sources(row_of([10, 30]), 2)) == [0, 1]
+    assert not gx.degree_power_law(row_of([10, 30]))
+    rng = np.random.default_rng(17)
+    seen = set()
+    for trial in range(40):
+        n = int(rng.integers(50, 5000))
+        base = int(rng.integers(8, 30))
+        # a median near base and a tail whose weight moves the mean across
+        # 1.25 x median
+        tail = rng.random(n) < rng.uniform(0.0, 0.3)
+        deg = np.where(tail, rng.integers(base, 6 * base, n), rng.integers(base - 2, base + 3, n))
+        deg[rng.integers(0, n)] = max(deg.max(), 1)
+        row = row_of(deg)
+        got = gx.degree_power_law(row)
+        assert got == _power_law_restated(row), trial
+        seen.add(got)
+    assert seen == {True, False}
+    out = C.c_int()
+    bad = np.array([0, 5, 7], np.int32)
+    assert _lib.lib().gx_degree_power_law(2, 6, bad.ctypes.data, C.byref(out)) == _lib.GX_ERR_INVALID
+
+
 def test_no_device_reported_on_cpu_host():
     import torch
     if torch.cuda.is_available():

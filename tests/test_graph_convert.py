@@ -103,6 +103,51 @@ def test_random_weights(tmp_path):
     assert (tmp_path / "w.gr").read_bytes() == (tmp_path / "w2.gr").read_bytes()  # fixed default seed
 
 
+def _mt19937_weights(tmp_path, m, kind, lo, hi):
+    """m draws of std::mt19937 (default seed) through uniform_int_distribution
+    <uint32_t>(lo, hi) or uniform_real_distribution<float>(lo, hi), the recipe
+    of RandomizeEdgeWeights (graph-convert.cpp:1144-1192), from a standalone
+    build of exactly that standard-library code."""
+    dist = ("std::uniform_int_distribution<uint32_t>" if kind == "uint32"
+            else "std::uniform_real_distribution<float>")
+    src = f"""
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <random>
+int main() {{
+  std::mt19937 gen;
+  {dist} d({lo}, {hi});
+  for (long i = 0; i < {m}L; ++i) {{
+    auto v = d(gen);
+    uint32_t bits;
+    std::memcpy(&bits, &v, 4);
+    std::printf("%u\\n", bits);
+  }}
+}}
+"""
+    cpp, exe = tmp_path / f"mt_{kind}.cpp", tmp_path / f"mt_{kind}"
+    cpp.write_text(src)
+    subprocess.run(["g++", "-O1", str(cpp), "-o", str(exe)], check=True)
+    out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    return np.array([int(x) for x in out], np.uint64).astype(np.uint32)
+
+
+@pytest.mark.parametrize("kind,lo,hi", [("uint32", 1, 255), ("uint32", 1, 1000), ("float32", 0, 1)])
+def test_random_weights_follow_mt19937(tmp_path, kind, lo, hi):
+    """The weights equal the standard-library draws of the reference's recipe
+    in CSR edge order, bit for bit."""
+    import galois_amd as gx
+    rng = np.random.default_rng(8)
+    row, col, _ = random_graph(rng, n=200, m=5000)
+    gx.write_gr(str(tmp_path / "in.gr"), row, col)
+    conv("-gr2randomweightgr", f"-edgeType={kind}", f"-minValue={lo}", f"-maxValue={hi}", tmp_path / "in.gr",
+         tmp_path / "w.gr")
+    _, wcol, wdat, esize = gx.read_gr(str(tmp_path / "w.gr"))
+    assert esize == 4 and (wcol == col).all()
+    np.testing.assert_array_equal(wdat.view(np.uint32), _mt19937_weights(tmp_path, len(col), kind, lo, hi))
+
+
 def test_converter_errors(tmp_path):
     r = subprocess.run([str(TOOL), "-gr2edgelist", str(tmp_path / "missing.gr"), str(tmp_path / "x")],
                        capture_output=True, text=True)
