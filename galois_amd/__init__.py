@@ -76,6 +76,14 @@ def tune_get(key: str) -> int:
     return v.value
 
 
+def tune_effective(key: str, n: int, m: int) -> int:
+    """Value of a knob that an entry point uses on a graph of n vertices and m
+    edges: the value set, else the built-in default for that size."""
+    v = C.c_int64()
+    check(lib().gx_tune_effective(key.encode(), int(n), int(m), C.byref(v)))
+    return v.value
+
+
 class Graph:
     """A CSR graph resident in device memory (owns a gx_graph handle)."""
 
@@ -316,8 +324,9 @@ def kcore(g: Graph, k: int):
 
 
 def pagerank(g: Graph, alpha: float = 0.85, tol: float = 1e-9, max_iter: int = 100, precision: str = "fp32"):
-    """Normalised pull PageRank; precision 'fp32' (exact fixed-point row sums in the default blocked sweep,
-    Kahan sums with pr.blocking = 0) or 'fp64'. Returns (pr, Stats)."""
+    """Normalised pull PageRank; precision 'fp32' or 'fp64'.  FP32 row sums are exact 64-bit fixed-point
+    sums in the default propagation-blocked sweep; the merge-path pull (pr.blocking = 0) uses Kahan sums.
+    Returns (pr, Stats)."""
     torch = _torch()
     fp64 = precision == "fp64"
     pr = _empty(g.n, torch.float64 if fp64 else torch.float32)

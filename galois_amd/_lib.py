@@ -82,6 +82,7 @@ SIGNATURES: dict[str, tuple] = {
     "gx_set_profiling": (None, [INT]),
     "gx_tune_set": (INT, [C.c_char_p, C.c_int64]),
     "gx_tune_get": (INT, [C.c_char_p, C.POINTER(C.c_int64)]),
+    "gx_tune_effective": (INT, [C.c_char_p, C.c_int64, C.c_int64, C.POINTER(C.c_int64)]),
     "gx_graph_create": (INT, [I64, I64, P, P, P, U32, PG]),
     "gx_graph_create_device": (INT, [I64, I64, P, P, P, U32, PG]),
     "gx_graph_free": (INT, [P]),

@@ -47,8 +47,9 @@ KNOBS: dict[str, list[str]] = {
 DEFAULT_CACHE = Path(os.environ.get("XDG_CACHE_HOME", Path.home() / ".cache")) / "galois_amd" / "tune.json"
 
 
-def signature(gx, G, algo: str) -> str:
-    """Coarse workload key: algorithm, device, log2 n, log2 avg/max degree."""
+def signature(gx, G, algo: str, args: dict | None = None) -> str:
+    """Coarse workload key: algorithm, device, log2 n, log2 avg/max degree,
+    symmetry, and the arguments that pick a different kernel (precision)."""
     import torch
     row, _, _ = G.to_host() if G.m < (1 << 24) else (None, None, None)
     if row is not None:
@@ -58,8 +59,24 @@ def signature(gx, G, algo: str) -> str:
         dmax = -1  # large graphs: degree extremes are not downloaded
     dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
     sym = bool(G.flags & 1)
+    variant = (args or {}).get("precision", "fp32") if algo == "pagerank" else ""
     return json.dumps([algo, dev, round(math.log2(max(G.n, 1))), round(math.log2(max(G.m / max(G.n, 1), 1))),
-                       -1 if dmax < 0 else round(math.log2(max(dmax, 1))), sym])
+                       -1 if dmax < 0 else round(math.log2(max(dmax, 1))), sym, variant])
+
+
+def _applies(knob: str, algo: str, args: dict, choice: dict) -> bool:
+    """Whether the kernel that runs responds to `knob`, given the choices made
+    so far.  PageRank: pr.blocking candidates (1, 0) differ only for FP32 (FP64
+    takes the merge-path pull under both); pr.hot_sources splits the
+    merge-path pull only, so it is not timed while the blocked sweep runs."""
+    if algo != "pagerank":
+        return True
+    fp64 = args.get("precision", "fp32") == "fp64"
+    if knob == "pr.blocking":
+        return not fp64
+    if knob == "pr.hot_sources":
+        return fp64 or choice.get("pr.blocking", 1) == 0
+    return True
 
 
 def _runner(gx, G, algo: str, args: dict):
@@ -95,7 +112,7 @@ def tune(G, algo: str, repeats: int = 2, cache: Path | None = DEFAULT_CACHE, ret
     values set.  Timing uses profiling mode (HIP events per dominant launch)."""
     import galois_amd as gx
     knobs = KNOBS[algo]
-    key = signature(gx, G, algo)
+    key = signature(gx, G, algo, args)
     table = {}
     if cache is not None and cache.exists():
         try:
@@ -112,6 +129,11 @@ def tune(G, algo: str, repeats: int = 2, cache: Path | None = DEFAULT_CACHE, ret
     try:
         run()  # warm-up: plans, transposes, captured graphs
         for knob in knobs:
+            if not _applies(knob, algo, args, choice):
+                gx.tune_set(knob, -1)  # the built-in default; the kernel that runs ignores it
+                choice[knob] = -1
+                timings[knob] = "not timed: the chosen kernel does not read this knob"
+                continue
             best_v, best_t = None, math.inf
             timings[knob] = {}
             for v in CANDIDATES[knob]:

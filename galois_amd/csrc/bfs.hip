@@ -362,7 +362,7 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   // more than they save (RMAT-18 0.455 -> 0.52 ms, RMAT-22 2.03 -> 2.21 ms
   // with it; RMAT-26 33.9 -> 24.7 ms).
   const int64_t words = (n + 31) / 32;
-  const bool use_vis = tune_value("bfs.visited_bitmap", n >= (int64_t(1) << 25) ? 1 : 0) != 0;
+  const bool use_vis = tune_effective("bfs.visited_bitmap", n, g->m) != 0;
   uint32_t* vis = use_vis ? g->scratch<uint32_t>(WS_BITMAP0, words) : nullptr;
   Bufs B{g->row, g->col, level, nullptr, {g->scratch<uint32_t>(WS_Q0, n), g->scratch<uint32_t>(WS_Q1, n)},
          g->scratch<uint32_t>(WS_Q2, n), g->scratch<BfsState>(WS_COUNTERS, 1), cap,

@@ -38,17 +38,23 @@ extern "C" void gx_set_profiling(int on) { g_profiling.store(on ? 1 : 0); }
 
 namespace gx {
 namespace {
+// Built-in defaults, which may depend on the graph's size.
+int64_t default_hub_degree(int64_t, int64_t) { return 128; }
+int64_t default_hot_sources(int64_t, int64_t) { return 0; }
+int64_t default_blocking(int64_t, int64_t) { return 1; }
+int64_t default_visited_bitmap(int64_t n, int64_t) { return n >= (int64_t(1) << 25) ? 1 : 0; }
 struct Knob {
   const char* key;
   std::atomic<int64_t> value;
   int64_t lo, hi;
+  int64_t (*dflt)(int64_t n, int64_t m);
 };
 Knob g_knobs[] = {
-    {"frontier.hub_degree", {-1}, 64, int64_t(1) << 30},
-    {"pr.hot_sources", {-1}, 0, int64_t(1) << 31},
-    {"pr.blocking", {-1}, 0, 2},
-    {"pr.task_edges", {-1}, int64_t(1) << 12, int64_t(1) << 30},
-    {"bfs.visited_bitmap", {-1}, 0, 1},
+    {"frontier.hub_degree", {-1}, 64, int64_t(1) << 30, default_hub_degree},
+    {"pr.hot_sources", {-1}, 0, int64_t(1) << 31, default_hot_sources},
+    {"pr.blocking", {-1}, 0, 2, default_blocking},
+    {"pr.task_edges", {-1}, int64_t(1) << 12, int64_t(1) << 30, pr_task_edges_default},
+    {"bfs.visited_bitmap", {-1}, 0, 1, default_visited_bitmap},
 };
 Knob* find_knob(const char* key) {
   if (!key) return nullptr;
@@ -58,10 +64,16 @@ Knob* find_knob(const char* key) {
 }
 }  // namespace
 
-int64_t tune_value(const char* key, int64_t dflt) {
+int64_t pr_task_edges_default(int64_t, int64_t m) {
+  constexpr int64_t lo = int64_t(1) << 19, hi = int64_t(1) << 20;
+  return std::min(std::max(m >> 10, lo), hi);
+}
+
+int64_t tune_effective(const char* key, int64_t n, int64_t m) {
   Knob* k = find_knob(key);
-  const int64_t v = k ? k->value.load() : -1;
-  return v < 0 ? dflt : v;
+  if (!k) return -1;
+  const int64_t v = k->value.load();
+  return v < 0 ? k->dflt(n, m) : v;
 }
 }  // namespace gx
 
@@ -80,6 +92,16 @@ extern "C" gx_status gx_tune_get(const char* key, int64_t* value) {
   GX_REQUIRE(k, std::string("unknown tuning key ") + (key ? key : "(null)"));
   check_ptr(value, "value");
   *value = k->value.load();
+  GX_API_END
+}
+
+extern "C" gx_status gx_tune_effective(const char* key, int64_t n, int64_t m, int64_t* value) {
+  GX_API_BEGIN
+  Knob* k = find_knob(key);
+  GX_REQUIRE(k, std::string("unknown tuning key ") + (key ? key : "(null)"));
+  check_ptr(value, "value");
+  GX_REQUIRE(n >= 0 && m >= 0, "negative graph size");
+  *value = tune_effective(key, n, m);
   GX_API_END
 }
 
@@ -145,6 +167,18 @@ extern "C" gx_status gx_gr_read(const char* path, gx_host_csr* out) {
   if (n >= (uint64_t(1) << 31) || m >= (uint64_t(1) << 31))
     fail(GX_ERR_IO, "graph too large for the int32 CSR layout");
   if (esz > 8) fail(GX_ERR_IO, "edge datum size " + std::to_string(esz) + " unsupported");
+  {
+    // The header sizes every allocation below: a file shorter than the
+    // sections its header announces is rejected before anything is allocated.
+    const uint64_t dsz = ver == 1 ? 4 : 8;
+    const uint64_t need = sizeof(hdr) + 8 * n + dsz * (m + (m % 2)) + esz * m;
+    if (std::fseek(f.get(), 0, SEEK_END) != 0) fail(GX_ERR_IO, std::string("cannot seek in ") + path);
+    const long have = std::ftell(f.get());
+    if (have < 0 || static_cast<uint64_t>(have) < need)
+      fail(GX_ERR_IO, "file is shorter than its header announces (" + std::to_string(have < 0 ? 0 : have) +
+                          " of " + std::to_string(need) + " bytes)");
+    if (std::fseek(f.get(), sizeof(hdr), SEEK_SET) != 0) fail(GX_ERR_IO, std::string("cannot seek in ") + path);
+  }
   gx_host_csr g{};
   g.n = static_cast<int64_t>(n);
   g.m = static_cast<int64_t>(m);

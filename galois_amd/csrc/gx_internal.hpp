@@ -230,13 +230,16 @@ struct gx_graph {
 namespace gx {
 
 bool profiling_enabled();
-// Current value of a tuning knob (gx_tune_set), `dflt` when unset.
-int64_t tune_value(const char* key, int64_t dflt);
+// Value of a tuning knob (gx_tune_set) for a graph of n vertices and m
+// edges: the value set, or the built-in default for that size when unset.
+int64_t tune_effective(const char* key, int64_t n, int64_t m);
+// Default of "pr.task_edges": m / 1024 within [2^19, 2^20].
+int64_t pr_task_edges_default(int64_t n, int64_t m);
 // Degree above which a frontier vertex is expanded by the whole grid.
 // Frontier vertices of higher degree are expanded by the whole grid (default
 // 128: a workgroup batch of 256 frontier vertices then holds at most 32K
 // edges, so short power-law frontiers still spread over the grid).
-inline uint32_t hub_degree() { return static_cast<uint32_t>(tune_value("frontier.hub_degree", 128)); }
+inline uint32_t hub_degree() { return static_cast<uint32_t>(tune_effective("frontier.hub_degree", 0, 0)); }
 
 // Accumulates the device time of the launches of one kernel with HIP events
 // (enabled by gx_set_profiling).  Events are recorded on the graph's stream,

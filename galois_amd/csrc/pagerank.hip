@@ -369,7 +369,7 @@ static void build_split(gx_graph* g, int64_t hot) {
 // pass, which measured faster on RMAT-26; the autotuner may pick a split).
 static int64_t hot_sources(int64_t n, size_t real_bytes) {
   (void)real_bytes;
-  const int64_t h = tune_value("pr.hot_sources", 0);
+  const int64_t h = tune_effective("pr.hot_sources", n, 0);
   return (h <= 0 || h >= n) ? 0 : h;
 }
 
@@ -378,7 +378,7 @@ static int64_t hot_sources(int64_t n, size_t real_bytes) {
 // bit), 0 = merge-path pull for both.  Graphs outside the layout's table
 // limits always use the pull.
 static bool use_blocking(const gx_graph* g, size_t real_bytes) {
-  const int64_t k = tune_value("pr.blocking", 1);
+  const int64_t k = tune_effective("pr.blocking", g->n, g->m);
   const bool on = real_bytes == sizeof(float) ? k != 0 : k == 2;
   return on && pb::supported(g->n, g->m);
 }

@@ -275,11 +275,11 @@ void build_layout(gx_graph* g) {
   // Tasks: a bin with at most `task_entries` edges (knob "pr.task_edges")
   // is one task; a heavier bin is cut into slices of about that many (long,
   // then mid, then short jobs) that share a global accumulator slot.  The
-  // default, m / 1024 within [kTaskEntries, 2 kTaskEntries], grows the tasks
-  // of graphs with more than 2^29 edges only, which still get about four
-  // tasks per CU.
-  const int64_t task_default = std::min(std::max(m >> 10, kTaskEntries), 2 * kTaskEntries);
-  const uint64_t task_entries = static_cast<uint64_t>(tune_value("pr.task_edges", task_default));
+  // default, m / 1024 within [2^19, 2^20] (pr_task_edges_default), grows the
+  // tasks of graphs with more than 2^29 edges only, which still get about
+  // four tasks per CU.  A slice closes at the first job that reaches the
+  // limit, so it holds up to kJob - 1 edges more.
+  const uint64_t task_entries = static_cast<uint64_t>(tune_effective("pr.task_edges", n, m));
   std::vector<std::pair<uint64_t, Task>> tasks;
   std::vector<Multi> multi;
   for (int64_t b = 0; b < nb; ++b) {

@@ -65,7 +65,6 @@ constexpr int kWaves = kBlock / kWave;
 constexpr int kJob = 512;            // entries per long job: 8 per lane
 constexpr int kMid = 64;             // mid jobs: 8 lanes x 8 entries
 constexpr int kShort = 8;            // short jobs: 1 lane x 8 entries
-constexpr int64_t kTaskEntries = int64_t(1) << 19;   // bins above this are sliced (up to 2x on large graphs)
 constexpr int64_t kPieceEntries = int64_t(1) << 18;  // phase-1 piece length (groups)
 constexpr int64_t kMaxKeys = int64_t(1) << 28;       // chunk x bin table limit
 constexpr uint32_t kSingle = 0xffffffffu;            // task finalises its bin itself

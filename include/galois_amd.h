@@ -125,9 +125,15 @@ void gx_set_profiling(int on);
  *   "bfs.visited_bitmap"   gx_bfs: 1 = test a visited bitmap before the
  *                          level word of every edge's target, 0 = not; the
  *                          default uses it from 2^25 vertices on
- * Unknown keys and out-of-range values are GX_ERR_INVALID. */
+ * Unknown keys and out-of-range values are GX_ERR_INVALID.
+ * gx_tune_get returns the value set (-1 when unset); gx_tune_effective the
+ * value an entry point uses on a graph of n vertices and m edges: the value
+ * set, else the built-in default for that size.  A slice may exceed
+ * "pr.task_edges" by up to 511 edges: it is closed at the first job that
+ * reaches the limit, and a job holds at most 512 edges. */
 gx_status gx_tune_set(const char* key, int64_t value);
 gx_status gx_tune_get(const char* key, int64_t* value);
+gx_status gx_tune_effective(const char* key, int64_t n, int64_t m, int64_t* value);
 
 /* ---------------------------------------------------------- graph lifecycle
  * Replaces CSRGraph::copy_to_gpu (libgpu/src/csr_graph.cu:274-293) and the

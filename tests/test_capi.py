@@ -82,6 +82,22 @@ def test_gr_errors(tmp_path):
     assert e.value.code == _lib.GX_ERR_IO
     with pytest.raises(_lib.GxError):
         gx.read_gr(tmp_path / "missing.gr")
+    # a few-byte file whose header announces 2^30 vertices and edges is
+    # rejected before its sections are allocated
+    p.write_bytes(np.array([1, 4, 1 << 30, 1 << 30], "<u8").tobytes() + np.zeros(2, "<u8").tobytes())
+    with pytest.raises(_lib.GxError) as e:
+        gx.read_gr(p)
+    assert e.value.code == _lib.GX_ERR_IO and "shorter than its header" in e.value.message
+    # one byte short of a valid odd-m v1 file with 4-byte edge data
+    n, m = 2, 3
+    good = (np.array([1, 4, n, m], "<u8").tobytes() + np.array([2, 3], "<u8").tobytes()
+            + np.array([1, 0, 0, 0], "<u4").tobytes() + np.array([7, 8, 9], "<u4").tobytes())
+    p.write_bytes(good)
+    assert gx.read_gr(p)[3] == 4
+    p.write_bytes(good[:-1])
+    with pytest.raises(_lib.GxError) as e:
+        gx.read_gr(p)
+    assert "shorter than its header" in e.value.message
 
 
 def test_null_arguments_are_rejected():
@@ -200,6 +216,21 @@ def test_tuning_knobs():
     gx.tune_set("pr.task_edges", -1)
     with pytest.raises(GxError):
         gx.tune_set("pr.task_edges", 1000)  # below 2^12
+    # resolved values: the task size default is m / 1024 clamped to
+    # [2^19, 2^20]; the visited bitmap is on from 2^25 vertices
+    for m, want in [(0, 1 << 19), (1 << 28, 1 << 19), ((1 << 29) - 1, 1 << 19), (1 << 29, 1 << 19),
+                    ((1 << 29) + (1 << 12), (1 << 19) + 4), (3 << 28, 3 << 18), (1 << 30, 1 << 20),
+                    ((1 << 31) - 1, 1 << 20)]:
+        assert gx.tune_effective("pr.task_edges", 1 << 20, m) == want, m
+    gx.tune_set("pr.task_edges", 1 << 13)
+    assert gx.tune_effective("pr.task_edges", 1 << 20, 1 << 30) == 1 << 13
+    gx.tune_set("pr.task_edges", -1)
+    assert gx.tune_effective("bfs.visited_bitmap", (1 << 25) - 1, 0) == 0
+    assert gx.tune_effective("bfs.visited_bitmap", 1 << 25, 0) == 1
+    assert gx.tune_effective("frontier.hub_degree", 1, 1) == 128
+    assert gx.tune_effective("pr.blocking", 1, 1) == 1
+    with pytest.raises(GxError):
+        gx.tune_effective("no.such.knob", 1, 1)
     with pytest.raises(GxError):
         gx.tune_set("no.such.knob", 1)
     with pytest.raises(GxError):
