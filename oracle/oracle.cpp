@@ -548,23 +548,38 @@ int oracle_pagerank_push_f32(int64_t n, const int32_t* row, const int32_t* col, 
 
 // Differentiable PageRank, FP64: P[u,v] = w_uv / sum_v' w_uv', dangling rows
 // uniform, x = alpha*P^T x + alpha*(D/n) + (1-alpha)*t with D the dangling
-// mass; Jacobi to sum|x'-x| < tol or max_iter.  Pinned by a dense
-// numpy solve on small graphs.
-int oracle_pagerank_weighted_fp64(int64_t n, const int32_t* row, const int32_t* col, const double* w,
-                                  const double* t, double alpha, double tol, uint32_t max_iter, double* x_out,
-                                  uint32_t* iters_out) {
+// mass; Jacobi to sum|x'-x| < tol or max_iter.  The O(m) part runs in
+// parallel as a pull over the in-edges, whose sources are ascending, so every
+// x'[v] adds its contributions in the order of the serial push over u: the
+// result does not depend on the thread count.  Scalar sums (dangling mass,
+// residual) run serially in vertex order.  Pinned by a dense numpy solve on
+// small graphs.
+int oracle_pagerank_weighted_fp64_mt(int64_t n, const int32_t* row, const int32_t* col, const double* w,
+                                     const double* t, double alpha, double tol, uint32_t max_iter, double* x_out,
+                                     uint32_t* iters_out, int threads) {
+  Pool pool(threads);
   std::vector<double> rs(n, 0.0);
-  for (int64_t u = 0; u < n; ++u)
-    for (int32_t e = row[u]; e < row[u + 1]; ++e) rs[u] += w[e];
-  std::vector<double> x(n, 1.0 / n), xn(n);
+  do_all(pool, 0, n, 1024, [&](int64_t u, int) {
+    double s = 0;
+    for (int32_t e = row[u]; e < row[u + 1]; ++e) s += w[e];
+    rs[u] = s;
+  });
+  std::vector<int32_t> trow, tcol, perm;
+  transpose(n, row, col, trow, tcol, &perm);
+  std::vector<double> x(n, 1.0 / n), xn(n), y(n);
   uint32_t it = 0;
   while (it < max_iter) {
     double dang = 0;
     for (int64_t u = 0; u < n; ++u)
       if (row[u + 1] == row[u]) dang += x[u];
-    for (int64_t v = 0; v < n; ++v) xn[v] = (1.0 - alpha) * t[v] + alpha * dang / n;
-    for (int64_t u = 0; u < n; ++u)
-      for (int32_t e = row[u]; e < row[u + 1]; ++e) xn[col[e]] += alpha * x[u] * w[e] / rs[u];
+    do_all(pool, 0, n, 1024, [&](int64_t v, int) {
+      double a = (1.0 - alpha) * t[v] + alpha * dang / n;
+      for (int32_t k = trow[v]; k < trow[v + 1]; ++k) {
+        const int32_t u = tcol[k];
+        a += alpha * x[u] * w[perm[k]] / rs[u];
+      }
+      xn[v] = a;
+    });
     double r = 0;
     for (int64_t v = 0; v < n; ++v) r += std::fabs(xn[v] - x[v]);
     x.swap(xn);
@@ -576,23 +591,34 @@ int oracle_pagerank_weighted_fp64(int64_t n, const int32_t* row, const int32_t* 
   return 0;
 }
 
+int oracle_pagerank_weighted_fp64(int64_t n, const int32_t* row, const int32_t* col, const double* w,
+                                  const double* t, double alpha, double tol, uint32_t max_iter, double* x_out,
+                                  uint32_t* iters_out) {
+  return oracle_pagerank_weighted_fp64_mt(n, row, col, w, t, alpha, tol, max_iter, x_out, iters_out, 1);
+}
+
 // Adjoint of oracle_pagerank_weighted_fp64 for L = c.x: lambda solves
 // lambda = alpha*(P lambda + (1/n) sum(lambda) on dangling rows) + c;
 // dL/dt = (1-alpha)*lambda; dL/dw_uv = alpha*x_u*(lambda_v - (P lambda)_u)/s_u
-// for a non-dangling row u with weight sum s_u.  Pinned by central finite
-// differences of oracle_pagerank_weighted_fp64 in tests.
-int oracle_pagerank_adjoint_fp64(int64_t n, const int32_t* row, const int32_t* col, const double* w,
-                                 const double* x, const double* c, double alpha, double tol, uint32_t max_iter,
-                                 double* grad_w, double* grad_t) {
+// for a non-dangling row u with weight sum s_u.  Rows are summed in edge
+// order in parallel; scalar sums run serially in vertex order, so the result
+// does not depend on the thread count.  Pinned by central finite differences
+// of oracle_pagerank_weighted_fp64 in tests.
+int oracle_pagerank_adjoint_fp64_mt(int64_t n, const int32_t* row, const int32_t* col, const double* w,
+                                    const double* x, const double* c, double alpha, double tol, uint32_t max_iter,
+                                    double* grad_w, double* grad_t, int threads) {
+  Pool pool(threads);
   std::vector<double> rs(n, 0.0);
-  for (int64_t u = 0; u < n; ++u)
-    for (int32_t e = row[u]; e < row[u + 1]; ++e) rs[u] += w[e];
+  do_all(pool, 0, n, 1024, [&](int64_t u, int) {
+    double s = 0;
+    for (int32_t e = row[u]; e < row[u + 1]; ++e) s += w[e];
+    rs[u] = s;
+  });
   std::vector<double> lam(c, c + n), ln(n), plam(n);
   for (uint32_t it = 0; it < max_iter; ++it) {
     double tot = 0;
     for (int64_t v = 0; v < n; ++v) tot += lam[v];
-    double r = 0;
-    for (int64_t u = 0; u < n; ++u) {
+    do_all(pool, 0, n, 1024, [&](int64_t u, int) {
       double s;
       if (row[u + 1] == row[u]) {
         s = tot / n;
@@ -602,21 +628,26 @@ int oracle_pagerank_adjoint_fp64(int64_t n, const int32_t* row, const int32_t* c
         s /= rs[u];
       }
       ln[u] = c[u] + alpha * s;
-      r += std::fabs(ln[u] - lam[u]);
-    }
+    });
+    double r = 0;
+    for (int64_t u = 0; u < n; ++u) r += std::fabs(ln[u] - lam[u]);
     lam.swap(ln);
     if (r < tol) break;
   }
-  for (int64_t u = 0; u < n; ++u) {
+  do_all(pool, 0, n, 1024, [&](int64_t u, int) {
     double s = 0;
     for (int32_t e = row[u]; e < row[u + 1]; ++e) s += w[e] * lam[col[e]];
     plam[u] = (row[u + 1] > row[u]) ? s / rs[u] : 0.0;
-  }
-  for (int64_t u = 0; u < n; ++u)
-    for (int32_t e = row[u]; e < row[u + 1]; ++e)
-      grad_w[e] = alpha * x[u] * (lam[col[e]] - plam[u]) / rs[u];
+    for (int32_t e = row[u]; e < row[u + 1]; ++e) grad_w[e] = alpha * x[u] * (lam[col[e]] - plam[u]) / rs[u];
+  });
   for (int64_t v = 0; v < n; ++v) grad_t[v] = (1.0 - alpha) * lam[v];
   return 0;
+}
+
+int oracle_pagerank_adjoint_fp64(int64_t n, const int32_t* row, const int32_t* col, const double* w,
+                                 const double* x, const double* c, double alpha, double tol, uint32_t max_iter,
+                                 double* grad_w, double* grad_t) {
+  return oracle_pagerank_adjoint_fp64_mt(n, row, col, w, x, c, alpha, tol, max_iter, grad_w, grad_t, 1);
 }
 
 // ------------------------------------------------------------------- BC

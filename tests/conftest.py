@@ -55,6 +55,8 @@ class Oracle:
             "oracle_pagerank_push_f32": [I64, P, P, F, F, U32, P, P],
             "oracle_pagerank_weighted_fp64": [I64, P, P, P, P, D, D, U32, P, P],
             "oracle_pagerank_adjoint_fp64": [I64, P, P, P, P, P, D, D, U32, P, P],
+            "oracle_pagerank_weighted_fp64_mt": [I64, P, P, P, P, D, D, U32, P, P, INT],
+            "oracle_pagerank_adjoint_fp64_mt": [I64, P, P, P, P, P, D, D, U32, P, P, INT],
             "oracle_bc": [I64, P, P, P, U32, P, INT],
             "oracle_transpose": [I64, P, P, P, P, P],
         }
@@ -126,26 +128,36 @@ class Oracle:
                                                  self._p(x), C.byref(it)) == 0
         return x, it.value
 
-    def pagerank_weighted(self, row, col, w, t, alpha=0.85, tol=1e-12, max_iter=1000):
+    def pagerank_weighted(self, row, col, w, t, alpha=0.85, tol=1e-12, max_iter=1000, threads=1):
         n = len(row) - 1
         x = np.empty(n, np.float64)
         it = C.c_uint32()
         w = np.ascontiguousarray(w, np.float64)
         t = np.ascontiguousarray(t, np.float64)
-        assert self.lib.oracle_pagerank_weighted_fp64(n, self._p(row), self._p(col), self._p(w), self._p(t), alpha,
-                                                      tol, max_iter, self._p(x), C.byref(it)) == 0
+        if threads == 1:
+            assert self.lib.oracle_pagerank_weighted_fp64(n, self._p(row), self._p(col), self._p(w), self._p(t),
+                                                          alpha, tol, max_iter, self._p(x), C.byref(it)) == 0
+        else:
+            assert self.lib.oracle_pagerank_weighted_fp64_mt(n, self._p(row), self._p(col), self._p(w), self._p(t),
+                                                             alpha, tol, max_iter, self._p(x), C.byref(it),
+                                                             threads) == 0
         return x
 
-    def pagerank_adjoint(self, row, col, w, x, c, alpha=0.85, tol=1e-12, max_iter=1000):
+    def pagerank_adjoint(self, row, col, w, x, c, alpha=0.85, tol=1e-12, max_iter=1000, threads=1):
         n, m = len(row) - 1, len(col)
         gw = np.empty(max(m, 1), np.float64)
         gt = np.empty(n, np.float64)
         w = np.ascontiguousarray(w, np.float64)
         x = np.ascontiguousarray(x, np.float64)
         c = np.ascontiguousarray(c, np.float64)
-        assert self.lib.oracle_pagerank_adjoint_fp64(n, self._p(row), self._p(col), self._p(w), self._p(x),
-                                                     self._p(c), alpha, tol, max_iter, self._p(gw),
-                                                     self._p(gt)) == 0
+        if threads == 1:
+            assert self.lib.oracle_pagerank_adjoint_fp64(n, self._p(row), self._p(col), self._p(w), self._p(x),
+                                                         self._p(c), alpha, tol, max_iter, self._p(gw),
+                                                         self._p(gt)) == 0
+        else:
+            assert self.lib.oracle_pagerank_adjoint_fp64_mt(n, self._p(row), self._p(col), self._p(w), self._p(x),
+                                                            self._p(c), alpha, tol, max_iter, self._p(gw),
+                                                            self._p(gt), threads) == 0
         return gw[:m], gt
 
     def bc(self, row, col, sources, threads=4):

@@ -147,6 +147,30 @@ def test_weighted_pagerank_and_adjoint(oracle, golden):
         np.testing.assert_allclose(gt, g["grad_t"], rtol=1e-5, atol=1e-8)
 
 
+def test_weighted_pagerank_threads_bit_identical(oracle):
+    """The threaded forward and adjoint (pull over ascending sources, scalar
+    sums serial) equal the serial restatement bit for bit, so the fixtures that
+    pin the serial one pin them too."""
+    rng = np.random.default_rng(7)
+    n = 3000
+    deg = rng.integers(0, 12, n)
+    deg[rng.integers(0, n, 40)] = 0  # dangling rows
+    row = np.concatenate([[0], np.cumsum(deg)]).astype(np.int32)
+    col = rng.integers(0, n, row[-1]).astype(np.int32)
+    col[::97] = col[::97] // 2  # some duplicate targets inside rows
+    w = rng.lognormal(0, 1, row[-1])
+    t = rng.random(n)
+    t /= t.sum()
+    c = rng.standard_normal(n)
+    x1 = oracle.pagerank_weighted(row, col, w, t, tol=1e-13, max_iter=300)
+    x4 = oracle.pagerank_weighted(row, col, w, t, tol=1e-13, max_iter=300, threads=4)
+    np.testing.assert_array_equal(x1, x4)
+    g1 = oracle.pagerank_adjoint(row, col, w, x1, c, tol=1e-13, max_iter=300)
+    g4 = oracle.pagerank_adjoint(row, col, w, x1, c, tol=1e-13, max_iter=300, threads=4)
+    np.testing.assert_array_equal(g1[0], g4[0])
+    np.testing.assert_array_equal(g1[1], g4[1])
+
+
 def test_transpose_order(oracle, golden):
     for g in _cases(golden) + golden["directed"]:
         row, col, _ = csr_arrays(g)
