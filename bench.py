@@ -18,16 +18,18 @@ Also reported, one line of JSON in total:
                 g = (source, destination bin) groups) and measured HBM
                 traffic from the committed rocprofv3 summary;
   cpu_baseline  the CPU restatement (oracle, FP64 pull) timed on this host's
-                cores on a bounded sample (a few PageRank iterations on the
-                same graph);
+                cores on a bounded sample: 3 PageRank iterations on the same
+                graph, median of 5 after a warm-up;
   kernels       GTEPS and %HBM of every kernel on the other workloads (BFS on
                 symmetric RMAT-18; SSSP/CC/k-core on symmetric RMAT-22 and the
                 2048x2048 road-like grid; batched vs one-source-per-pass BC,
                 on RMAT-22 and on 64 uneven graphs with 4 and with 64
-                sources per graph;
-                adjoint vs forward PageRank; the D-IrGL personality loops;
-                BFS/SSSP/CC/k-core on symmetric RMAT-26), plus the
-                FP32-vs-FP64 PageRank error.
+                sources per graph, with the autotuner's batched-BC choices;
+                batched vs one-source SSSP; adjoint vs forward PageRank; the
+                D-IrGL personality loops; BFS/SSSP/CC/k-core on symmetric
+                RMAT-26), each beside the oracle's time on this host's cores
+                (cpu_ms: median after a warm-up) and whether the outputs are
+                equal, plus the PageRank error against the CPU FP64 solve.
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
 from __future__ import annotations
@@ -72,9 +74,40 @@ def measured_traffic(scale: int, sweep: str):
 
 
 # ------------------------------------------------------------ CPU baseline
-def cpu_baseline_pagerank(G, iters: int) -> dict:
+CPU_THREADS = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+def oracle():
+    """The CPU restatement (tests/conftest.py's wrapper of oracle/liboracle.so):
+    the checker and the timed CPU baseline, never the product."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    from conftest import ORACLE_SO, Oracle
+    return Oracle(ORACLE_SO)
+
+
+def cpu_time(fn, reps: int):
+    """Median wall time (ms) of `reps` runs after one warm-up, and the last
+    result (the reference's Timer_0 region: the algorithm only)."""
+    out = fn()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append((time.perf_counter() - t0) * 1e3)
+    return float(np.median(ts)), out
+
+
+def cpu_row(ms: float, work: int, reps: int, equal=None) -> dict:
+    r = {"cpu_ms": round(ms, 3), "cpu_GTEPS": round(work / ms / 1e6, 4), "cpu_threads": CPU_THREADS,
+         "cpu_runs": f"median of {reps} after a warm-up"}
+    if equal is not None:
+        r["equal_oracle"] = bool(equal)
+    return r
+
+
+def cpu_baseline_pagerank(G, iters: int, reps: int = 5) -> dict:
     """Times the oracle's FP64 pull PageRank (CSC given) for `iters` iterations
-    on the host cores.  The oracle is the checker/baseline, never the product."""
+    on the host cores, median of `reps` after a warm-up."""
     so = ROOT / "oracle" / "liboracle.so"
     lib = C.CDLL(str(so))
     fn = lib.oracle_pagerank_fp64_csc
@@ -87,15 +120,14 @@ def cpu_baseline_pagerank(G, iters: int) -> dict:
     x = np.empty(G.n, np.float64)
     it = C.c_uint32()
     res = C.c_double()
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    t0 = time.perf_counter()
-    rc = fn(G.n, outdeg.ctypes.data, trow.ctypes.data, tcol.ctypes.data, 0.85, 0.0, iters, x.ctypes.data,
-            C.byref(it), C.byref(res), cores)
-    dt = time.perf_counter() - t0
-    assert rc == 0
-    return {"value": round(G.m * it.value / dt / 1e9, 4), "unit": "GTEPS", "cores": cores, "kind": "port",
-            "sample": f"{it.value} FP64 pull PageRank iterations on the same RMAT-26 graph "
-                      f"({G.m} edges), {dt:.2f} s, threads={cores}"}
+
+    def run():
+        assert fn(G.n, outdeg.ctypes.data, trow.ctypes.data, tcol.ctypes.data, 0.85, 0.0, iters, x.ctypes.data,
+                  C.byref(it), C.byref(res), CPU_THREADS) == 0
+    ms, _ = cpu_time(run, reps)
+    return {"value": round(G.m * it.value / ms / 1e6, 4), "unit": "GTEPS", "cores": CPU_THREADS, "kind": "port",
+            "sample": f"{it.value} FP64 pull PageRank iterations on the same RMAT-26 graph ({G.m} edges), "
+                      f"median of {reps} after a warm-up: {ms / 1e3:.2f} s, threads={CPU_THREADS}"}
 
 
 # ------------------------------------------------------------ kernel table
@@ -103,14 +135,16 @@ def pct(bytes_, ms):
     return round(100.0 * bytes_ / (ms * 1e-3) / HBM_PEAK, 2)
 
 
-def kernel_table(gx, quick: bool) -> dict:
+def kernel_table(gx, quick: bool, tuned: dict) -> dict:
     out = {}
+    O = oracle()
     u32 = lambda t: t.cpu().numpy().view(np.uint32)  # noqa: E731
     # Config 1: BFS on symmetric RMAT-18 from the max-degree vertex.
     G = gx.Graph.rmat(18, 16, symmetric=True)
     row, col, _ = G.to_host()
     n, m = G.n, G.m
     src = int(np.argmax(np.diff(row)))
+    cms, exp = cpu_time(lambda: O.bfs(row, col, src, CPU_THREADS), 5)
     for name, fn in (("bfs_rmat18", lambda: gx.bfs(G, src)), ("bfs_do_rmat18", lambda: gx.bfs_do(G, src))):
         fn()
         best = min((fn() for _ in range(3)), key=lambda r: r[-1].elapsed_ms)
@@ -120,7 +154,8 @@ def kernel_table(gx, quick: bool) -> dict:
         m_r = st.edges_visited
         byt = 4 * (n + 1) + 4 * m_r + 4 * n + 4 * n_r
         out[name] = {"ms": round(st.elapsed_ms, 4), "GTEPS": round(m_r / st.elapsed_ms / 1e6, 3),
-                     "pct_hbm": pct(byt, st.elapsed_ms), "levels": st.iterations, "n": n, "m": m}
+                     "pct_hbm": pct(byt, st.elapsed_ms), "levels": st.iterations, "n": n, "m": m,
+                     **cpu_row(cms, m_r, 5, np.array_equal(lv, exp))}
     del G
     # Config 2A / 2B: SSSP (delta 64 = shift 6), CC, k-core (k=10).
     graphs = [("rmat22", lambda: gx.Graph.rmat(22, 16, symmetric=True, weights="uniform", wlo=1, whi=255)),
@@ -135,27 +170,51 @@ def kernel_table(gx, quick: bool) -> dict:
         deg = np.diff(row).astype(np.int64)
         runs = {"sssp": lambda: gx.sssp(G, 0, 6), "cc": lambda: gx.connected_components(G),
                 "kcore10": lambda: gx.kcore(G, 10)}
+        cpu = {"sssp": lambda: O.sssp(row, col, wts, 0, 6, CPU_THREADS), "cc": lambda: O.cc(row, col, CPU_THREADS),
+               "kcore10": lambda: O.kcore(row, col, 10, CPU_THREADS)}
         for kname, fn in runs.items():
             fn()
             res = min((fn() for _ in range(3)), key=lambda r: r[-1].elapsed_ms)
             st = res[-1]
+            extra = {}
             if kname == "sssp":
                 d = u32(res[0])
                 m_r = int(deg[d != gx.GX_DIST_INFINITY].sum())
                 byt = 8 * m_r + 12 * n
+                extra = {"edges_relaxed": st.edges_visited, "relaxed_per_edge": round(st.edges_visited / max(m, 1), 3)}
             elif kname == "cc":
                 byt = 4 * m + 12 * n
             else:
                 alive = res[1].cpu().numpy()
                 m_dead = int(deg[alive == 0].sum())
                 byt = 4 * (n + 1) + 4 * m_dead + 8 * n
+            cms, exp = cpu_time(cpu[kname], 5)
             out[f"{kname}_{gname}"] = {"ms": round(st.elapsed_ms, 4), "GTEPS": round(m / st.elapsed_ms / 1e6, 3),
                                        "pct_hbm": pct(byt, st.elapsed_ms), "rounds": st.iterations, "n": n,
-                                       "m": m}
+                                       "m": m, **extra, **cpu_row(cms, m, 5, np.array_equal(u32(res[0]), exp))}
         # Config 4a: BC on the symmetric RMAT-22, 256 seeded sources.
         if gname.startswith("rmat"):
             nsrc = 64 if quick else 256
             srcs = gx.pick_sources(row, nsrc)
+            # Batched SSSP: 64 seeded sources at once vs one source per call,
+            # 8 of the rows checked against the oracle.
+            bs, stbs = gx.sssp_batched(G, srcs[:64], 6)
+            bs, stbs = gx.sssp_batched(G, srcs[:64], 6)
+            one = [gx.sssp(G, int(x), 6)[1].elapsed_ms for x in srcs[:8]]
+            bsh = bs.cpu().numpy().view(np.uint32)
+            eq = all(np.array_equal(bsh[i], O.sssp(row, col, wts, int(srcs[i]), 6, CPU_THREADS)) for i in range(8))
+            cms, _ = cpu_time(lambda: O.sssp(row, col, wts, int(srcs[0]), 6, CPU_THREADS), 5)
+            out[f"sssp_batched_{gname}"] = {
+                "sources": 64, "ms": round(stbs.elapsed_ms, 3), "rounds": stbs.iterations,
+                "plain_ms_per_source": round(float(np.mean(one)), 3),
+                "speedup_batched_vs_plain": round(float(np.mean(one)) * 64 / stbs.elapsed_ms, 2),
+                "rows_equal_oracle_8_sources": bool(eq),
+                "cpu_ms_per_source": round(cms, 3), "cpu_threads": CPU_THREADS}
+            del bs, bsh
+            from galois_amd import autotune
+            t0 = time.perf_counter()
+            tuned[f"bc_{gname}"] = {"choice": autotune.tune(G, "bc", cache=None, sources=srcs),
+                                    "seconds": round(time.perf_counter() - t0, 1)}
             gx.betweenness_centrality_batched(G, srcs[:64])
             bat, stb = gx.betweenness_centrality_batched(G, srcs)
             nplain = 16 if quick else 32
@@ -166,9 +225,15 @@ def kernel_table(gx, quick: bool) -> dict:
             rel = float(np.max(np.abs(a - b) / np.maximum(np.abs(a), 1.0)))
             per_src_plain = stp.elapsed_ms / nplain
             byt = (8 * m + 40 * n) * nsrc
+            cms, _ = cpu_time(lambda: O.bc(row, col, srcs[:16], CPU_THREADS), 3)
             out[f"bc_batched_{gname}"] = {"sources": nsrc, "ms": round(stb.elapsed_ms, 3),
                                           "TEPS_G": round(nsrc * m / stb.elapsed_ms / 1e6, 3),
-                                          "pct_hbm_vs_unbatched_bytes": pct(byt, stb.elapsed_ms)}
+                                          "pct_hbm_vs_unbatched_bytes": pct(byt, stb.elapsed_ms),
+                                          "knobs": {k: gx.tune_get(k) for k in ("bc.chunk", "bc.grid")},
+                                          "cpu_ms_per_source": round(cms / 16, 3), "cpu_threads": CPU_THREADS,
+                                          "cpu_runs": "16 sources, median of 3 after a warm-up"}
+            from galois_amd import autotune
+            autotune.reset()
             out[f"bc_plain_{gname}"] = {"sources_timed": nplain, "ms_per_source": round(per_src_plain, 3),
                                         "TEPS_G": round(m / per_src_plain / 1e6, 3)}
             out[f"bc_speedup_batched_vs_plain_{gname}"] = round(per_src_plain * nsrc / stb.elapsed_ms, 2)
@@ -192,6 +257,10 @@ def kernel_table(gx, quick: bool) -> dict:
         srcs.extend((gx.pick_sources(prow, 4, seed=SEED + i) + offs[i]).tolist())
     del parts
     srcs = np.array(srcs, np.uint32)
+    from galois_amd import autotune
+    t0 = time.perf_counter()
+    tuned["bc_uneven64"] = {"choice": autotune.tune(U, "bc", cache=None, sources=srcs),
+                            "seconds": round(time.perf_counter() - t0, 1)}
     gx.betweenness_centrality_batched(U, srcs[:64])
     bat, stb = gx.betweenness_centrality_batched(U, srcs)
     nplain = 16
@@ -211,7 +280,11 @@ def kernel_table(gx, quick: bool) -> dict:
         "ms": round(stb.elapsed_ms, 3), "TEPS_G": round(len(srcs) * U.m / ngraphs / stb.elapsed_ms / 1e6, 3),
         "plain_ms_per_source": round(per_src_plain, 3),
         "speedup_batched_vs_plain": round(per_src_plain * len(srcs) / stb.elapsed_ms, 2),
-        "max_rel_err_vs_oracle_8_sources": float(np.max(np.abs(got - exp) / np.maximum(np.abs(exp), 1.0)))}
+        "max_rel_err_vs_oracle_8_sources": float(np.max(np.abs(got - exp) / np.maximum(np.abs(exp), 1.0))),
+        "knobs": {k: gx.tune_get(k) for k in ("bc.chunk", "bc.grid")}}
+    cms, _ = cpu_time(lambda: O.bc(urow, ucol, srcs, CPU_THREADS), 3)
+    out["bc_batched_uneven64"].update({"cpu_ms": round(cms, 3), "cpu_threads": CPU_THREADS,
+                                       "cpu_runs": "all 256 sources, median of 3 after a warm-up"})
     # The same 64 graphs with 64 seeded sources each (4096 sources): every
     # 64-source batch then lies in one graph, so all its lanes work at every
     # vertex it reaches (with 4 sources per graph a batch spans 16 graphs and
@@ -237,6 +310,7 @@ def kernel_table(gx, quick: bool) -> dict:
         "plain_ms_per_source": round(per_src_plain64, 3),
         "speedup_batched_vs_plain": round(per_src_plain64 * len(srcs64) / stb64.elapsed_ms, 2),
         "max_rel_err_vs_oracle_8_sources": float(np.max(np.abs(got64 - exp64) / np.maximum(np.abs(exp64), 1.0)))}
+    autotune.reset()
     del U, bat, pl, sub, bat64, pl64, sub64
     # Config 5: differentiable PageRank on directed weighted RMAT-22 (FP32
     # lognormal weights, row-normalised), loss sum(c * x), c ~ N(0,1).
@@ -282,7 +356,7 @@ def kernel_table(gx, quick: bool) -> dict:
         "fd_max_rel_err_16_edges": float(np.max(np.abs(fd - a64[top]) / np.abs(a64[top])))}
     del G
     if not quick:
-        out.update(rmat26_table(gx))
+        out.update(rmat26_table(gx, O))
     return out
 
 
@@ -365,26 +439,30 @@ def personality_rows(gx, row, col, wts, gname) -> dict:
     return out
 
 
-def rmat26_table(gx) -> dict:
+def rmat26_table(gx, O) -> dict:
     """BFS (top-down and direction-optimising, from the max-degree vertex),
     SSSP (delta 64 from vertex 0), CC and k-core (k=10) on the symmetrised
-    RMAT-26 graph with uniform weights 1..255 (2.1e9 directed edges): the
-    scale the plain kernels are targeted at.  Parity is tested at smaller
-    scales; here only times, GTEPS (m / time, the Graph500 convention for the
-    direction-optimising BFS) and the algorithmic-byte share of HBM peak."""
+    RMAT-26 graph with uniform weights 1..255 (2.1e9 directed edges): times,
+    GTEPS (m / time, the Graph500 convention for the direction-optimising BFS)
+    and the algorithmic-byte share of HBM peak, beside the oracle's time on
+    this host's cores (median of 3 after a warm-up) and whether the outputs
+    are bit-equal."""
     out = {}
     G = gx.Graph.rmat(26, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
-    row = G.row_host()
+    row, col, wts = G.to_host()
     n, m = G.n, G.m
     deg = np.diff(row).astype(np.int64)
-    del row
     src = int(np.argmax(deg))
+    cpu = {"bfs": lambda: O.bfs(row, col, src, CPU_THREADS), "sssp": lambda: O.sssp(row, col, wts, 0, 6, CPU_THREADS),
+           "cc": lambda: O.cc(row, col, CPU_THREADS), "kcore10": lambda: O.kcore(row, col, 10, CPU_THREADS)}
+    ref = {}
     runs = {"bfs": lambda: gx.bfs(G, src), "bfs_do": lambda: gx.bfs_do(G, src), "sssp": lambda: gx.sssp(G, 0, 6),
             "cc": lambda: gx.connected_components(G), "kcore10": lambda: gx.kcore(G, 10)}
     for kname, fn in runs.items():
         fn()
         res = min((fn() for _ in range(2)), key=lambda r: r[-1].elapsed_ms)
         st = res[-1]
+        extra = {}
         if kname.startswith("bfs"):
             lv = res[0].cpu().numpy().view(np.uint32)
             reached = lv != gx.GX_DIST_INFINITY
@@ -392,14 +470,21 @@ def rmat26_table(gx) -> dict:
         elif kname == "sssp":
             d = res[0].cpu().numpy().view(np.uint32)
             byt = 8 * int(deg[d != gx.GX_DIST_INFINITY].sum()) + 12 * n
+            extra = {"edges_relaxed": st.edges_visited, "relaxed_per_edge": round(st.edges_visited / max(m, 1), 3)}
         elif kname == "cc":
             byt = 4 * m + 12 * n
         else:
             alive = res[1].cpu().numpy()
             byt = 4 * (n + 1) + 4 * int(deg[alive == 0].sum()) + 8 * n
+        base = "bfs" if kname.startswith("bfs") else kname
+        if base not in ref:
+            ref[base] = cpu_time(cpu[base], 3)
+        cms, exp = ref[base]
         out[f"{kname}_rmat26"] = {"ms": round(st.elapsed_ms, 4), "GTEPS": round(m / st.elapsed_ms / 1e6, 3),
-                                  "pct_hbm": pct(byt, st.elapsed_ms), "rounds": st.iterations, "n": n, "m": m}
-    del G
+                                  "pct_hbm": pct(byt, st.elapsed_ms), "rounds": st.iterations, "n": n, "m": m,
+                                  **extra, **cpu_row(cms, m, 3, np.array_equal(res[0].cpu().numpy().view(np.uint32),
+                                                                               exp))}
+    del G, row, col, wts
     return out
 
 
@@ -417,9 +502,10 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="run galois_amd.autotune on the headline graph before the timed steps")
     ap.add_argument("--frontier-pass", action="store_true",
-                    help="workload for rocprofv3 kernel traces of the frontier kernels: k-core (k=10), SSSP "
-                         "(delta 64) and top-down BFS on symmetric weighted RMAT at --scale, each twice; prints no "
-                         "JSON")
+                    help="workload for rocprofv3 kernel traces and counters of the frontier kernels: k-core "
+                         "(k=10), SSSP (delta 64), top-down and direction-optimising BFS and CC on symmetric "
+                         "weighted RMAT at --scale, each twice, then a 1 GiB device copy (the calibration "
+                         "reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="workload for rocprofv3 --pmc runs: 3 pull sweeps on the headline graph, then a 1 GiB "
                          "device copy (the calibration reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
@@ -444,10 +530,18 @@ def main() -> int:
         src = int(np.argmax(np.diff(row)))
         del row
         for name, fn in (("kcore", lambda: gx.kcore(G, 10)), ("sssp", lambda: gx.sssp(G, 0, 6)),
-                         ("bfs", lambda: gx.bfs(G, src))):
+                         ("bfs", lambda: gx.bfs(G, src)), ("bfs_do", lambda: gx.bfs_do(G, src)),
+                         ("cc", lambda: gx.connected_components(G))):
             fn()
             st = fn()[-1]
-            log(f"frontier pass: {name} {st.elapsed_ms:.2f} ms, {st.iterations} rounds")
+            log(f"frontier pass: {name} {st.elapsed_ms:.2f} ms, {st.iterations} rounds, "
+                f"{st.edges_visited} edges visited")
+        del G
+        a = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        b.copy_(a)
+        sync()
+        log("frontier pass: 1 GiB copy")
         return 0
 
     log(f"generating directed RMAT-{args.scale}")
@@ -473,7 +567,7 @@ def main() -> int:
     if args.autotune:
         from galois_amd import autotune
         t0 = time.perf_counter()
-        tuned = autotune.tune(G, "pagerank", cache=None, iterations=3)
+        tuned = {"pagerank": autotune.tune(G, "pagerank", cache=None, iterations=3)}
         log(f"autotune: {tuned} in {time.perf_counter() - t0:.1f} s")
     for _ in range(args.warmup):
         step()
@@ -504,10 +598,10 @@ def main() -> int:
     # row offsets + 4n ranks gathered (each once) + 4n ranks written + 4n
     # out-degrees = 4m + 16n + 4.  The bytes of the sweep actually run, as the
     # library states them (gx_stats.main_kernel_bytes), are reported beside:
-    #  blocked (default): phase 1 reads a 2-byte source id per group and y
-    #    (4n), writes one 4-byte contribution per group; phase 2 reads a
-    #    2-byte destination id per edge, the contributions, its job table
-    #    (12 bytes per job), x_old and out-degrees (8n), writes x' and y' (8n).
+    #  blocked (default): phase 1 reads a 2-byte source id per group, x and
+    #    the out-degrees (8n), writes one 4-byte contribution per group; phase
+    #    2 reads a 2-byte destination id per edge, the contributions, its job
+    #    table (12 bytes per job) and x_old (4n), writes x' (4n).
     #  pull: 4m column ids + 4(n+1) row offsets + 4n y (read once) + 4n x_old +
     #    4n out-degrees + 8n (x', y' written).
     algo_bytes = 4 * m + 4 * (n + 1) + 12 * n
@@ -537,12 +631,27 @@ def main() -> int:
     log(f"fp32 vs fp64: L1 {pr_err['l1_fp32_vs_fp64']:.3e}")
 
     cpu = None
+    layout = gx.pagerank_layout_info(G)
     if not args.no_cpu:
         log("cpu baseline")
         cpu = cpu_baseline_pagerank(G, iters=3)
+        # One full FP64 solve on the CPU (tol 1e-9, at most 100 iterations):
+        # its time and the GPU runs' error against it.
+        row, col, _ = G.to_host()
+        t0 = time.perf_counter()
+        ref, it_cpu, _ = oracle().pagerank(row, col, threads=CPU_THREADS)
+        dt = time.perf_counter() - t0
+        del row, col
+        pr_err.update({"l1_fp32_vs_cpu_fp64": float(np.abs(a - ref).sum()),
+                       "linf_fp32_vs_cpu_fp64": float(np.abs(a - ref).max()),
+                       "l1_fp64_vs_cpu_fp64": float(np.abs(b - ref).sum()),
+                       "linf_fp64_vs_cpu_fp64": float(np.abs(b - ref).max()), "iterations_cpu": it_cpu,
+                       "cpu_full_solve_ms": round(dt * 1e3, 1), "cpu_GTEPS": round(m * it_cpu / dt / 1e9, 4),
+                       "cpu_threads": CPU_THREADS, "cpu_runs": "one full solve"})
+        del ref
     del G, x32, x64
     torch.cuda.empty_cache()
-    kernels = {} if args.no_kernels else kernel_table(gx, args.quick)
+    kernels = {} if args.no_kernels else kernel_table(gx, args.quick, tuned)
     kernels["pagerank_rmat26"] = pr_err
 
     out = {
@@ -561,7 +670,11 @@ def main() -> int:
         "config": {"workload": f"pagerank_fp32_rmat{args.scale}_directed", "n": n, "m": m,
                    "edge_factor": 16, "alpha": 0.85, "tol": 1e-9, "max_iter": 100, "seed": SEED,
                    "sweep": "propagation_blocked" if blocked else "merge_path_pull",
-                   "knobs": {k: gx.tune_get(k) for k in ("pr.blocking", "pr.hot_sources", "frontier.hub_degree")},
+                   "knobs": {k: {"set": gx.tune_get(k), "effective": gx.tune_effective(k, n, m)}
+                             for k in ("pr.blocking", "pr.hot_sources", "pr.task_edges", "frontier.hub_degree",
+                                       "bfs.visited_bitmap", "sssp.max_merge", "sssp.defer_hubs", "bc.chunk",
+                                       "bc.grid")},
+                   "layout": layout,
                    "autotuned": tuned},
         "roofline": roofline,
         "cpu_baseline": cpu,

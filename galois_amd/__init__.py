@@ -300,6 +300,17 @@ def sssp(g: Graph, source: int, shift: int = 13):
     return dist[:g.n], _stats(st)
 
 
+def sssp_batched(g: Graph, sources, shift: int = 13):
+    """Shortest paths from every source at once (64 per wavefront); returns
+    (dist int32 tensor [len(sources), n], Stats), row i for sources[i]."""
+    torch = _torch()
+    src = np.ascontiguousarray(sources, dtype=np.uint32)
+    dist = torch.empty((len(src), g.n), dtype=torch.int32, device="cuda")
+    st = gx_stats()
+    check(lib().gx_sssp_batched(g.handle, src.ctypes.data, len(src), shift, _ptr(dist), C.byref(st)))
+    return dist, _stats(st)
+
+
 def connected_components(g: Graph):
     """Min-label connected components; returns (label, Stats)."""
     torch = _torch()
@@ -333,6 +344,14 @@ def pagerank(g: Graph, alpha: float = 0.85, tol: float = 1e-9, max_iter: int = 1
     st = gx_stats()
     check(lib().gx_pagerank_pull(g.handle, alpha, tol, max_iter, 1 if fp64 else 0, _ptr(pr), C.byref(st)))
     return pr[:g.n], _stats(st)
+
+
+def pagerank_layout_info(g: Graph) -> dict:
+    """Shape of the propagation-blocked layout of g (empty dict before the
+    first FP32 pagerank call builds it)."""
+    info = _lib.gx_pb_layout_info()
+    check(lib().gx_pagerank_layout_info(g.handle, C.byref(info)))
+    return info.as_dict() if info.built else {}
 
 
 def pagerank_ref(g: Graph, variant: str = "residual", alpha: float = 0.85, tol: float = 1e-3,

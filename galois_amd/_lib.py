@@ -44,6 +44,15 @@ class gx_stats(C.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
 
 
+class gx_pb_layout_info(C.Structure):
+    _fields_ = [(name, C.c_int64) for name in ("bins", "groups", "pieces", "tasks", "sliced_bins", "jobs_long",
+                                                "jobs_mid", "jobs_short", "task_edges")] + [
+        ("built", C.c_int32), ("reserved", C.c_int32)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_ if name != "reserved"}
+
+
 class gx_host_csr(C.Structure):
     _fields_ = [
         ("n", C.c_int64),
@@ -83,6 +92,7 @@ SIGNATURES: dict[str, tuple] = {
     "gx_tune_set": (INT, [C.c_char_p, C.c_int64]),
     "gx_tune_get": (INT, [C.c_char_p, C.POINTER(C.c_int64)]),
     "gx_tune_effective": (INT, [C.c_char_p, C.c_int64, C.c_int64, C.POINTER(C.c_int64)]),
+    "gx_pagerank_layout_info": (INT, [P, C.POINTER(gx_pb_layout_info)]),
     "gx_graph_create": (INT, [I64, I64, P, P, P, U32, PG]),
     "gx_graph_create_device": (INT, [I64, I64, P, P, P, U32, PG]),
     "gx_graph_free": (INT, [P]),
@@ -103,6 +113,7 @@ SIGNATURES: dict[str, tuple] = {
     "gx_bfs": (INT, [P, U32, P, PST]),
     "gx_bfs_do": (INT, [P, U32, U32, U32, P, P, PST]),
     "gx_sssp_delta": (INT, [P, U32, U32, P, PST]),
+    "gx_sssp_batched": (INT, [P, P, U32, U32, P, PST]),
     "gx_cc": (INT, [P, P, PST]),
     "gx_kcore": (INT, [P, U32, P, P, C.POINTER(U64), PST]),
     "gx_pagerank_pull": (INT, [P, D, D, U32, INT, P, PST]),

@@ -18,6 +18,10 @@ Knobs and candidates:
                        2 also sends FP64 through the blocked sweep (not a
                        candidate: it changes FP64 results in the last bit)
   pr.hot_sources       sources of the first PageRank pull pass (0 = one pass)
+  sssp.max_merge       most delta buckets one SSSP phase may cover
+  bc.chunk             batched BC: edges per work item, the degree-bin boundary
+                       between whole-adjacency items and split ones
+  bc.grid              batched BC: workgroups of each level sweep
 """
 from __future__ import annotations
 
@@ -32,15 +36,19 @@ CANDIDATES: dict[str, list[int]] = {
     "frontier.hub_degree": [64, 128, 256, 512, 1024, 4096],
     "pr.blocking": [1, 0],
     "pr.hot_sources": [0, 1 << 17, 1 << 18, 1 << 19, 1 << 20, 1 << 21],
+    "sssp.max_merge": [1, 16, 64, 256, 1024],
+    "bc.chunk": [64, 128, 256, 512, 1024],
+    "bc.grid": [2048, 4096, 8192, 16384],
 }
 
 # Knobs each algorithm responds to.
 KNOBS: dict[str, list[str]] = {
     "bfs": ["frontier.hub_degree"],
-    "sssp": ["frontier.hub_degree"],
+    "sssp": ["frontier.hub_degree", "sssp.max_merge"],
     "cc": ["frontier.hub_degree"],
     "kcore": ["frontier.hub_degree"],
-    "bc": ["frontier.hub_degree"],
+    "bc": ["bc.chunk", "bc.grid"],
+    "bc_plain": ["frontier.hub_degree"],
     "pagerank": ["pr.blocking", "pr.hot_sources"],
 }
 
@@ -59,7 +67,10 @@ def signature(gx, G, algo: str, args: dict | None = None) -> str:
         dmax = -1  # large graphs: degree extremes are not downloaded
     dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
     sym = bool(G.flags & 1)
-    variant = (args or {}).get("precision", "fp32") if algo == "pagerank" else ""
+    a = args or {}
+    variant = a.get("precision", "fp32") if algo == "pagerank" else ""
+    if algo == "bc":  # a batch's group count shapes the sweeps
+        variant = f"groups={(len(a.get('sources', [0])) + 63) // 64}"
     return json.dumps([algo, dev, round(math.log2(max(G.n, 1))), round(math.log2(max(G.m / max(G.n, 1), 1))),
                        -1 if dmax < 0 else round(math.log2(max(dmax, 1))), sym, variant])
 
@@ -93,6 +104,9 @@ def _runner(gx, G, algo: str, args: dict):
         k = args.get("k", 10)
         return lambda: gx.kcore(G, k)[3].elapsed_ms
     if algo == "bc":
+        srcs = args.get("sources", [0])
+        return lambda: gx.betweenness_centrality_batched(G, srcs)[1].elapsed_ms
+    if algo == "bc_plain":
         srcs = args.get("sources", [0])
         return lambda: gx.betweenness_centrality(G, srcs)[1].elapsed_ms
     if algo == "pagerank":

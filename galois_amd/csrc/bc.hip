@@ -26,9 +26,16 @@
 // vertex's level-(L+1) mask.  Backward is the matching pull over out-edges
 // against the level-(L+1) masks.  Adjacencies longer than one chunk are
 // split over several wavefronts that combine with atomics; all other
-// vertices are written by a single wavefront.  The backward level loop has a
-// fixed length once the forward sweep is done and is captured in a HIP graph
-// and replayed.
+// vertices are written by a single wavefront.  A vertex needs only the lanes
+// whose source lies in its own connected component (on symmetric graphs the
+// component labels are known): once those lanes have reached it, it is
+// skipped, so in a union of graphs a group's sweep stops touching a graph
+// when its few sources there are done.  The forward level loop is driven on
+// the device (a state word holds the level and the stop flag; level buffers
+// come from a device table the host extends ahead of each replay) and
+// captured in HIP graphs of kFwdLevels levels, so the host synchronises once
+// per replay; the backward level loop has a fixed length once the forward
+// sweep is done and is captured and replayed the same way.
 #include <algorithm>
 #include <vector>
 
@@ -207,39 +214,46 @@ __global__ void bc_plain_reset_kernel(const uint32_t* __restrict__ q, unsigned c
 }
 
 // ================================================================ batched
-// Work item = (chunk, group); chunk = a vertex and at most kChunk of its
+// Work item = (chunk, group); chunk = a vertex and at most `chunk` of its
 // edges (in-edges for the forward pull, out-edges for the backward pull).
-constexpr int kChunk = 256;
+// The chunk length is the degree bin boundary of the batched sweeps: an
+// adjacency up to one chunk is one wavefront's whole item, a longer one is
+// split over several wavefronts that combine with atomics (knob bc.chunk,
+// default kEdgeChunk).
+// Forward levels per captured replay of the batched BC.
+constexpr int kFwdLevels = 8;
 
 struct ChunkTable {
   uint32_t* vtx;     // owning vertex
-  int32_t* begin;    // first edge (end = min(begin + kChunk, row[vtx + 1]))
+  int32_t* begin;    // first edge (end = min(begin + chunk, row[vtx + 1]))
   uint8_t* whole;    // 1 when the chunk is the vertex's whole adjacency
   int64_t count;
   uint32_t* multi;   // vertices with more than one chunk
   int64_t nmulti;
+  int chunk;         // edges per chunk
 };
 
-__global__ void chunk_count_kernel(const int32_t* __restrict__ row, int64_t n, uint32_t* __restrict__ cnt,
-                                   uint32_t* __restrict__ is_multi) {
+__global__ void chunk_count_kernel(const int32_t* __restrict__ row, int64_t n, int chunk,
+                                   uint32_t* __restrict__ cnt, uint32_t* __restrict__ is_multi) {
   for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
        v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int64_t d = row[v + 1] - row[v];
-    const uint32_t c = static_cast<uint32_t>((d + kChunk - 1) / kChunk);
+    const uint32_t c = static_cast<uint32_t>((d + chunk - 1) / chunk);
     cnt[v] = c;
     is_multi[v] = c > 1 ? 1u : 0u;
   }
 }
 
-__global__ void chunk_fill_kernel(const int32_t* __restrict__ row, int64_t n, const uint32_t* __restrict__ first,
+__global__ void chunk_fill_kernel(const int32_t* __restrict__ row, int64_t n, int chunk,
+                                  const uint32_t* __restrict__ first,
                                   uint32_t* __restrict__ vtx, int32_t* __restrict__ begin,
                                   uint8_t* __restrict__ whole) {
   for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
        v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
     const int32_t b = row[v], e = row[v + 1];
     uint32_t k = first[v];
-    const uint8_t w = (e - b) <= kChunk ? 1 : 0;
-    for (int32_t x = b; x < e; x += kChunk, ++k) {
+    const uint8_t w = (e - b) <= chunk ? 1 : 0;
+    for (int32_t x = b; x < e; x += chunk, ++k) {
       vtx[k] = static_cast<uint32_t>(v);
       begin[k] = x;
       whole[k] = w;
@@ -357,10 +371,13 @@ __global__ void bcb_comp_hi_kernel(const uint32_t* __restrict__ label, int64_t n
   }
 }
 
-// Level-0 masks, visited masks and path counts of the sources.
+// Level-0 masks, visited masks and path counts of the sources; the component
+// label of every lane's source (lanes past `count`: no component).
 __global__ void bcb_seed_kernel(const uint32_t* __restrict__ src, int count, BatchDims D, uint64_t* __restrict__ F0,
-                                uint64_t* __restrict__ vis, float* __restrict__ sigma) {
+                                uint64_t* __restrict__ vis, float* __restrict__ sigma,
+                                const uint32_t* __restrict__ comp, uint32_t* __restrict__ slab) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < D.groups * 64) slab[i] = (i < count && comp) ? comp[src[i]] : 0xffffffffu;
   if (i >= count) return;
   const int g = i / 64, j = i % 64;
   const uint32_t s = src[i];
@@ -369,11 +386,46 @@ __global__ void bcb_seed_kernel(const uint32_t* __restrict__ src, int count, Bat
   sigma[(static_cast<int64_t>(g) * D.n + s) * 64 + j] = 1.f;
 }
 
-// Forward pull for level L -> L+1.
+// Device state of the forward level loop.
+struct FwdState {
+  unsigned long long found;  // new (vertex, source) pairs of the level (forward count)
+  unsigned long long exact;  // the same, counted once per vertex by the visit kernel
+  unsigned level;            // current level L (masks F[L] -> F[L + 1])
+  unsigned done;             // set when a level finds nothing new; deepest = level
+};
+
+// Start of a level: clears the counters and the next level's masks.
+__global__ void bcb_level_begin_kernel(FwdState* S, uint64_t* const* __restrict__ F, int64_t total) {
+  if (S->done) return;
+  uint64_t* Fn = F[S->level + 1];
+  for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
+       i += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    Fn[i] = 0ull;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    S->found = 0;
+    S->exact = 0;
+  }
+}
+
+// End of a level: stop when it found nothing, else move to the next level.
+__global__ void bcb_level_end_kernel(FwdState* S) {
+  if (threadIdx.x != 0 || S->done) return;
+  if (S->exact == 0) S->done = 1;
+  else S->level += 1;
+}
+
+// Forward pull for level L -> L+1 (L from the state).  slab: component label
+// of each lane's source, comp: component label of each vertex (null when
+// components are not known: every lane of the group).
 __global__ void __launch_bounds__(kBlock) bcb_forward_kernel(
     const int32_t* __restrict__ trow, const int32_t* __restrict__ tcol, ChunkTable T, BatchDims D,
-    const uint64_t* __restrict__ FL, uint64_t* __restrict__ Fn, const uint64_t* __restrict__ vis,
-    float* __restrict__ sigma, unsigned long long* __restrict__ newcount, const GroupRange* __restrict__ R) {
+    uint64_t* const* __restrict__ F, FwdState* __restrict__ S, const uint64_t* __restrict__ vis,
+    float* __restrict__ sigma, const GroupRange* __restrict__ R, const uint32_t* __restrict__ comp,
+    const uint32_t* __restrict__ slab) {
+  if (S->done) return;
+  const uint64_t* __restrict__ FL = F[S->level];
+  uint64_t* __restrict__ Fn = F[S->level + 1];
+  unsigned long long* newcount = &S->found;
   const unsigned lane = lane_id();
   const int64_t wave = (blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x) / kWave;
   const int64_t nwaves = static_cast<int64_t>(gridDim.x) * blockDim.x / kWave;
@@ -386,10 +438,12 @@ __global__ void __launch_bounds__(kBlock) bcb_forward_kernel(
     const int64_t c = GL.clo[g] + (it - GL.istart[g]);
     const uint32_t v = T.vtx[c];
     const int64_t goff = static_cast<int64_t>(g) * D.n;
-    const uint64_t need = group_mask(D, g) & ~vis[goff + v];
+    uint64_t lanes = group_mask(D, g);
+    if (comp) lanes &= __ballot(slab[g * 64 + lane] == comp[v]);
+    const uint64_t need = lanes & ~vis[goff + v];
     if (need == 0) continue;
     const int32_t eb = T.begin[c];
-    const int32_t ee = min(eb + kChunk, trow[v + 1]);
+    const int32_t ee = min(eb + T.chunk, trow[v + 1]);
     uint64_t acc_mask = 0;
     float sig = 0.f;
     for (int32_t e0 = eb; e0 < ee; e0 += kWave) {
@@ -427,9 +481,11 @@ __global__ void __launch_bounds__(kBlock) bcb_forward_kernel(
 // visited |= new frontier.  Counting of new (vertex, source) pairs is done
 // by the forward kernel; multi-chunk vertices may be counted more than once
 // there, so the exact count is recomputed here.
-__global__ void __launch_bounds__(kBlock) bcb_visit_kernel(const uint64_t* __restrict__ Fn,
-                                                           uint64_t* __restrict__ vis, int64_t total,
-                                                           unsigned long long* __restrict__ exact) {
+__global__ void __launch_bounds__(kBlock) bcb_visit_kernel(uint64_t* const* __restrict__ F, FwdState* S,
+                                                           uint64_t* __restrict__ vis, int64_t total) {
+  if (S->done) return;
+  const uint64_t* __restrict__ Fn = F[S->level + 1];
+  unsigned long long* exact = &S->exact;
   unsigned long long cnt = 0;
   for (int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; i < total;
        i += static_cast<int64_t>(gridDim.x) * blockDim.x) {
@@ -464,7 +520,7 @@ __global__ void __launch_bounds__(kBlock) bcb_backward_kernel(
     const uint64_t mine = FL[goff + v];
     if (mine == 0) continue;
     const int32_t eb = T.begin[c];
-    const int32_t ee = min(eb + kChunk, row[v + 1]);
+    const int32_t ee = min(eb + T.chunk, row[v + 1]);
     double acc = 0.0;
     for (int32_t e0 = eb; e0 < ee; e0 += kWave) {
       const int32_t e = e0 + static_cast<int32_t>(lane);
@@ -555,15 +611,18 @@ struct CachedChunks {
   bool ready = false;
 };
 
-void build_chunks(const int32_t* row, int64_t n, int64_t m, hipStream_t s, gx_graph* g, CachedChunks& cc) {
-  if (cc.ready) return;
+void build_chunks(const int32_t* row, int64_t n, int64_t m, hipStream_t s, gx_graph* g, CachedChunks& cc,
+                  int chunk) {
+  if (cc.ready && cc.t.chunk == chunk) return;
+  cc.ready = false;
   DevBuf cnt, flag, pos;
   DevBuf& first = cc.first;
   cnt.ensure((n + 1) * 4);
   first.ensure((n + 1) * 4);
   flag.ensure((n + 1) * 4);
   pos.ensure((n + 1) * 4);
-  chunk_count_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(row, n, cnt.as<uint32_t>(), flag.as<uint32_t>());
+  chunk_count_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(row, n, chunk, cnt.as<uint32_t>(),
+                                                                    flag.as<uint32_t>());
   GX_LAUNCHED("chunk_count_kernel");
   fill_async<uint32_t>(cnt.as<uint32_t>() + n, 0u, 1, s);
   fill_async<uint32_t>(flag.as<uint32_t>() + n, 0u, 1, s);
@@ -582,8 +641,9 @@ void build_chunks(const int32_t* row, int64_t n, int64_t m, hipStream_t s, gx_gr
   cc.t.whole = reinterpret_cast<uint8_t*>(base + C * 8 + M * 4);
   cc.t.count = C;
   cc.t.nmulti = M;
+  cc.t.chunk = chunk;
   if (C > 0) {
-    chunk_fill_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(row, n, first.as<uint32_t>(), cc.t.vtx,
+    chunk_fill_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(row, n, chunk, first.as<uint32_t>(), cc.t.vtx,
                                                                     cc.t.begin, cc.t.whole);
     GX_LAUNCHED("chunk_fill_kernel");
   }
@@ -600,7 +660,11 @@ void build_chunks(const int32_t* row, int64_t n, int64_t m, hipStream_t s, gx_gr
 // Per-graph caches of the batched BC (chunk tables of CSR and CSC).
 struct BcCache {
   CachedChunks out, in;
+  CachedChunks fixed;          // out-edge chunks of kEdgeChunk edges (out_edge_chunks)
   std::vector<DevBuf> levels;  // frontier masks per level
+  DevBuf level_table;          // device copy of the level buffer pointers
+  size_t table_cap = 0;
+  CapturedBatch fwd;           // captured forward levels
   // Symmetric graphs: component label (smallest vertex id) per vertex and,
   // per label, one past the component's largest vertex id.
   DevBuf comp_lo, comp_hi;
@@ -618,6 +682,14 @@ static BcCache& bc_cache(gx_graph* g) {
   if (!g->bc_cache) g->bc_cache = std::make_shared<gx_bc_cache_holder>();
   return std::static_pointer_cast<gx_bc_cache_holder>(g->bc_cache)->c;
 }
+
+namespace gx {
+EdgeChunks out_edge_chunks(gx_graph* g) {
+  BcCache& cache = bc_cache(g);
+  build_chunks(g->row, g->n, g->m, g->stream, g, cache.fixed, kEdgeChunk);
+  return EdgeChunks{cache.fixed.t.vtx, cache.fixed.t.begin, cache.fixed.t.count};
+}
+}  // namespace gx
 
 extern "C" gx_status gx_bc(gx_graph* g, const uint32_t* sources, uint32_t count, double* bc, gx_stats* st) {
   GX_API_BEGIN
@@ -742,7 +814,9 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
   const int64_t n = g->n;
   hipStream_t s = g->stream;
   BcCache& cache = bc_cache(g);
-  build_chunks(g->row, n, g->m, s, g, cache.out);
+  const int chunk = static_cast<int>(tune_effective("bc.chunk", n, g->m));
+  const unsigned wgrid = static_cast<unsigned>(tune_effective("bc.grid", n, g->m));  // workgroups per sweep
+  build_chunks(g->row, n, g->m, s, g, cache.out, chunk);
   const int32_t* trow = g->row;
   const int32_t* tcol = g->col;
   ChunkTable tin = cache.out.t;
@@ -750,7 +824,7 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
     build_transpose(g);
     trow = g->trow();
     tcol = g->tcol();
-    build_chunks(trow, n, g->m, s, g, cache.in);
+    build_chunks(trow, n, g->m, s, g, cache.in, chunk);
     tin = cache.in.t;
   }
   const ChunkTable tout = cache.out.t;
@@ -782,11 +856,8 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
   uint32_t launches = 0, levels_total = 0;
   fill_async<double>(bc, 0.0, n, s);
   ++launches;
-  unsigned long long* ctr = g->scratch<unsigned long long>(WS_COUNTERS, 4);
-  unsigned long long* host = static_cast<unsigned long long*>(g->host_pinned(4 * sizeof(unsigned long long)));
   uint32_t* dsrc = g->scratch<uint32_t>(WS_Q3, std::max<uint32_t>(count, 1));
   if (count > 0) GX_HIP(hipMemcpyAsync(dsrc, sources, count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
-  const unsigned wgrid = 256 * 32;  // workgroups of the persistent-style sweeps
   // Group ranges over the in-edge chunks (forward) and the out-edge chunks
   // (backward); 65 entries each.
   GroupRange* Rin = g->scratch<GroupRange>(WS_V5, 2 * 65);
@@ -829,31 +900,67 @@ extern "C" gx_status gx_bc_batched(gx_graph* g, const uint32_t* sources, uint32_
       cache.levels[L].ensure(gn * sizeof(uint64_t));
       return cache.levels[L].as<uint64_t>();
     };
-    uint64_t* F0 = level_buf(0);
-    fill_async<uint64_t>(F0, 0ull, gn, s);
-    bcb_seed_kernel<<<grid_for(cnt, kBlock), kBlock, 0, s>>>(dsrc + first, static_cast<int>(cnt), D, F0, vis, sigma);
-    GX_LAUNCHED("bcb_seed_kernel");
-    launches += 2;
-    // Forward sweep: level masks F[0..deepest].
-    ktimer.begin();
-    size_t L = 0;
-    for (;;) {
-      uint64_t* FL = level_buf(L);
-      uint64_t* Fn = level_buf(L + 1);
-      FL = cache.levels[L].as<uint64_t>();
-      fill_async<uint64_t>(Fn, 0ull, gn, s);
-      fill_async<unsigned long long>(ctr, 0ull, 2, s);
-      bcb_forward_kernel<<<wgrid, kBlock, 0, s>>>(trow, tcol, tin, D, FL, Fn, vis, sigma, ctr, Rin);
-      GX_LAUNCHED("bcb_forward_kernel");
-      bcb_visit_kernel<<<grid_for(gn, kBlock, 8192), kBlock, 0, s>>>(Fn, vis, gn, ctr + 1);
-      GX_LAUNCHED("bcb_visit_kernel");
-      launches += 4;
-      GX_HIP(hipMemcpyAsync(host, ctr, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, s));
+    // Level buffers are allocated ahead of each replay of the captured
+    // forward levels; the device table holds their addresses.
+    auto ensure_levels = [&](size_t upto) {
+      for (size_t L = 0; L <= upto; ++L) level_buf(L);
+      if (cache.table_cap < cache.levels.size()) {
+        cache.table_cap = std::max<size_t>(cache.levels.size(), 2 * cache.table_cap);
+        cache.level_table.ensure(cache.table_cap * sizeof(uint64_t*));
+      }
+      std::vector<uint64_t*> ptrs(cache.levels.size());
+      for (size_t L = 0; L < ptrs.size(); ++L) ptrs[L] = cache.levels[L].as<uint64_t>();
+      GX_HIP(hipMemcpyAsync(cache.level_table.p, ptrs.data(), ptrs.size() * sizeof(uint64_t*),
+                            hipMemcpyHostToDevice, s));
       GX_HIP(hipStreamSynchronize(s));
-      if (host[1] == 0) break;
-      ++L;
-      GX_REQUIRE(L < 65535, "BC level count exceeds 65534");
+    };
+    ensure_levels(kFwdLevels + 1);
+    uint64_t* F0 = cache.levels[0].as<uint64_t>();
+    fill_async<uint64_t>(F0, 0ull, gn, s);
+    uint32_t* slab = g->scratch<uint32_t>(WS_V4, 64 * 64);
+    bcb_seed_kernel<<<grid_for(D.groups * 64, kBlock), kBlock, 0, s>>>(dsrc + first, static_cast<int>(cnt), D, F0,
+                                                                        vis, sigma, comp_lo, slab);
+    GX_LAUNCHED("bcb_seed_kernel");
+    FwdState* fst = g->scratch<FwdState>(WS_BC_LEVEL, 1);
+    fill_async<unsigned long long>(reinterpret_cast<unsigned long long*>(fst), 0ull,
+                                   sizeof(FwdState) / sizeof(unsigned long long), s);
+    launches += 3;
+    // Forward sweep: level masks F[0..deepest], kFwdLevels levels per replay.
+    ktimer.begin();
+    FwdState fh{};
+    {
+      uint64_t key = 0;
+      for (const void* p : {(const void*)trow, (const void*)tcol, (const void*)tin.vtx, (const void*)vis,
+                            (const void*)sigma, (const void*)Rin, (const void*)comp_lo, (const void*)slab,
+                            (const void*)fst})
+        key = mix_key(key, reinterpret_cast<uintptr_t>(p));
+      key = mix_key(mix_key(mix_key(key, static_cast<uint64_t>(gn)), D.last_mask), cache.table_cap);
+      key = mix_key(mix_key(mix_key(key, static_cast<uint64_t>(tin.count)), static_cast<uint64_t>(tin.chunk)), wgrid);
+      uint64_t* const* table = cache.level_table.as<uint64_t*>();
+      for (;;) {
+        const uint64_t k = mix_key(key, reinterpret_cast<uintptr_t>(cache.level_table.p));
+        g->replay(cache.fwd, k, [&](hipStream_t cs) {
+          for (int i = 0; i < kFwdLevels; ++i) {
+            bcb_level_begin_kernel<<<grid_for(gn, kBlock, 8192), kBlock, 0, cs>>>(fst, table, gn);
+            bcb_forward_kernel<<<wgrid, kBlock, 0, cs>>>(trow, tcol, tin, D, table, fst, vis, sigma, Rin, comp_lo,
+                                                         slab);
+            bcb_visit_kernel<<<grid_for(gn, kBlock, 8192), kBlock, 0, cs>>>(table, fst, vis, gn);
+            bcb_level_end_kernel<<<1, 64, 0, cs>>>(fst);
+          }
+          GX_LAUNCHED("bcb forward levels");
+        });
+        launches += 4 * kFwdLevels;
+        GX_HIP(hipMemcpyAsync(&fh, fst, sizeof(FwdState), hipMemcpyDeviceToHost, s));
+        GX_HIP(hipStreamSynchronize(s));
+        if (fh.done) break;
+        GX_REQUIRE(fh.level < 65535 - kFwdLevels, "BC level count exceeds the level table");
+        const size_t before = cache.table_cap;
+        ensure_levels(fh.level + kFwdLevels + 1);
+        table = cache.level_table.as<uint64_t*>();
+        if (cache.table_cap != before) key = mix_key(key, cache.table_cap);
+      }
     }
+    size_t L = fh.level;
     // Levels 0..L are non-empty (L = deepest).  Backward sweep L-1 .. 1,
     // captured once into a HIP graph and replayed.
     const size_t deepest = L;

@@ -43,6 +43,10 @@ int64_t default_hub_degree(int64_t, int64_t) { return 128; }
 int64_t default_hot_sources(int64_t, int64_t) { return 0; }
 int64_t default_blocking(int64_t, int64_t) { return 1; }
 int64_t default_visited_bitmap(int64_t n, int64_t) { return n >= (int64_t(1) << 25) ? 1 : 0; }
+int64_t default_max_merge(int64_t, int64_t) { return 256; }
+int64_t default_bc_chunk(int64_t, int64_t) { return kEdgeChunk; }
+int64_t default_off(int64_t, int64_t) { return 0; }
+int64_t default_bc_grid(int64_t, int64_t) { return 256 * 32; }
 struct Knob {
   const char* key;
   std::atomic<int64_t> value;
@@ -55,6 +59,10 @@ Knob g_knobs[] = {
     {"pr.blocking", {-1}, 0, 2, default_blocking},
     {"pr.task_edges", {-1}, int64_t(1) << 12, int64_t(1) << 30, pr_task_edges_default},
     {"bfs.visited_bitmap", {-1}, 0, 1, default_visited_bitmap},
+    {"sssp.max_merge", {-1}, 1, int64_t(1) << 16, default_max_merge},
+    {"sssp.defer_hubs", {-1}, 0, 1, default_off},
+    {"bc.chunk", {-1}, 64, 4096, default_bc_chunk},
+    {"bc.grid", {-1}, 256, 65536, default_bc_grid},
 };
 Knob* find_knob(const char* key) {
   if (!key) return nullptr;

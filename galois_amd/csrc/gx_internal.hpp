@@ -129,6 +129,8 @@ struct PbLayout {
   bool ready = false;
   int64_t nbins = 0, ngroups = 0, npieces = 0, ntasks = 0, nmulti = 0;
   int64_t njobs[3] = {0, 0, 0};  // long, mid, short jobs
+  int64_t task_entries = 0;      // slice size (edges) the tasks were cut with
+  int64_t nonzero = 0;           // vertices with out-edges: degree-order ids [0, nonzero)
   DevBuf gsrc16, dst16, vals, pieces, jobs, tasks, multi, gacc;
 };
 
@@ -190,7 +192,7 @@ struct gx_graph {
   std::shared_ptr<void> bc_cache;  // batched-BC tables (bc.hip)
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // non-null stream for HIP graph capture
-  gx::CapturedBatch sssp_batch, bfs_batch, bfs_head, kcore_batch, kcore_head;
+  gx::CapturedBatch sssp_batch, sssp_batched_batch, bfs_batch, bfs_head, kcore_batch, kcore_head;
   gx::DevBuf ws[gx::WS_NUM_SLOTS];
   void* pinned = nullptr;  // small pinned host block for counters
   size_t pinned_bytes = 0;
@@ -326,6 +328,16 @@ void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStr
 int64_t max_degree(gx_graph* g);
 // Ascending radix sort of count keys over bits [0, end_bit) (in is not changed).
 void sort_keys_u64(const uint64_t* in, uint64_t* out, int64_t count, int end_bit, hipStream_t s);
+// Out-edge chunks of the CSR: every vertex's adjacency cut into pieces of at
+// most kEdgeChunk edges, (vertex, first edge) per chunk in vertex order;
+// built once per graph and kept with it (bc.hip).
+constexpr int kEdgeChunk = 256;
+struct EdgeChunks {
+  const uint32_t* vtx;
+  const int32_t* begin;
+  int64_t count;
+};
+EdgeChunks out_edge_chunks(gx_graph* g);
 // Device->host copy of a few bytes through the graph's pinned block.
 template <class T> T read_scalar(gx_graph* g, const T* dev) {
   T* h = static_cast<T*>(g->host_pinned(sizeof(T)));

@@ -157,13 +157,14 @@ struct PartFin {
 
 // Row finalisation of the propagation-blocked FP32 sweep: the row sum
 // arrives exact (64-bit fixed point) as an FP32 pair; x' = base + alpha*sum
-// by the compensated FP32 update; residual and dangling mass in FP64.
+// by the compensated FP32 update; residual and dangling mass in FP64.  The
+// next sweep's phase 1 derives y' = x'/outdeg itself; rows from `nonzero` on
+// have no out-edges (the tail of the degree order) and carry dangling mass.
 struct PbFin {
   const PrState* state;
   const float* __restrict__ x_old;
   float* __restrict__ x_new;
-  float* __restrict__ y_new;
-  const uint32_t* __restrict__ outdeg;
+  int64_t nonzero;
   F32Pair base, alpha;
   __device__ __forceinline__ void load() {
     base = split_f64(state->base);
@@ -175,24 +176,17 @@ struct PbFin {
     const float xn = pr_update_f32(base, alpha, sh, sl);
     resid += fabs(static_cast<double>(xn) - static_cast<double>(rs::ld_stream(&x_old[r])));
     __builtin_nontemporal_store(xn, &x_new[r]);
-    const uint32_t d = rs::ld_stream(&outdeg[r]);
-    if (d) {
-      y_new[r] = xn / static_cast<float>(d);
-    } else {
-      y_new[r] = 0.0f;
-      dang += static_cast<double>(xn);
-    }
+    if (r >= nonzero) dang += static_cast<double>(xn);
   }
 };
 
 // Row finalisation of the propagation-blocked FP64 sweep: x' = base +
-// alpha*sum in FP64, as PrFin<double>.
+// alpha*sum in FP64, as PrFin<double>; x' only, as PbFin.
 struct PbFin64 {
   const PrState* state;
   const double* __restrict__ x_old;
   double* __restrict__ x_new;
-  double* __restrict__ y_new;
-  const uint32_t* __restrict__ outdeg;
+  int64_t nonzero;
   double base, alpha;
   __device__ __forceinline__ void load() {
     base = state->base;
@@ -202,13 +196,7 @@ struct PbFin64 {
     const double xn = base + alpha * sum;
     resid += fabs(xn - rs::ld_stream(&x_old[r]));
     __builtin_nontemporal_store(xn, &x_new[r]);
-    const uint32_t d = rs::ld_stream(&outdeg[r]);
-    if (d) {
-      y_new[r] = xn / static_cast<double>(d);
-    } else {
-      y_new[r] = 0.0;
-      dang += xn;
-    }
+    if (r >= nonzero) dang += xn;
   }
 };
 
@@ -385,13 +373,13 @@ static bool use_blocking(const gx_graph* g, size_t real_bytes) {
 
 // Phase 1, phase 2 and the sliced-bin finalisation of one blocked sweep.
 template <class Real, class Fin>
-static uint32_t launch_blocked(const PbLayout& L, const Real* y, typename pb::PbAcc<Real>::V* vals, int64_t n,
-                               const Fin& fin,
+static uint32_t launch_blocked(const PbLayout& L, const Real* x, const uint32_t* outdeg,
+                               typename pb::PbAcc<Real>::V* vals, int64_t n, const Fin& fin,
                                double* part_resid, double* part_dang, const unsigned* stop, hipStream_t s) {
   using Acc = typename pb::PbAcc<Real>::T;
   const pb::Job* jobs = L.jobs.as<pb::Job>();
   pb::gather_kernel<Real><<<static_cast<unsigned>(L.npieces), pb::kBlock, 0, s>>>(
-      L.pieces.as<pb::Piece>(), L.gsrc16.as<uint16_t>(), y, n, vals, stop);
+      L.pieces.as<pb::Piece>(), L.gsrc16.as<uint16_t>(), x, outdeg, n, vals, stop);
   GX_LAUNCHED("pb::gather_kernel");
   pb::accum_kernel<Real, Fin><<<static_cast<unsigned>(L.ntasks), pb::kBlock, 0, s>>>(
       L.tasks.as<pb::Task>(), jobs, jobs + L.njobs[0], jobs + L.njobs[0] + L.njobs[1], vals, L.dst16.as<uint16_t>(),
@@ -462,13 +450,13 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
       if (blocked) {
         ktimer.begin();
         if constexpr (std::is_same<Real, float>::value) {
-          PbFin fin{state, x[cur], x[nxt], y[nxt], outdeg, {}, {}};
-          launches += launch_blocked<float>(L, y[cur], L.vals.as<uint32_t>(), n, fin, part_resid, part_dang,
-                                            &state->converged, s);
+          PbFin fin{state, x[cur], x[nxt], L.nonzero, {}, {}};
+          launches += launch_blocked<float>(L, x[cur], outdeg, L.vals.as<uint32_t>(), n, fin, part_resid,
+                                            part_dang, &state->converged, s);
         } else {
-          PbFin64 fin{state, x[cur], x[nxt], y[nxt], outdeg, 0.0, 0.0};
-          launches += launch_blocked<double>(L, y[cur], L.vals.as<double>(), n, fin, part_resid, part_dang,
-                                             &state->converged, s);
+          PbFin64 fin{state, x[cur], x[nxt], L.nonzero, 0.0, 0.0};
+          launches += launch_blocked<double>(L, x[cur], outdeg, L.vals.as<double>(), n, fin, part_resid,
+                                             part_dang, &state->converged, s);
         }
         ktimer.end();
         pr_reduce_kernel<<<1, kBlock, 0, s>>>(state, part_resid, part_dang, static_cast<int>(L.ntasks + L.nmulti),
@@ -518,16 +506,17 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
     st->elapsed_ms = ms;
     st->main_kernel_ms = ktimer.total_ms();
     st->main_kernel_launches = ktimer.count();
-    // Algorithmic bytes of one sweep.  Per row, both sweeps read y once (the
-    // blocked sweep stages it in LDS), x_old and the out-degree and write x'
-    // and y': 4R + 4.  Blocked: 2 bytes of destination id per edge; per group
-    // a 2-byte source id, the contribution written and read back; 12 bytes
-    // per phase-2 job.  Pull: 4 bytes of source id per edge and the row
-    // offsets (twice, and the partial sums, with the hot/cold split).
+    // Algorithmic bytes of one sweep.  Pull: per row y read once, x_old and
+    // the out-degree read, x' and y' written (4R + 4); 4 bytes of source id
+    // per edge and the row offsets (twice, and the partial sums, with the
+    // hot/cold split).  Blocked: per row x and the out-degree read by phase 1,
+    // x_old read and x' written by the finaliser (3R + 4); 2 bytes of
+    // destination id per edge; per group a 2-byte source id, the
+    // contribution written and read back; 12 bytes per phase-2 job.
     const int64_t R = sizeof(Real), m = g->m;
     const int64_t njobs = L.njobs[0] + L.njobs[1] + L.njobs[2];
     st->main_kernel_bytes = static_cast<uint64_t>(
-        blocked ? 2 * m + (2 + 2 * R) * L.ngroups + (4 * R + 4) * n + 12 * njobs
+        blocked ? 2 * m + (2 + 2 * R) * L.ngroups + (3 * R + 4) * n + 12 * njobs
                 : 4 * m + (hot > 0 ? 2 : 1) * 4 * (n + 1) + (4 * R + 4) * n + (hot > 0 ? 2 * R * n : 0));
   }
 }
@@ -544,6 +533,27 @@ extern "C" gx_status gx_pagerank_pull(gx_graph* g, double alpha, double tol, uin
     run_pull<float>(g, alpha, tol, max_iter, static_cast<float*>(pr), st);
   else
     run_pull<double>(g, alpha, tol, max_iter, static_cast<double*>(pr), st);
+  GX_API_END
+}
+
+extern "C" gx_status gx_pagerank_layout_info(const gx_graph* g, gx_pb_layout_info* info) {
+  GX_API_BEGIN
+  check_graph(g);
+  check_ptr(info, "info");
+  std::memset(info, 0, sizeof(*info));
+  const PbLayout& L = g->degree_order.pb;
+  if (L.ready) {
+    info->built = 1;
+    info->bins = L.nbins;
+    info->groups = L.ngroups;
+    info->pieces = L.npieces;
+    info->tasks = L.ntasks;
+    info->sliced_bins = L.nmulti;
+    info->jobs_long = L.njobs[0];
+    info->jobs_mid = L.njobs[1];
+    info->jobs_short = L.njobs[2];
+    info->task_edges = L.task_entries;
+  }
   GX_API_END
 }
 

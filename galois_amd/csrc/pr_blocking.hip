@@ -121,6 +121,19 @@ __global__ void __launch_bounds__(kBuildBlock) job_base_kernel(Job* __restrict__
     jobs[j].gbase = gpos[jobs[j].gbase] - 1u;
 }
 
+// Number of vertices with out-edges: out-degrees are non-increasing in
+// degree order, so it is the first index whose out-degree is 0 (one thread,
+// binary search).
+__global__ void nonzero_kernel(const uint32_t* __restrict__ outdeg, int64_t n, int64_t* __restrict__ out) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = lo + ((hi - lo) >> 1);
+    if (outdeg[mid] > 0) lo = mid + 1; else hi = mid;
+  }
+  *out = lo;
+}
+
 }  // namespace
 
 void build_layout(gx_graph* g) {
@@ -389,6 +402,14 @@ void build_layout(gx_graph* g) {
   L.npieces = static_cast<int64_t>(pieces.size());
   L.ntasks = static_cast<int64_t>(tv.size());
   L.nmulti = static_cast<int64_t>(multi.size());
+  L.task_entries = static_cast<int64_t>(task_entries);
+  {
+    DevBuf nz;
+    nz.ensure(8);
+    nonzero_kernel<<<1, 64, 0, s>>>(D.outdeg.as<uint32_t>(), n, nz.as<int64_t>());
+    GX_LAUNCHED("pb nonzero_kernel");
+    GX_HIP(hipMemcpy(&L.nonzero, nz.p, 8, hipMemcpyDeviceToHost));
+  }
   L.ready = true;
 }
 

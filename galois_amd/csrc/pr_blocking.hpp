@@ -19,15 +19,18 @@
 //           the in-chunk source (gsrc16) and the contribution slot
 //           vals[group].
 //   phase 1 (gather_kernel) one workgroup per piece of a chunk: the chunk's
-//           kSpan y values are staged in LDS (FP32: encoded as fixed-point
-//           words, see encode_fixed) and vals[q] = y[gsrc16[q]] is streamed
+//           kSpan contributions y = x / outdeg are computed from x and the
+//           out-degrees and staged in LDS (FP32: encoded as fixed-point
+//           words, see encode_fixed), and vals[q] = y[gsrc16[q]] is streamed
 //           out in layout order (2-byte read, 4-byte write per group).
 //   phase 2 (accum_kernel) one workgroup per task (a bin, or a slice of a
 //           heavy bin): the bin's runs are read from every chunk, the group
 //           of each edge is counted from the group flags with wavefront
 //           ballots, and vals[group] is added into an LDS accumulator per
 //           destination (2 bytes per edge and 4 per group read), then the
-//           bin's rows are finalised.
+//           bin's rows are finalised (x' only: the vertices without
+//           out-edges, whose mass is redistributed, are the tail of the
+//           degree order, so no out-degree is read there).
 //
 // On RMAT graphs a hub source has many edges in one bin, so there are far
 // fewer groups than edges (about 0.33 groups per edge at RMAT-26): per edge
@@ -160,11 +163,13 @@ struct PbAcc<double> {
 };
 
 // Phase 1: vals[q] = encode(y[chunk base + gsrc16[q]]) over one piece of
-// groups (the chunk's y is encoded once, as it is staged).
+// groups, with y = x / outdeg computed (and encoded) once per vertex as the
+// chunk is staged: the finaliser then writes x' only.
 template <class Real>
 __global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict__ pieces,
                                                         const uint16_t* __restrict__ src16,
-                                                        const Real* __restrict__ y, int64_t n,
+                                                        const Real* __restrict__ x,
+                                                        const uint32_t* __restrict__ outdeg, int64_t n,
                                                         typename PbAcc<Real>::V* __restrict__ vals,
                                                         const unsigned* stop) {
   using V = typename PbAcc<Real>::V;
@@ -173,7 +178,11 @@ __global__ void __launch_bounds__(kBlock) gather_kernel(const Piece* __restrict_
   const Piece P = pieces[blockIdx.x];
   const int64_t base = static_cast<int64_t>(P.chunk) << kShift;
   const int cnt = static_cast<int>(n - base < kSpan ? n - base : kSpan);
-  for (int i = threadIdx.x; i < cnt; i += kBlock) ys[i] = PbAcc<Real>::encode(rs::ld_stream(&y[base + i]));
+  for (int i = threadIdx.x; i < cnt; i += kBlock) {
+    const uint32_t d = rs::ld_stream(&outdeg[base + i]);
+    const Real xv = rs::ld_stream(&x[base + i]);
+    ys[i] = PbAcc<Real>::encode(d ? xv / static_cast<Real>(d) : Real(0));
+  }
   __syncthreads();
   // Body in 4-entry groups: one 8-byte index load and one 16-byte store per
   // lane, contiguous across the wavefront; the unaligned head and tail of the

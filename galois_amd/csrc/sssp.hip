@@ -23,6 +23,23 @@
 //    and keeps the rest.  The running minimum can name a bucket whose items
 //    have meanwhile moved nearer; that split then moves nothing and the next
 //    one uses the exact minimum of the items it kept.
+//  - Merged buckets: a step covers the buckets [bucket, bucket + width).
+//    width starts at 1 and doubles at an advance when the phase just ended
+//    admitted fewer than kMergeLow items (up to the knob sssp.max_merge),
+//    and halves when it admitted more than kMergeHigh: on road-like graphs
+//    a delta-64 bucket holds a thin shell of a few thousand vertices, so
+//    merging cuts the number of dependent steps, at the cost of a few
+//    vertices expanded before their distance is final (they are expanded
+//    again when it improves, as in any label-correcting order).  Power-law
+//    graphs keep width 1 (their buckets hold millions of items).
+//  - Hubs (degree above frontier.hub_degree) admitted in a relax step are
+//    expanded over the whole grid in the same step.  With the knob
+//    sssp.defer_hubs they instead join a pending list (once per list, mark
+//    word per vertex) that is expanded in a step of its own when the near
+//    frontier has drained, so a hub whose distance improves several times
+//    inside a bucket is expanded once per drain rather than once per
+//    improvement; that costs extra steps and measured slower on RMAT-22 and
+//    no faster on RMAT-26, so it is off by default.
 //  - The loop is driven on the device: every step is the same fixed kernel
 //    sequence (step; work, which is either the relax, with the stale-item
 //    filter fused in as the expansion's admission test, or the far split, as
@@ -52,8 +69,11 @@ constexpr int kGrid = 256 * 8;
 // 21.7 ms, BFS 2.03 -> 1.57 ms), while RMAT frontiers need the full grid.
 constexpr int kGridNoHubs = 512;
 constexpr int kStepsPerGraph = 16;
+// Bucket merging thresholds (items admitted per phase).
+constexpr unsigned long long kMergeLow = 1ull << 14;
+constexpr unsigned long long kMergeHigh = 1ull << 17;
 
-enum Mode : unsigned { kIdle = 0, kRelax = 1, kAdvance = 2 };
+enum Mode : unsigned { kIdle = 0, kRelax = 1, kAdvance = 2, kHubs = 3 };
 
 struct SsspState {
   unsigned near_count[2];  // near queues (double buffered by q)
@@ -66,6 +86,11 @@ struct SsspState {
   unsigned q;              // near queue read this step
   unsigned far_sel;        // far pile appended to
   unsigned bucket;
+  unsigned width;          // buckets merged into the current phase
+  unsigned max_width;      // knob sssp.max_merge
+  unsigned hub_gen;        // generation of the pending hub list
+  unsigned hub_expand;     // hubs expanded this step (hub_count in a hub step, else 0)
+  unsigned long long phase_items0;  // items admitted before the current phase
   unsigned round;
   unsigned phase;
   unsigned steps;
@@ -83,6 +108,7 @@ struct Bufs {
   uint32_t* stamp;
   uint32_t* farmark;
   uint32_t* done;
+  uint32_t* hubmark;  // generation of the pending hub list a hub is in
   uint32_t* nearq[2];
   uint32_t* farq[2];
   uint32_t* hubq;
@@ -91,11 +117,12 @@ struct Bufs {
   unsigned cap;
   uint32_t* hub_off;
   uint32_t hub;  // frontier.hub_degree
+  uint32_t defer;  // sssp.defer_hubs: pending hub list expanded in hub steps
 };
 
 __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
-                                 uint32_t* __restrict__ farmark, uint32_t* __restrict__ done, int64_t n,
-                                 uint32_t src) {
+                                 uint32_t* __restrict__ farmark, uint32_t* __restrict__ done,
+                                 uint32_t* __restrict__ hubmark, int64_t n, uint32_t src) {
   int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (; v < n; v += stride) {
@@ -103,10 +130,12 @@ __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restri
     stamp[v] = 0xffffffffu;
     farmark[v] = 0xffffffffu;
     done[v] = kInf;
+    hubmark[v] = 0xffffffffu;
   }
 }
 
-__global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, uint32_t* dist) {
+__global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, uint32_t* dist,
+                                  unsigned max_width) {
   if (threadIdx.x == 0) {
     S->dist_out = dist;
     S->near_count[0] = 1;
@@ -120,6 +149,11 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
     S->q = 0;
     S->far_sel = 0;
     S->bucket = 0;
+    S->width = 1;
+    S->max_width = max_width;
+    S->hub_gen = 0;
+    S->hub_expand = 0;
+    S->phase_items0 = 0;
     S->round = 0;
     S->phase = 0;
     S->steps = 0;
@@ -130,10 +164,16 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
 }
 
 // Closes a step: the queue it filled becomes the input; after a split the
-// far piles swap.
+// far piles swap; after a hub step the pending list starts empty under a new
+// generation.
 __device__ __forceinline__ void close_step(SsspState* S) {
   if (S->mode == kRelax) {
     S->q ^= 1;
+  } else if (S->mode == kHubs) {
+    S->q ^= 1;
+    S->hub_count = 0;
+    S->hub_expand = 0;
+    S->hub_gen += 1;
   } else if (S->mode == kAdvance) {
     S->far_count[S->far_sel] = 0;
     S->far_sel ^= 1;
@@ -143,20 +183,31 @@ __device__ __forceinline__ void close_step(SsspState* S) {
 }
 
 // Closes the previous step, then chooses this step's mode and clears the
-// counters it appends to.  An advance takes the running far minimum as the
-// new bucket; with far items left but no live far bucket (minimum unset)
-// every far item is settled and the run is done.
-__global__ void step_kernel(SsspState* S) {
+// counters it appends to: relax while the near frontier has items, then
+// expand the pending hubs, then advance.  An advance takes the running far
+// minimum as the new first bucket and sets the phase's width from the items
+// the last phase admitted; with far items left but no live far bucket
+// (minimum unset) every far item is settled and the run is done.
+__global__ void step_kernel(SsspState* S, uint32_t defer) {
   if (threadIdx.x != 0) return;
   close_step(S);
   if (S->done || S->overflow) return;
   S->round += 1;
-  S->hub_count = 0;
   S->near_count[S->q ^ 1] = 0;
+  if (!defer) S->hub_count = 0;  // hubs of a relax step are expanded in that step
   if (S->near_count[S->q] > 0) {
     S->mode = kRelax;
+  } else if (defer && S->hub_count > 0) {
+    S->mode = kHubs;
+    S->hub_expand = S->hub_count;
   } else if (S->far_count[S->far_sel] > 0 && S->far_min != 0xffffffffu) {
     S->mode = kAdvance;
+    const unsigned long long admitted = S->items - S->phase_items0;
+    if (admitted < kMergeLow && S->width < S->max_width)
+      S->width = min(2 * S->width, S->max_width);
+    else if (admitted > kMergeHigh && S->width > 1)
+      S->width >>= 1;
+    S->phase_items0 = S->items;
     S->phase += 1;
     S->bucket = S->far_min;
     S->far_min = 0xffffffffu;
@@ -189,16 +240,20 @@ struct RelaxOp {
   unsigned* near_out_count;
   uint32_t* farq;
   unsigned* far_count;
-  uint32_t bucket, round, phase;
+  uint32_t bucket_end, round, phase, hub_gen;  // bucket_end: last bucket of the phase
   uint32_t far_min = 0xffffffffu;  // this lane's smallest far bucket improved into
   unsigned long long edges = 0, items = 0;
   // Stale-item filter (the reference's test at SSSP.cpp:138): a near item is
-  // expanded only if its distance changed since it was last expanded.
+  // expanded only if its distance changed since it was last admitted.  A
+  // hub passes into the pending list once per list generation (its edges
+  // are counted when the list is expanded).
   __device__ __forceinline__ bool admit(uint32_t u) {
     const uint32_t d = dist[u];
     if (B.done[u] == d) return false;
     B.done[u] = d;
-    edges += static_cast<unsigned long long>(B.row[u + 1] - B.row[u]);
+    const uint32_t deg = static_cast<uint32_t>(B.row[u + 1] - B.row[u]);
+    if (deg > B.hub) return !B.defer || atomicExch(&B.hubmark[u], hub_gen) != hub_gen;
+    edges += deg;
     items += 1;
     return true;
   }
@@ -212,7 +267,7 @@ struct RelaxOp {
         const uint32_t old = atomicMin(&dist[v], nd);
         if (nd < old) {
           const uint32_t nb = nd >> B.shift;
-          if (nb <= bucket) {
+          if (nb <= bucket_end) {
             to_near = atomicExch(&B.stamp[v], round) != round;
           } else {
             to_far = atomicExch(&B.farmark[v], phase) != phase;
@@ -249,8 +304,19 @@ __device__ __forceinline__ RelaxOp make_relax(const Bufs& B, LdsQueue<kQCap>* qn
   SsspState* S = B.S;
   const unsigned q = S->q, fs = S->far_sel;
   return RelaxOp{B, S->dist_out, qn, qf, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.farq[fs], &S->far_count[fs],
-                 S->bucket, S->round, S->phase};
+                 S->bucket + (S->width - 1), S->round, S->phase, S->hub_gen};
 }
+
+// Relax operation over hub edges: counts every valid edge it relaxes.
+template <class Op>
+struct HubCount {
+  Op& op;
+  __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
+    op.edges += valid ? 1ull : 0ull;
+    op(valid, u, e);
+  }
+  __device__ __forceinline__ void after_step() { op.after_step(); }
+};
 
 // Relax step: expand the near items that pass the stale-item filter.
 __device__ __forceinline__ void relax_step(const Bufs& B, LdsQueue<kQCap>& qn, LdsQueue<kQCap>& qf) {
@@ -262,14 +328,20 @@ __device__ __forceinline__ void relax_step(const Bufs& B, LdsQueue<kQCap>& qn, L
   op.finish();
 }
 
+// The hubs of a relax step (hub_count), or with deferral the pending hubs
+// of a hub step (hub_expand, 0 outside hub steps), spread over the grid;
+// hub_offsets ran over the same count.
 __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
   SsspState* S = B.S;
-  if (S->mode != kRelax || S->hub_count == 0) return;
+  const unsigned hubs = B.defer ? (S->mode == kHubs ? S->hub_expand : 0u) : (S->mode == kRelax ? S->hub_count : 0u);
+  if (hubs == 0) return;
   __shared__ LdsQueue<kQCap> qn, qf;
   lq_init(qn);
   lq_init(qf);
   RelaxOp op = make_relax(B, &qn, &qf);
-  expand_hubs<kBlock>(B.row, B.hubq, S->hub_count, B.cap, B.hub_off, op);
+  HubCount<RelaxOp> hc{op};
+  expand_hubs<kBlock>(B.row, B.hubq, hubs, B.cap, B.hub_off, hc);
+  if (blockIdx.x == 0 && threadIdx.x == 0) op.items += min(hubs, B.cap);
   op.finish();
 }
 
@@ -279,7 +351,7 @@ __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
 __device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& qn, LdsQueue<kQCap>& qk) {
   SsspState* S = B.S;
   const uint32_t* __restrict__ dist = S->dist_out;
-  const uint32_t next = S->bucket;
+  const uint32_t next = S->bucket, last = S->bucket + (S->width - 1);
   const unsigned fs = S->far_sel, q = S->q;
   const unsigned cnt = S->far_count[fs];
   const uint32_t* farq = B.farq[fs];
@@ -292,9 +364,9 @@ __device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& q
     if (i < cnt) {
       v = farq[i];
       const uint32_t b = dist[v] >> B.shift;
-      if (b == next) {
+      if (b >= next && b <= last) {
         to_near = atomicExch(&B.stamp[v], round) != round;
-      } else if (b > next) {
+      } else if (b > last) {
         keep = atomicExch(&B.farmark[v], phase) != phase;
         kept_min = min(kept_min, b);
       }
@@ -327,10 +399,10 @@ __global__ void __launch_bounds__(kBlock) work_kernel(Bufs B) {
 // `hubs` false: no vertex exceeds the hub threshold, so the hub list stays
 // empty and its two launches are left out of the step.
 void enqueue_step(const Bufs& B, bool hubs, hipStream_t s) {
-  step_kernel<<<1, 64, 0, s>>>(B.S);
+  step_kernel<<<1, 64, 0, s>>>(B.S, B.defer);
   work_kernel<<<hubs ? kGrid : kGridNoHubs, kBlock, 0, s>>>(B);
   if (hubs) {
-    launch_hub_offsets<kBlock>(B.row, B.hubq, &B.S->hub_count, B.cap, B.hub_off, s);
+    launch_hub_offsets<kBlock>(B.row, B.hubq, B.defer ? &B.S->hub_expand : &B.S->hub_count, B.cap, B.hub_off, s);
     hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
   }
   GX_LAUNCHED("sssp step");
@@ -361,27 +433,31 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.stamp = g->scratch<uint32_t>(WS_V1, cap);
   B.farmark = g->scratch<uint32_t>(WS_V2, cap);
   B.done = g->scratch<uint32_t>(WS_V3, cap);
+  B.hubmark = g->scratch<uint32_t>(WS_V4, cap);
   B.S = g->scratch<SsspState>(WS_COUNTERS, 1);
   B.shift = shift;
   B.cap = cap;
   B.hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(cap));
   B.hub = hub_degree();
+  B.defer = static_cast<uint32_t>(tune_effective("sssp.defer_hubs", n, g->m));
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
   SsspState* host = static_cast<SsspState*>(g->host_pinned(sizeof(SsspState)));
 
   RegionTimer region(s);
   KernelTimer ktimer(s);
-  sssp_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(dist, B.stamp, B.farmark, B.done, n, src);
+  const unsigned max_width = static_cast<unsigned>(tune_effective("sssp.max_merge", n, g->m));
+  sssp_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(dist, B.stamp, B.farmark, B.done, B.hubmark, n,
+                                                                 src);
   GX_LAUNCHED("sssp_init_kernel");
-  sssp_reset_kernel<<<1, 64, 0, s>>>(B.S, B.nearq[0], src, dist);
+  sssp_reset_kernel<<<1, 64, 0, s>>>(B.S, B.nearq[0], src, dist, max_width);
   GX_LAUNCHED("sssp_reset_kernel");
   uint64_t key = 0;
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w,
                         (const void*)B.nearq[0], (const void*)B.nearq[1], (const void*)B.farq[0],
                         (const void*)B.farq[1], (const void*)B.hubq, (const void*)B.stamp, (const void*)B.farmark,
-                        (const void*)B.done, (const void*)B.S, (const void*)B.hub_off})
+                        (const void*)B.done, (const void*)B.hubmark, (const void*)B.S, (const void*)B.hub_off})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
-  key = mix_key(mix_key(mix_key(mix_key(key, shift), cap), B.hub), hubs ? 1u : 0u);
+  key = mix_key(mix_key(mix_key(mix_key(mix_key(key, shift), cap), B.hub), hubs ? 1u : 0u), B.defer);
   uint32_t replays = 0;
   for (;;) {
     ktimer.begin();

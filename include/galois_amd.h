@@ -125,6 +125,22 @@ void gx_set_profiling(int on);
  *   "bfs.visited_bitmap"   gx_bfs: 1 = test a visited bitmap before the
  *                          level word of every edge's target, 0 = not; the
  *                          default uses it from 2^25 vertices on
+ *   "sssp.max_merge"       gx_sssp_delta: most consecutive buckets one
+ *                          phase may cover (a phase merges buckets while the
+ *                          previous one admitted fewer than 2^14 items);
+ *                          1 = one bucket per phase as in the reference's
+ *                          OBIM order; default 256, range [1, 2^16]
+ *   "sssp.defer_hubs"      gx_sssp_delta: 1 = hubs admitted in a relax step
+ *                          wait in a pending list expanded once the near
+ *                          frontier has drained; 0 (default) = expanded in
+ *                          the same step
+ *   "bc.chunk"             gx_bc_batched: edges per work item, the degree
+ *                          bin boundary (an adjacency up to one chunk is one
+ *                          wavefront's item, a longer one is split over
+ *                          wavefronts that combine with atomics); default
+ *                          256, range [64, 4096]
+ *   "bc.grid"              gx_bc_batched: workgroups of each level sweep;
+ *                          default 8192, range [256, 65536]
  * Unknown keys and out-of-range values are GX_ERR_INVALID.
  * gx_tune_get returns the value set (-1 when unset); gx_tune_effective the
  * value an entry point uses on a graph of n vertices and m edges: the value
@@ -243,6 +259,16 @@ gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32_t beta,
  * (lonestar/analytics/cpu/sssp/SSSP.cpp:119-173). */
 gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift,
                         uint32_t* dist, gx_stats* st);
+/* Shortest paths from many sources at once: dist (device uint32[count * n],
+ * source-major: dist[i * n + v] for sources[i]) equals gx_sssp_delta from
+ * each source (GX_DIST_INFINITY where unreached).  Sources run 64 per
+ * wavefront (one lane each) with a shared delta-bucket order (bucket =
+ * dist >> shift, phases may merge buckets as gx_sssp_delta's do); the round
+ * loop is device-driven and captured in HIP graphs.  The reference runs one
+ * source per execution of SSSP.cpp:119-173; no counterpart for the batch.
+ * sources: HOST array. */
+gx_status gx_sssp_batched(gx_graph* g, const uint32_t* sources, uint32_t count, uint32_t shift, uint32_t* dist,
+                          gx_stats* st);
 
 /* Connected components of a symmetric graph; label[v] = minimum vertex id
  * of v's component (the canonical result of the reference union-find,
@@ -281,6 +307,19 @@ gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t* alive,
 gx_status gx_pagerank_pull(gx_graph* g, double alpha, double tol,
                            uint32_t max_iter, int precision, void* pr,
                            gx_stats* st);
+/* Shape of the propagation-blocked layout a graph's FP32 pull PageRank runs
+ * on (built by the first such call and kept with the graph; all zero
+ * before): destination bins (= source chunks) of 16384 vertices, (source,
+ * bin) groups, phase-1 pieces, phase-2 tasks, bins summed in slices, phase-2
+ * jobs per length class, and the slice size in edges it was built with
+ * ("pr.task_edges").  No reference counterpart. */
+typedef struct gx_pb_layout_info {
+  int64_t bins, groups, pieces, tasks, sliced_bins;
+  int64_t jobs_long, jobs_mid, jobs_short;
+  int64_t task_edges;
+  int32_t built, reserved;
+} gx_pb_layout_info;
+gx_status gx_pagerank_layout_info(const gx_graph* g, gx_pb_layout_info* info);
 
 /* Reference-semantics pull PageRank (unnormalised, no dangling
  * redistribution), FP32, the two reference variants:

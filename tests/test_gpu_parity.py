@@ -324,7 +324,10 @@ def test_pagerank_task_knobs(gx, knobs):
     bit-identical to the default's (2^19)."""
     say(f"rmat18 pagerank {knobs}")
     G = gx.Graph.rmat(18, 16, symmetric=False)
+    assert gx.pagerank_layout_info(G) == {}
     ref, st_ref = gx.pagerank(G, tol=1e-9, max_iter=100, precision="fp32")
+    info_ref = gx.pagerank_layout_info(G)
+    assert info_ref["bins"] == 16 and info_ref["task_edges"] == 1 << 19
     try:
         for k, v in knobs.items():
             gx.tune_set(k, v)
@@ -333,6 +336,13 @@ def test_pagerank_task_knobs(gx, knobs):
     finally:
         for k in knobs:
             gx.tune_set(k, -1)
+    info = gx.pagerank_layout_info(G2)
+    assert info["task_edges"] == knobs.get("pr.task_edges", 1 << 19)
+    assert info["groups"] == info_ref["groups"] and info["bins"] == info_ref["bins"]
+    if info["task_edges"] < 1 << 13:
+        assert info["sliced_bins"] > 0 and info["tasks"] > info_ref["tasks"]
+    else:
+        assert info["sliced_bins"] == 0 and info["tasks"] == 16
     assert st.iterations == st_ref.iterations
     np.testing.assert_array_equal(x.cpu().numpy(), ref.cpu().numpy())
 
@@ -422,6 +432,74 @@ def test_hub_degree_knob(gx, oracle, rmat16, hub):
     np.testing.assert_array_equal(u32(d), oracle.sssp(row, col, w, s, 6, threads=8))
     np.testing.assert_array_equal(u32(lab), oracle.cc(row, col, threads=8))
     np.testing.assert_array_equal(u32(deg), oracle.kcore(row, col, 10, threads=8))
+
+
+@pytest.mark.parametrize("merge,defer", [(1, 0), (4096, 0), (256, 1)])
+def test_sssp_bucket_merge_knob(gx, oracle, rmat16, merge, defer):
+    """Distances do not depend on how many buckets a phase may merge
+    (sssp.max_merge; 1 = one bucket per phase) or on deferring hubs
+    (sssp.defer_hubs)."""
+    G, row, col, w = rmat16
+    Gg = gx.Graph.grid(256)
+    grow, gcol, gw = Gg.to_host()
+    say(f"sssp max_merge={merge} defer_hubs={defer}")
+    gx.tune_set("sssp.max_merge", merge)
+    gx.tune_set("sssp.defer_hubs", defer)
+    try:
+        d, _ = gx.sssp(G, 0, 6)
+        dg, stg = gx.sssp(Gg, 0, 6)
+        dg2, _ = gx.sssp(Gg, 77, 0)
+    finally:
+        gx.tune_set("sssp.max_merge", -1)
+        gx.tune_set("sssp.defer_hubs", -1)
+    np.testing.assert_array_equal(u32(d), oracle.sssp(row, col, w, 0, 6, threads=8))
+    np.testing.assert_array_equal(u32(dg), oracle.sssp(grow, gcol, gw, 0, 6, threads=8))
+    np.testing.assert_array_equal(u32(dg2), oracle.sssp(grow, gcol, gw, 77, 0, threads=8))
+
+
+def _sssp_rows_match(gx, oracle, G, row, col, w, srcs, shift):
+    got, st = gx.sssp_batched(G, srcs, shift)
+    got = got.cpu().numpy().view(np.uint32)
+    assert got.shape == (len(srcs), G.n)
+    for i, s in enumerate(srcs):
+        np.testing.assert_array_equal(got[i], oracle.sssp(row, col, w, int(s), shift, threads=8),
+                                      err_msg=f"source {s} (row {i})")
+    return st
+
+
+def test_sssp_batched_golden(gx, golden):
+    for g in golden["undirected"]:
+        G = dev(gx, g)
+        srcs = [int(s) for s in g["sssp"]]
+        for shift in (0, 6):
+            say(f"sssp batched {g['name']} shift={shift} sources={srcs}")
+            got, _ = gx.sssp_batched(G, srcs, shift)
+            got = got.cpu().numpy().view(np.uint32)
+            for i, s in enumerate(srcs):
+                np.testing.assert_array_equal(got[i], np.asarray(g["sssp"][str(s)], np.uint32), err_msg=g["name"])
+
+
+def test_sssp_batched_rmat16_grid_directed(gx, oracle, rmat16):
+    """150 seeded sources on RMAT-16 (two full groups and a partial one, a
+    repeated source), 70 on a 128-grid, 40 on a directed RMAT-12: every row
+    bit-exact against the oracle's single-source delta-stepping."""
+    G, row, col, w = rmat16
+    srcs = gx.pick_sources(row, 150)
+    srcs[7] = srcs[3]  # a source twice in one group
+    say("sssp batched rmat16 150 sources shift 6")
+    st = _sssp_rows_match(gx, oracle, G, row, col, w, srcs, 6)
+    assert st.iterations > 0
+    say("sssp batched grid128 70 sources shift 6")
+    Gg = gx.Graph.grid(128)
+    grow, gcol, gw = Gg.to_host()
+    _sssp_rows_match(gx, oracle, Gg, grow, gcol, gw, gx.pick_sources(grow, 70), 6)
+    say("sssp batched directed rmat12 40 sources shift 4")
+    Gd = gx.Graph.rmat(12, 8, symmetric=False, weights="uniform", wlo=1, whi=255)
+    drow, dcol, dw = Gd.to_host()
+    _sssp_rows_match(gx, oracle, Gd, drow, dcol, dw, np.arange(0, 4000, 100, dtype=np.uint32), 4)
+    # no sources: nothing to do
+    got, _ = gx.sssp_batched(G, np.zeros(0, np.uint32), 6)
+    assert got.shape == (0, G.n)
 
 
 def test_grid_parity(gx, oracle):
@@ -556,6 +634,30 @@ def test_bc_rmat16_plain_vs_batched(gx, oracle, rmat16):
     say("bc rmat16 batched")
     batched, st = gx.betweenness_centrality_batched(G, srcs)
     _bc_close(batched.cpu().numpy(), exp)
+
+
+@pytest.mark.parametrize("chunk,grid", [(64, 256), (1024, 16384)])
+def test_bc_batched_knobs(gx, oracle, rmat16, chunk, grid):
+    """Batched BC is the same at any chunk length (degree-bin boundary) and
+    sweep grid: RMAT-16 (150 sources) and a union of 6 uneven graphs."""
+    G, row, col, w = rmat16
+    srcs = gx.pick_sources(row, 150)
+    parts = [gx.Graph.rmat(sc, ef, seed=900 + i, symmetric=True) for i, (sc, ef) in
+             enumerate([(9, 8), (11, 16), (7, 4), (10, 32), (8, 8), (12, 8)])]
+    U, offs = gx.Graph.union(parts)
+    urow, ucol, _ = U.to_host()
+    usrc = np.concatenate([gx.pick_sources(P.to_host()[0], 5, seed=i) + offs[i] for i, P in enumerate(parts)])
+    say(f"bc batched bc.chunk={chunk} bc.grid={grid}")
+    gx.tune_set("bc.chunk", chunk)
+    gx.tune_set("bc.grid", grid)
+    try:
+        got, _ = gx.betweenness_centrality_batched(G, srcs)
+        ugot, _ = gx.betweenness_centrality_batched(U, usrc.astype(np.uint32))
+    finally:
+        gx.tune_set("bc.chunk", -1)
+        gx.tune_set("bc.grid", -1)
+    _bc_close(got.cpu().numpy(), oracle.bc(row, col, srcs, threads=8))
+    _bc_close(ugot.cpu().numpy(), oracle.bc(urow, ucol, usrc, threads=8))
 
 
 # ------------------------------------------------------------- differentiable PageRank
@@ -706,8 +808,12 @@ def test_autotuner_picks_and_caches(gx, tmp_path):
             assert c2["pr.hot_sources"] == -1 and gx.tune_get("pr.hot_sources") == -1
         c3 = autotune.tune(G, "pagerank", cache=cache, iterations=2, precision="fp64")
         assert c3["pr.blocking"] == -1 and c3["pr.hot_sources"] in autotune.CANDIDATES["pr.hot_sources"]
+        row, _, _ = G.to_host()
+        c4 = autotune.tune(G, "bc", cache=cache, sources=gx.pick_sources(row, 70))
+        assert c4["bc.chunk"] in autotune.CANDIDATES["bc.chunk"] and c4["bc.grid"] in autotune.CANDIDATES["bc.grid"]
+        assert gx.tune_get("bc.chunk") == c4["bc.chunk"]
         autotune.reset()
         assert autotune.tune(G, "bfs", cache=cache, source=0) == c1  # from the cache
-        assert len(__import__("json").loads(cache.read_text())) == 3  # FP32 and FP64 keyed apart
+        assert len(__import__("json").loads(cache.read_text())) == 4  # FP32 and FP64 keyed apart
     finally:
         autotune.reset()
