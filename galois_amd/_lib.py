@@ -115,6 +115,7 @@ SIGNATURES: dict[str, tuple] = {
     "gx_sssp_delta": (INT, [P, U32, U32, P, PST]),
     "gx_sssp_batched": (INT, [P, P, U32, U32, P, PST]),
     "gx_cc": (INT, [P, P, PST]),
+    "gx_cc_noalloc": (INT, [P, P]),
     "gx_kcore": (INT, [P, U32, P, P, C.POINTER(U64), PST]),
     "gx_pagerank_pull": (INT, [P, D, D, U32, INT, P, PST]),
     "gx_pagerank_pull_ref": (INT, [P, INT, F, F, U32, P, PST]),

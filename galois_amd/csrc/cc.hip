@@ -25,6 +25,7 @@
 // atomicCAS of the higher root onto the lower one, so roots stay component
 // minima and the labels are the canonical min labels.
 #include <algorithm>
+#include <mutex>
 
 #include "frontier.hpp"
 #include "gx_device.hpp"
@@ -233,5 +234,90 @@ extern "C" gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* stp) {
     stp->main_kernel_ms = ktimer.total_ms();
     stp->main_kernel_launches = ktimer.count();
   }
+  GX_API_END
+}
+
+// ------------------------------------------------------------ no allocation
+// gx_cc_noalloc runs the kernels of gx_cc with their scratch (union-find
+// parents, the vertex list, the hub list and its offsets, four counters) in
+// one file-scope device array instead of the graph's workspace, so a call
+// allocates nothing.  Each device holds its own copy of the array from module
+// load until the process ends; its address on the current device is resolved
+// once per device (gx_init, or the first call on that device) and kept.
+namespace {
+
+constexpr int64_t kNoAllocMaxVertices = int64_t(1) << 22;
+constexpr int64_t align256(int64_t b) { return (b + 255) & ~int64_t(255); }
+// Byte offsets of the scratch parts for n vertices: counters, parents, vertex
+// list, hub list, hub offsets.
+struct NoAllocParts {
+  int64_t parent, rest, hubq, hub_off, end;
+};
+constexpr NoAllocParts noalloc_parts(int64_t n) {
+  const int64_t w = align256(4 * n);
+  return NoAllocParts{256, 256 + w, 256 + 2 * w, 256 + 3 * w, 256 + 3 * w + align256(4 * hub_off_words(n))};
+}
+constexpr int64_t kNoAllocScratchBytes = noalloc_parts(kNoAllocMaxVertices).end;
+
+__device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char* g_scratch_ptr[kMaxDevices] = {};
+
+}  // namespace
+
+namespace gx {
+unsigned char* noalloc_scratch() {
+  int dev = 0;
+  GX_HIP(hipGetDevice(&dev));
+  GX_REQUIRE(dev >= 0 && dev < kMaxDevices, "device index beyond the no-allocation table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    GX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    GX_REQUIRE(p != nullptr, "no device address for the no-allocation scratch");
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+}  // namespace gx
+
+extern "C" gx_status gx_cc_noalloc(gx_graph* g, uint32_t* label) {
+  GX_API_BEGIN
+  check_graph(g);
+  check_ptr(label, "label");
+  GX_REQUIRE(g->flags & GX_GRAPH_SYMMETRIC, "connected components needs a symmetric graph");
+  const int64_t n = g->n;
+  if (n > kNoAllocMaxVertices)
+    fail(GX_ERR_CAPACITY, "gx_cc_noalloc: the graph has " + std::to_string(n) +
+                              " vertices; its static scratch holds at most " + std::to_string(kNoAllocMaxVertices));
+  if (n == 0) return GX_OK;
+  hipStream_t s = g->stream;
+  unsigned char* base = noalloc_scratch();  // this device's address of `scratch`
+  const NoAllocParts P = noalloc_parts(n);
+  unsigned* ctr = reinterpret_cast<unsigned*>(base);  // [0] hubs, [1] overflow, [2] rest, [3] largest
+  uint32_t* parent = reinterpret_cast<uint32_t*>(base + P.parent);
+  uint32_t* rest = reinterpret_cast<uint32_t*>(base + P.rest);
+  uint32_t* hubq = reinterpret_cast<uint32_t*>(base + P.hubq);
+  uint32_t* hub_off = reinterpret_cast<uint32_t*>(base + P.hub_off);
+  const unsigned cap = static_cast<unsigned>(n);  // lists hold at most n vertices: no overflow
+  const unsigned vgrid = grid_for(n, kBlock, 4096);
+  cc_zero_kernel<<<1, 64, 0, s>>>(ctr);
+  cc_init_kernel<<<vgrid, kBlock, 0, s>>>(parent, n);
+  GX_LAUNCHED("cc_init_kernel");
+  for (int r = 0; r < 2; ++r) cc_sample_kernel<<<vgrid, kBlock, 0, s>>>(g->row, g->col, parent, n, r);
+  cc_compress_kernel<<<vgrid, kBlock, 0, s>>>(parent, label, n);
+  cc_largest_kernel<<<1, 256, 0, s>>>(parent, n, &ctr[3]);
+  cc_rest_list_kernel<<<vgrid, kBlock, 0, s>>>(g->row, parent, n, &ctr[3], rest, &ctr[2], cap, &ctr[1]);
+  HookOp op{g->row, g->col, parent};
+  cc_hook_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, rest, &ctr[2], hubq, &ctr[0], cap, &ctr[1], hub_degree());
+  GX_LAUNCHED("cc_hook_kernel");
+  launch_hub_offsets<kBlock>(g->row, hubq, &ctr[0], cap, hub_off, s);
+  cc_hubs_kernel<<<256 * 8, kBlock, 0, s>>>(g->row, op, hubq, &ctr[0], cap, hub_off);
+  GX_LAUNCHED("cc_hubs_kernel");
+  cc_compress_kernel<<<vgrid, kBlock, 0, s>>>(parent, label, n);
+  GX_LAUNCHED("cc_compress_kernel");
+  GX_HIP(hipStreamSynchronize(s));
   GX_API_END
 }

@@ -90,7 +90,7 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
 // adds the slice bases; it must be given the same `cap`.  `off` holds
 // hub_off_words(cap) words.
 constexpr int kHubSlices = 256;
-inline int64_t hub_off_words(int64_t cap) { return cap + 1 + kHubSlices; }
+constexpr int64_t hub_off_words(int64_t cap) { return cap + 1 + kHubSlices; }
 
 // log2 of the slice length: the smallest power of two with
 // kHubSlices << shift >= hubs (a shift, not a division, in the search).

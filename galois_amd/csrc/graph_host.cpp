@@ -124,6 +124,7 @@ extern "C" gx_status gx_init(int device) {
   GX_HIP(hipGetDeviceProperties(&prop, device));
   if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
     fail(GX_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", library is built for gfx950");
+  (void)noalloc_scratch();  // this device's copy of gx_cc_noalloc's scratch
   GX_API_END
 }
 

@@ -232,6 +232,9 @@ struct gx_graph {
 namespace gx {
 
 bool profiling_enabled();
+// This device's address of the static scratch array of gx_cc_noalloc
+// (resolved with hipGetSymbolAddress once per device, then cached).
+unsigned char* noalloc_scratch();
 // Value of a tuning knob (gx_tune_set) for a graph of n vertices and m
 // edges: the value set, or the built-in default for that size when unset.
 int64_t tune_effective(const char* key, int64_t n, int64_t m);

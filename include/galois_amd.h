@@ -276,6 +276,15 @@ gx_status gx_sssp_batched(gx_graph* g, const uint32_t* sources, uint32_t count, 
  * ConnectedComponents.cpp:183-230).  Replaces cc gg_main
  * (lonestar/analytics/gpu/connected-components/cc.cu:561-655). */
 gx_status gx_cc(gx_graph* g, uint32_t* label, gx_stats* st);
+/* gx_cc without allocating: no counterpart in the reference.  For callers
+ * that cannot allocate device or pinned host memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  Same
+ * labels as gx_cc; the kernels' scratch is a static device array of
+ * 64 MiB + 1.5 KiB that each GPU holds until the process ends, so the graph
+ * may have at most 2^22 vertices (GX_ERR_CAPACITY above that).  Calls that
+ * overlap on one GPU share that array and race: run them on one stream or
+ * serialise them.  No timing, no statistics. */
+gx_status gx_cc_noalloc(gx_graph* g, uint32_t* label);
 
 /* k-core of a symmetric graph by synchronous cascade peeling.
  * degree[v] ends as the number of alive neighbours (alive: degree >= k),
