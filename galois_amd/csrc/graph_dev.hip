@@ -16,8 +16,6 @@
 #include <cstring>
 #include <memory>
 
-#include <rocprim/device/device_radix_sort.hpp>
-#include <rocprim/device/device_scan.hpp>
 
 #include "gx_device.hpp"
 #include "gx_internal.hpp"
@@ -63,14 +61,7 @@ static void fill(T* p, T v, int64_t n, hipStream_t s) {
 }
 
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s) {
-  if (count <= 0) return;
-  size_t tmp = 0;
-  GX_HIP(rocprim::exclusive_scan(nullptr, tmp, in, out, 0u, static_cast<size_t>(count),
-                                 rocprim::plus<uint32_t>(), s));
-  DevBuf t;
-  t.ensure(tmp);
-  GX_HIP(rocprim::exclusive_scan(t.p, tmp, in, out, 0u, static_cast<size_t>(count),
-                                 rocprim::plus<uint32_t>(), s));
+  scan_u32_sum(in, out, count, s);
 }
 
 __global__ void max_degree_kernel(const int32_t* __restrict__ row, int64_t n, unsigned* __restrict__ out) {
@@ -99,12 +90,7 @@ int64_t max_degree(gx_graph* g) {
 }
 
 void sort_keys_u64(const uint64_t* in, uint64_t* out, int64_t count, int end_bit, hipStream_t s) {
-  if (count <= 0) return;
-  size_t tmp = 0;
-  GX_HIP(rocprim::radix_sort_keys(nullptr, tmp, in, out, static_cast<size_t>(count), 0, end_bit, s));
-  DevBuf t;
-  t.ensure(tmp);
-  GX_HIP(rocprim::radix_sort_keys(t.p, tmp, in, out, static_cast<size_t>(count), 0, end_bit, s));
+  radix_sort_u64(in, out, nullptr, nullptr, count, end_bit, s);
 }
 
 static int bits_for(int64_t n) {
@@ -329,11 +315,7 @@ void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipSt
   fill<uint32_t>(src, 0u, m, s);
   row_heads_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(row, src, n);
   GX_LAUNCHED("row_heads_kernel");
-  size_t tmp = 0;
-  GX_HIP(rocprim::inclusive_scan(nullptr, tmp, src, src, static_cast<size_t>(m), rocprim::maximum<uint32_t>(), s));
-  DevBuf t;
-  t.ensure(tmp);
-  GX_HIP(rocprim::inclusive_scan(t.p, tmp, src, src, static_cast<size_t>(m), rocprim::maximum<uint32_t>(), s));
+  inclusive_max_scan_u32(src, src, m, s);
 }
 
 void build_transpose(gx_graph* g) {
@@ -362,14 +344,7 @@ void build_transpose(gx_graph* g) {
   GX_LAUNCHED("transpose_keys_kernel");
   src.release();
   uint32_t* ids_out = reinterpret_cast<uint32_t*>(g->tperm());
-  size_t tmp = 0;
-  GX_HIP(rocprim::radix_sort_pairs(nullptr, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), ids_out,
-                                   static_cast<size_t>(m), 0, 2 * bits, s));
-  DevBuf t;
-  t.ensure(tmp);
-  GX_HIP(rocprim::radix_sort_pairs(t.p, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), ids_out,
-                                   static_cast<size_t>(m), 0, 2 * bits, s));
-  t.release();
+  radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), ids_out, m, 2 * bits, s);
   k0.release();
   i0.release();
   offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), m, bits, n, g->trow());
@@ -419,15 +394,7 @@ static void csr_from_keys(gx_graph* g, DevBuf& keys, int64_t cnt, int bits, int6
   GX_REQUIRE(cnt <= (int64_t(1) << 32), "too many generated edges for 32-bit compaction");
   DevBuf sorted;
   sorted.ensure(std::max<int64_t>(cnt, 1) * sizeof(uint64_t));
-  size_t tmp = 0;
-  if (cnt > 0) {
-    GX_HIP(rocprim::radix_sort_keys(nullptr, tmp, keys.as<uint64_t>(), sorted.as<uint64_t>(),
-                                    static_cast<size_t>(cnt), 0, 2 * bits, s));
-    DevBuf t;
-    t.ensure(tmp);
-    GX_HIP(rocprim::radix_sort_keys(t.p, tmp, keys.as<uint64_t>(), sorted.as<uint64_t>(), static_cast<size_t>(cnt),
-                                    0, 2 * bits, s));
-  }
+  if (cnt > 0) radix_sort_u64(keys.as<uint64_t>(), sorted.as<uint64_t>(), nullptr, nullptr, cnt, 2 * bits, s);
   keys.release();
   int64_t m = 0;
   DevBuf uniq;
@@ -470,6 +437,83 @@ static void csr_from_keys(gx_graph* g, DevBuf& keys, int64_t cnt, int bits, int6
   GX_HIP(hipStreamSynchronize(s));
 }
 
+// Relabelled out-edge keys (new src << bits | new dst) with the edge id.
+__global__ void relabel_out_keys_kernel(const int32_t* __restrict__ col, const uint32_t* __restrict__ src,
+                                        const uint32_t* __restrict__ new_of_old, uint64_t* __restrict__ keys,
+                                        uint32_t* __restrict__ ids, int64_t m, int bits) {
+  int64_t e = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; e < m; e += stride) {
+    keys[e] = (static_cast<uint64_t>(new_of_old[src[e]]) << bits) | new_of_old[col[e]];
+    ids[e] = static_cast<uint32_t>(e);
+  }
+}
+
+__global__ void gather_u32_kernel(const uint32_t* __restrict__ in, const uint32_t* __restrict__ idx,
+                                  uint32_t* __restrict__ out, int64_t m) {
+  int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
+  int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
+  for (; i < m; i += stride) out[i] = in[idx[i]];
+}
+
+void build_relabeled(gx_graph* g) {
+  Relabeled& R = g->relab;
+  if (R.ready) return;
+  GX_REQUIRE(g->flags & GX_GRAPH_WEIGHTED, "relabelled CSR needs uint32 weights");
+  const int64_t n = g->n, m = g->m;
+  hipStream_t s = g->stream;
+  const int bits = bits_for(n);
+  R.new_of_old.ensure(std::max<int64_t>(n, 1) * 4);
+  R.old_of_new.ensure(std::max<int64_t>(n, 1) * 4);
+  R.row.ensure((n + 1) * 4);
+  R.col.ensure(std::max<int64_t>(m, 1) * 4);
+  R.w.ensure(std::max<int64_t>(m, 1) * 4);
+  {
+    DevBuf k0, k1, deg;
+    k0.ensure(std::max<int64_t>(n, 1) * 8);
+    k1.ensure(std::max<int64_t>(n, 1) * 8);
+    deg.ensure(std::max<int64_t>(n, 1) * 4);
+    degree_keys_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(g->row, n, k0.as<uint64_t>());
+    GX_LAUNCHED("degree_keys_kernel");
+    radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), nullptr, nullptr, n, 62, s);
+    rank_from_sorted_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), n, g->row,
+                                                                       R.old_of_new.as<uint32_t>(),
+                                                                       R.new_of_old.as<uint32_t>(), deg.as<uint32_t>());
+    GX_LAUNCHED("rank_from_sorted_kernel");
+    GX_HIP(hipStreamSynchronize(s));
+  }
+  if (m == 0) {
+    fill<int32_t>(R.row.as<int32_t>(), 0, n + 1, s);
+  } else {
+    DevBuf src, k0, k1, i0, perm;
+    src.ensure(m * 4);
+    k0.ensure(m * 8);
+    i0.ensure(m * 4);
+    edge_sources(g->row, n, m, src.as<uint32_t>(), s);
+    relabel_out_keys_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(
+        g->col, src.as<uint32_t>(), R.new_of_old.as<uint32_t>(), k0.as<uint64_t>(), i0.as<uint32_t>(), m, bits);
+    GX_LAUNCHED("relabel_out_keys_kernel");
+    GX_HIP(hipStreamSynchronize(s));
+    src.release();
+    k1.ensure(m * 8);
+    perm.ensure(m * 4);
+    radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), perm.as<uint32_t>(), m, 2 * bits, s);
+    k0.release();
+    i0.release();
+    offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), m, bits, n,
+                                                                          R.row.as<int32_t>());
+    GX_LAUNCHED("offsets_from_sorted_kernel");
+    lo_bits_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), R.col.as<int32_t>(), m,
+                                                              (uint64_t(1) << bits) - 1);
+    GX_LAUNCHED("lo_bits_kernel");
+    gather_u32_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(static_cast<const uint32_t*>(g->w),
+                                                                 perm.as<uint32_t>(), R.w.as<uint32_t>(), m);
+    GX_LAUNCHED("gather_u32_kernel");
+  }
+  GX_HIP(hipStreamSynchronize(s));
+  R.ready = true;
+}
+
 void build_degree_order(gx_graph* g) {
   DegreeOrder& D = g->degree_order;
   if (D.ready) return;
@@ -488,13 +532,7 @@ void build_degree_order(gx_graph* g) {
     k1.ensure(std::max<int64_t>(n, 1) * 8);
     degree_keys_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(g->row, n, k0.as<uint64_t>());
     GX_LAUNCHED("degree_keys_kernel");
-    size_t tmp = 0;
-    GX_HIP(rocprim::radix_sort_keys(nullptr, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(n), 0,
-                                    62, s));
-    DevBuf t;
-    t.ensure(tmp);
-    GX_HIP(rocprim::radix_sort_keys(t.p, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(n), 0, 62,
-                                    s));
+    radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), nullptr, nullptr, n, 62, s);
     rank_from_sorted_kernel<<<grid_for(n, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), n, g->row,
                                                                        D.old_of_new.as<uint32_t>(),
                                                                        D.new_of_old.as<uint32_t>(),
@@ -514,14 +552,7 @@ void build_degree_order(gx_graph* g) {
                                                                    bits);
     GX_LAUNCHED("relabel_keys_kernel");
     src.release();
-    size_t tmp = 0;
-    GX_HIP(rocprim::radix_sort_keys(nullptr, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(m), 0,
-                                    2 * bits, s));
-    DevBuf t;
-    t.ensure(tmp);
-    GX_HIP(rocprim::radix_sort_keys(t.p, tmp, k0.as<uint64_t>(), k1.as<uint64_t>(), static_cast<size_t>(m), 0,
-                                    2 * bits, s));
-    t.release();
+    radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), nullptr, nullptr, m, 2 * bits, s);
     k0.release();
     offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), m, bits, n,
                                                                           D.trow.as<int32_t>());

@@ -46,6 +46,8 @@ int64_t default_visited_bitmap(int64_t n, int64_t) { return n >= (int64_t(1) << 
 int64_t default_max_merge(int64_t, int64_t) { return 256; }
 int64_t default_bc_chunk(int64_t, int64_t) { return kEdgeChunk; }
 int64_t default_off(int64_t, int64_t) { return 0; }
+int64_t default_on(int64_t, int64_t) { return 1; }
+int64_t default_batch_width(int64_t, int64_t) { return 64; }
 int64_t default_bc_grid(int64_t, int64_t) { return 256 * 32; }
 struct Knob {
   const char* key;
@@ -61,6 +63,8 @@ Knob g_knobs[] = {
     {"bfs.visited_bitmap", {-1}, 0, 1, default_visited_bitmap},
     {"sssp.max_merge", {-1}, 1, int64_t(1) << 16, default_max_merge},
     {"sssp.defer_hubs", {-1}, 0, 1, default_off},
+    {"sssp.batch_width", {-1}, 0, int64_t(1) << 24, default_batch_width},
+    {"frontier.relabel", {-1}, 0, 1, default_on},
     {"bc.chunk", {-1}, 64, 4096, default_bc_chunk},
     {"bc.grid", {-1}, 256, 65536, default_bc_grid},
 };

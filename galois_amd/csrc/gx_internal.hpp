@@ -145,6 +145,17 @@ struct DegreeOrder {
   PbLayout pb;
 };
 
+// Degree-ordered copy of a weighted CSR for the frontier kernels: vertices
+// renumbered by (degree descending, id ascending), out-edges sorted by new
+// target id, weights carried along.  The most often relaxed targets (the
+// hubs and their neighbourhoods) then occupy a few contiguous megabytes of
+// a per-vertex array, which stay in L2 instead of costing a line fetch per
+// access.  Built on demand, kept with the graph.
+struct Relabeled {
+  bool ready = false;
+  DevBuf new_of_old, old_of_new, row, col, w;
+};
+
 // A fixed kernel sequence captured once into an executable HIP graph and
 // replayed; re-captured when `key` (identity of the baked kernel arguments)
 // changes.  Used for the device-driven level/bucket loops.
@@ -189,6 +200,7 @@ struct gx_graph {
   gx::PullPlan pull_plan;            // degree-ordered transpose (PageRank pull)
   gx::PullPlan csc_plan, csr_plan;   // plain transpose / CSR (weighted PageRank and adjoint)
   gx::DegreeOrder degree_order;
+  gx::Relabeled relab;
   std::shared_ptr<void> bc_cache;  // batched-BC tables (bc.hip)
   hipStream_t stream = nullptr;
   hipStream_t capture_stream = nullptr;  // non-null stream for HIP graph capture
@@ -323,12 +335,21 @@ inline unsigned grid_for(int64_t items, int block, int64_t max_blocks = 1 << 20)
 void ensure_outdeg(gx_graph* g);    // g->outdeg_buf = out-degree per vertex
 void build_transpose(gx_graph* g);  // CSC + permutation (deterministic)
 void build_degree_order(gx_graph* g);  // g->degree_order (relabelled transpose)
+void build_relabeled(gx_graph* g);     // g->relab (degree-ordered weighted CSR)
 // Source vertex of every CSR edge into src[m].
 void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipStream_t s);
 // Exclusive prefix sum of `count` uint32 values into uint32 `out` (may alias).
 void exclusive_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s);
 // Largest out-degree of the graph (one reduction, then cached).
 int64_t max_degree(gx_graph* g);
+// Scans and radix sort of the build path (sort.hip): sum scan (exclusive),
+// max scan (inclusive), both may run in place; stable ascending LSD radix
+// sort of count < 2^32 keys over bits [0, end_bit) with optional uint32
+// values carried along (inputs are not changed; in and out must differ).
+void scan_u32_sum(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s);
+void inclusive_max_scan_u32(const uint32_t* in, uint32_t* out, int64_t count, hipStream_t s);
+void radix_sort_u64(const uint64_t* kin, uint64_t* kout, const uint32_t* vin, uint32_t* vout, int64_t count,
+                    int end_bit, hipStream_t s);
 // Ascending radix sort of count keys over bits [0, end_bit) (in is not changed).
 void sort_keys_u64(const uint64_t* in, uint64_t* out, int64_t count, int end_bit, hipStream_t s);
 // Out-edge chunks of the CSR: every vertex's adjacency cut into pieces of at

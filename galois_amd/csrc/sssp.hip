@@ -40,6 +40,12 @@
 //    inside a bucket is expanded once per drain rather than once per
 //    improvement; that costs extra steps and measured slower on RMAT-22 and
 //    no faster on RMAT-26, so it is off by default.
+//  - Locality: on graphs with hubs (knob frontier.relabel) the kernels run
+//    on the degree-ordered copy of the CSR (gx_graph::relab): the hubs and
+//    the vertices most often relaxed get the smallest ids, so their
+//    distance words share a few megabytes that stay cached, instead of each
+//    relaxation moving a line of a 268 MB array (RMAT-26).  The source is
+//    mapped in and the distances mapped back at the end.
 //  - The loop is driven on the device: every step is the same fixed kernel
 //    sequence (step; work, which is either the relax, with the stale-item
 //    filter fused in as the expansion's admission test, or the far split, as
@@ -119,6 +125,14 @@ struct Bufs {
   uint32_t hub;  // frontier.hub_degree
   uint32_t defer;  // sssp.defer_hubs: pending hub list expanded in hub steps
 };
+
+// out[v] = dist[new_of_old[v]]: back to the caller's vertex ids.
+__global__ void sssp_unpermute_kernel(const uint32_t* __restrict__ dist, const uint32_t* __restrict__ new_of_old,
+                                      uint32_t* __restrict__ out, int64_t n) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[v] = dist[new_of_old[v]];
+}
 
 __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
                                  uint32_t* __restrict__ farmark, uint32_t* __restrict__ done,
@@ -292,10 +306,8 @@ struct RelaxOp {
     const unsigned long long ed = wave_sum(edges), it = wave_sum(items);
     if (lane_id() == 0) {
       if (fm != 0xffffffffu) atomicMin(&B.S->far_min, fm);
-      if (it) {
-        atomicAdd(&B.S->edges, ed);
-        atomicAdd(&B.S->items, it);
-      }
+      if (ed) atomicAdd(&B.S->edges, ed);  // hub steps count edges in every workgroup, items in one
+      if (it) atomicAdd(&B.S->items, it);
     }
   }
 };
@@ -441,15 +453,26 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.hub = hub_degree();
   B.defer = static_cast<uint32_t>(tune_effective("sssp.defer_hubs", n, g->m));
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
+  // Graphs with hubs run on the degree-ordered copy (frontier.relabel).
+  const bool relabel = hubs && tune_effective("frontier.relabel", n, g->m) != 0;
+  uint32_t run_src = src;
+  if (relabel) {
+    build_relabeled(g);
+    B.row = g->relab.row.as<int32_t>();
+    B.col = g->relab.col.as<int32_t>();
+    B.w = g->relab.w.as<uint32_t>();
+    B.dist = g->scratch<uint32_t>(WS_V5, cap);
+    run_src = read_scalar(g, g->relab.new_of_old.as<uint32_t>() + src);
+  }
   SsspState* host = static_cast<SsspState*>(g->host_pinned(sizeof(SsspState)));
 
   RegionTimer region(s);
   KernelTimer ktimer(s);
   const unsigned max_width = static_cast<unsigned>(tune_effective("sssp.max_merge", n, g->m));
-  sssp_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(dist, B.stamp, B.farmark, B.done, B.hubmark, n,
-                                                                 src);
+  sssp_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(B.dist, B.stamp, B.farmark, B.done, B.hubmark, n,
+                                                                 run_src);
   GX_LAUNCHED("sssp_init_kernel");
-  sssp_reset_kernel<<<1, 64, 0, s>>>(B.S, B.nearq[0], src, dist, max_width);
+  sssp_reset_kernel<<<1, 64, 0, s>>>(B.S, B.nearq[0], run_src, B.dist, max_width);
   GX_LAUNCHED("sssp_reset_kernel");
   uint64_t key = 0;
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w,
@@ -472,6 +495,11 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
     GX_HIP(hipStreamSynchronize(s));
     if (host->overflow) fail(GX_ERR_CAPACITY, "SSSP queue overflow");
     if (host->done) break;
+  }
+  if (relabel) {
+    sssp_unpermute_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(B.dist, g->relab.new_of_old.as<uint32_t>(),
+                                                                        dist, n);
+    GX_LAUNCHED("sssp_unpermute_kernel");
   }
   double ms = region.stop();
   if (st) {

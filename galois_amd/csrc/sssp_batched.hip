@@ -24,8 +24,9 @@
 //    pending (carried by the vertex's first chunk).
 //  - Phases share one bucket range across the 64 sources: the limit is
 //    (first bucket + width) << shift; a phase ends when no pushed lane lands
-//    below it, the next one starts at the smallest pending bucket.  Width
-//    adapts as in gx_sssp_delta (knob sssp.max_merge).
+//    below it, the next one starts at the smallest pending bucket.  Width is
+//    fixed by the knob sssp.batch_width, or (0) adapts as in gx_sssp_delta
+//    (knob sssp.max_merge).
 //  - The round loop is driven on the device (a state word holds the buffer
 //    selector, the limit and the stop flag) and captured in HIP graphs of
 //    kRounds rounds, replayed until the state says done.
@@ -53,6 +54,7 @@ struct BState {
   unsigned far_min;            // smallest pending bucket at or above the limit
   unsigned bucket;             // first bucket of the phase
   unsigned width, max_width;   // buckets of the phase; most allowed
+  unsigned adapt;              // width adapts to the pushes of a phase
   unsigned limit;              // (bucket + width) << shift, saturated
   unsigned sel;                // pending masks read this round: P[sel]
   unsigned done;
@@ -99,7 +101,8 @@ __global__ void sb_init_kernel(uint32_t* __restrict__ dist, uint64_t* __restrict
 }
 
 __global__ void sb_seed_kernel(const uint32_t* __restrict__ src, int count, int64_t n, uint32_t* __restrict__ dist,
-                               uint64_t* __restrict__ p0, BState* S, unsigned max_width, unsigned shift) {
+                               uint64_t* __restrict__ p0, BState* S, unsigned width, unsigned max_width,
+                               unsigned shift) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i == 0) {
     S->near = 0;
@@ -107,9 +110,10 @@ __global__ void sb_seed_kernel(const uint32_t* __restrict__ src, int count, int6
     S->edges = 0;
     S->far_min = 0xffffffffu;
     S->bucket = 0;
-    S->width = 1;
+    S->width = width ? width : 1;
     S->max_width = max_width;
-    S->limit = sat_limit(0, 1, shift);
+    S->adapt = width ? 0u : 1u;
+    S->limit = sat_limit(0, S->width, shift);
     S->sel = 0;
     S->done = 0;
     S->rounds = 0;
@@ -244,8 +248,10 @@ __global__ void sb_step_kernel(BState* S, unsigned shift) {
     S->done = 1;
     return;
   }
-  if (S->pushed < kMergeLow && S->width < S->max_width) S->width = min(2 * S->width, S->max_width);
-  else if (S->pushed > kMergeHigh && S->width > 1) S->width >>= 1;
+  if (S->adapt) {
+    if (S->pushed < kMergeLow && S->width < S->max_width) S->width = min(2 * S->width, S->max_width);
+    else if (S->pushed > kMergeHigh && S->width > 1) S->width >>= 1;
+  }
   S->pushed = 0;
   S->bucket = S->far_min;
   S->limit = sat_limit(S->bucket, S->width, shift);
@@ -300,6 +306,7 @@ extern "C" gx_status gx_sssp_batched(gx_graph* g, const uint32_t* sources, uint3
     const int64_t budget = int64_t(64) << 30;
     const int max_groups = static_cast<int>(std::min<int64_t>(64, std::max<int64_t>(1, budget / per_group)));
     const unsigned max_width = static_cast<unsigned>(tune_effective("sssp.max_merge", n, g->m));
+    const unsigned width = static_cast<unsigned>(tune_effective("sssp.batch_width", n, g->m));
     uint32_t* dsrc = g->scratch<uint32_t>(WS_Q3, count);
     GX_HIP(hipMemcpyAsync(dsrc, sources, count * sizeof(uint32_t), hipMemcpyHostToDevice, s));
     BState* S = g->scratch<BState>(WS_COUNTERS, 1);
@@ -325,7 +332,7 @@ extern "C" gx_status gx_sssp_batched(gx_graph* g, const uint32_t* sources, uint3
       sb_init_kernel<<<grid_for(gn * 64, kBlock, 16384), kBlock, 0, s>>>(B.dist, B.pend[0], B.pend[1], gn);
       GX_LAUNCHED("sb_init_kernel");
       sb_seed_kernel<<<grid_for(cnt, kBlock), kBlock, 0, s>>>(dsrc + first, static_cast<int>(cnt), n, B.dist,
-                                                               B.pend[0], S, max_width, shift);
+                                                               B.pend[0], S, width, max_width, shift);
       GX_LAUNCHED("sb_seed_kernel");
       launches += 2;
       uint64_t key = 0;
