@@ -459,7 +459,7 @@ __global__ void gather_u32_kernel(const uint32_t* __restrict__ in, const uint32_
 void build_relabeled(gx_graph* g) {
   Relabeled& R = g->relab;
   if (R.ready) return;
-  GX_REQUIRE(g->flags & GX_GRAPH_WEIGHTED, "relabelled CSR needs uint32 weights");
+  const bool weighted = (g->flags & GX_GRAPH_WEIGHTED) != 0;  // uint32 weights are carried along
   const int64_t n = g->n, m = g->m;
   hipStream_t s = g->stream;
   const int bits = bits_for(n);
@@ -467,7 +467,7 @@ void build_relabeled(gx_graph* g) {
   R.old_of_new.ensure(std::max<int64_t>(n, 1) * 4);
   R.row.ensure((n + 1) * 4);
   R.col.ensure(std::max<int64_t>(m, 1) * 4);
-  R.w.ensure(std::max<int64_t>(m, 1) * 4);
+  if (weighted) R.w.ensure(std::max<int64_t>(m, 1) * 4);
   {
     DevBuf k0, k1, deg;
     k0.ensure(std::max<int64_t>(n, 1) * 8);
@@ -506,12 +506,27 @@ void build_relabeled(gx_graph* g) {
     lo_bits_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), R.col.as<int32_t>(), m,
                                                               (uint64_t(1) << bits) - 1);
     GX_LAUNCHED("lo_bits_kernel");
-    gather_u32_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(static_cast<const uint32_t*>(g->w),
-                                                                 perm.as<uint32_t>(), R.w.as<uint32_t>(), m);
-    GX_LAUNCHED("gather_u32_kernel");
+    if (weighted) {
+      gather_u32_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(static_cast<const uint32_t*>(g->w),
+                                                                   perm.as<uint32_t>(), R.w.as<uint32_t>(), m);
+      GX_LAUNCHED("gather_u32_kernel");
+    }
   }
   GX_HIP(hipStreamSynchronize(s));
   R.ready = true;
+}
+
+__global__ void unpermute_u32_kernel(const uint32_t* __restrict__ in, const uint32_t* __restrict__ new_of_old,
+                                     uint32_t* __restrict__ out, int64_t n) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
+    out[v] = in[new_of_old[v]];
+}
+
+void unpermute_u32(const uint32_t* in, const uint32_t* new_of_old, uint32_t* out, int64_t n, hipStream_t s) {
+  if (n <= 0) return;
+  unpermute_u32_kernel<<<grid_for(n, 256, 8192), 256, 0, s>>>(in, new_of_old, out, n);
+  GX_LAUNCHED("unpermute_u32_kernel");
 }
 
 void build_degree_order(gx_graph* g) {

@@ -277,10 +277,11 @@ struct KernelTimer {
     evs.push_back(e);
   }
   void end() { begin(); }
-  // Sum of (end - begin) over recorded pairs; the stream must be synchronised.
-  double total_ms() {
+  // Sum of (end - begin) over the first `limit` recorded pairs (all by
+  // default); the stream must be synchronised.
+  double total_ms(uint32_t limit = 0xffffffffu) {
     double t = 0;
-    for (size_t i = 0; i + 1 < evs.size(); i += 2) {
+    for (size_t i = 0; i + 1 < evs.size() && i / 2 < limit; i += 2) {
       float ms = 0;
       GX_HIP(hipEventElapsedTime(&ms, evs[i], evs[i + 1]));
       t += ms;
@@ -335,7 +336,9 @@ inline unsigned grid_for(int64_t items, int block, int64_t max_blocks = 1 << 20)
 void ensure_outdeg(gx_graph* g);    // g->outdeg_buf = out-degree per vertex
 void build_transpose(gx_graph* g);  // CSC + permutation (deterministic)
 void build_degree_order(gx_graph* g);  // g->degree_order (relabelled transpose)
-void build_relabeled(gx_graph* g);     // g->relab (degree-ordered weighted CSR)
+void build_relabeled(gx_graph* g);     // g->relab (degree-ordered CSR, uint32 weights carried)
+// out[v] = in[new_of_old[v]] for v < n (results back to the caller's ids).
+void unpermute_u32(const uint32_t* in, const uint32_t* new_of_old, uint32_t* out, int64_t n, hipStream_t s);
 // Source vertex of every CSR edge into src[m].
 void edge_sources(const int32_t* row, int64_t n, int64_t m, uint32_t* src, hipStream_t s);
 // Exclusive prefix sum of `count` uint32 values into uint32 `out` (may alias).

