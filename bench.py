@@ -678,7 +678,7 @@ def main() -> int:
                    "sweep": "propagation_blocked" if blocked else "merge_path_pull",
                    "knobs": {k: {"set": gx.tune_get(k), "effective": gx.tune_effective(k, n, m)}
                              for k in ("pr.blocking", "pr.hot_sources", "pr.task_edges", "frontier.hub_degree",
-                                       "bfs.visited_bitmap", "sssp.max_merge", "sssp.defer_hubs", "bc.chunk",
+                                       "bfs.visited_bitmap", "sssp.max_merge", "sssp.defer_hubs", "sssp.hub_refine", "bc.chunk",
                                        "bc.grid")},
                    "layout": layout,
                    "autotuned": tuned},

@@ -127,20 +127,27 @@ __global__ void transpose_keys_kernel(const int32_t* __restrict__ col, const uin
   }
 }
 
-// From edge keys sorted by (hi, lo) = (key >> bits, key & mask): the row
-// offsets of the hi ids.  Edge i writes row[x] = i for hi(i-1) < x <= hi(i);
-// the last edge also writes row[x] = m for x in (hi(m-1), n].
-__global__ void offsets_from_sorted_kernel(const uint64_t* __restrict__ keys, int64_t m, int bits, int64_t n,
-                                           int32_t* __restrict__ row) {
+// Row offsets from keys sorted by their hi ids (key >> bits < n): the last
+// edge of each hi id x writes ends[x] = its index + 1, where ends = row + 1
+// starts zeroed; a running maximum over ends then gives every vertex the
+// end of the last nonempty vertex at or before it, i.e. row[x + 1].  Work
+// and depth do not depend on how long the runs of edgeless vertices are.
+__global__ void run_ends_kernel(const uint64_t* __restrict__ keys, int64_t m, int bits, uint32_t* __restrict__ ends) {
   int64_t i = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x;
   int64_t stride = static_cast<int64_t>(gridDim.x) * blockDim.x;
   for (; i < m; i += stride) {
-    int64_t hi = static_cast<int64_t>(keys[i] >> bits);
-    int64_t prev = (i == 0) ? -1 : static_cast<int64_t>(keys[i - 1] >> bits);
-    for (int64_t x = prev + 1; x <= hi; ++x) row[x] = static_cast<int32_t>(i);
-    if (i == m - 1)
-      for (int64_t x = hi + 1; x <= n; ++x) row[x] = static_cast<int32_t>(m);
+    const uint64_t hi = keys[i] >> bits;
+    if (i == m - 1 || (keys[i + 1] >> bits) != hi) ends[hi] = static_cast<uint32_t>(i + 1);
   }
+}
+
+void offsets_from_sorted(const uint64_t* keys, int64_t m, int bits, int64_t n, int32_t* row, hipStream_t s) {
+  fill<int32_t>(row, 0, n + 1, s);
+  if (m == 0 || n == 0) return;
+  uint32_t* ends = reinterpret_cast<uint32_t*>(row + 1);
+  run_ends_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(keys, m, bits, ends);
+  GX_LAUNCHED("run_ends_kernel");
+  inclusive_max_scan_u32(ends, ends, n, s);
 }
 
 __global__ void lo_bits_kernel(const uint64_t* __restrict__ keys, int32_t* __restrict__ out, int64_t m,
@@ -347,8 +354,7 @@ void build_transpose(gx_graph* g) {
   radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), ids_out, m, 2 * bits, s);
   k0.release();
   i0.release();
-  offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), m, bits, n, g->trow());
-  GX_LAUNCHED("offsets_from_sorted_kernel");
+  offsets_from_sorted(k1.as<uint64_t>(), m, bits, n, g->trow(), s);
   lo_bits_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), g->tcol(), m,
                                                             (uint64_t(1) << bits) - 1);
   GX_LAUNCHED("lo_bits_kernel");
@@ -426,8 +432,7 @@ static void csr_from_keys(gx_graph* g, DevBuf& keys, int64_t cnt, int bits, int6
   g->row = g->row_buf.as<int32_t>();
   g->col = g->col_buf.as<int32_t>();
   if (m > 0) {
-    offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(uniq.as<uint64_t>(), m, bits, n, g->row);
-    GX_LAUNCHED("offsets_from_sorted_kernel");
+    offsets_from_sorted(uniq.as<uint64_t>(), m, bits, n, g->row, s);
     lo_bits_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(uniq.as<uint64_t>(), g->col, m,
                                                               (uint64_t(1) << bits) - 1);
     GX_LAUNCHED("lo_bits_kernel");
@@ -500,9 +505,8 @@ void build_relabeled(gx_graph* g) {
     radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), i0.as<uint32_t>(), perm.as<uint32_t>(), m, 2 * bits, s);
     k0.release();
     i0.release();
-    offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), m, bits, n,
-                                                                          R.row.as<int32_t>());
-    GX_LAUNCHED("offsets_from_sorted_kernel");
+    offsets_from_sorted(k1.as<uint64_t>(), m, bits, n,
+                                                                          R.row.as<int32_t>(), s);
     lo_bits_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), R.col.as<int32_t>(), m,
                                                               (uint64_t(1) << bits) - 1);
     GX_LAUNCHED("lo_bits_kernel");
@@ -569,9 +573,8 @@ void build_degree_order(gx_graph* g) {
     src.release();
     radix_sort_u64(k0.as<uint64_t>(), k1.as<uint64_t>(), nullptr, nullptr, m, 2 * bits, s);
     k0.release();
-    offsets_from_sorted_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), m, bits, n,
-                                                                          D.trow.as<int32_t>());
-    GX_LAUNCHED("offsets_from_sorted_kernel");
+    offsets_from_sorted(k1.as<uint64_t>(), m, bits, n,
+                                                                          D.trow.as<int32_t>(), s);
     lo_bits_kernel<<<grid_for(m, 256, 1 << 16), 256, 0, s>>>(k1.as<uint64_t>(), D.tcol.as<int32_t>(), m,
                                                               (uint64_t(1) << bits) - 1);
     GX_LAUNCHED("lo_bits_kernel");

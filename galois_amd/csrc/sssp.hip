@@ -38,8 +38,15 @@
 //    word per vertex) that is expanded in a step of its own when the near
 //    frontier has drained, so a hub whose distance improves several times
 //    inside a bucket is expanded once per drain rather than once per
-//    improvement; that costs extra steps and measured slower on RMAT-22 and
-//    no faster on RMAT-26, so it is off by default.
+//    improvement.  With sssp.hub_refine a hub step expands only the pending
+//    hubs within a distance slice above the smallest pending hub distance
+//    (one distance by default) and keeps the rest pending: the non-hub
+//    frontier drains between hub steps, so hubs go out nearly in distance
+//    order, almost all at their final distance, and a hub's hundreds of
+//    thousands of edges are relaxed about once instead of once per
+//    improvement (RMAT-26: 1.84 -> 1.12 relaxations per edge).  Hub steps
+//    are extra dependent steps, which pays on large power-law graphs only,
+//    so the default enables it from 2^24 vertices on.
 //  - Locality: on graphs with hubs (knob frontier.relabel) the kernels run
 //    on the degree-ordered copy of the CSR (gx_graph::relab): the hubs and
 //    the vertices most often relaxed get the smallest ids, so their
@@ -75,6 +82,7 @@ constexpr int kGrid = 256 * 8;
 // 21.7 ms, BFS 2.03 -> 1.57 ms), while RMAT frontiers need the full grid.
 constexpr int kGridNoHubs = 512;
 constexpr int kStepsPerGraph = 16;
+constexpr int kHubSelectGrid = 256;
 // Bucket merging thresholds (items admitted per phase).
 constexpr unsigned long long kMergeLow = 1ull << 14;
 constexpr unsigned long long kMergeHigh = 1ull << 17;
@@ -95,7 +103,12 @@ struct SsspState {
   unsigned width;          // buckets merged into the current phase
   unsigned max_width;      // knob sssp.max_merge
   unsigned hub_gen;        // generation of the pending hub list
-  unsigned hub_expand;     // hubs expanded this step (hub_count in a hub step, else 0)
+  unsigned hub_expand;     // hubs expanded this step (selected in a hub step, else 0)
+  unsigned hq;             // pending hub list appended to (double buffered)
+  unsigned hub_kept;       // pending hubs a hub step leaves for a later one
+  uint32_t hub_min;        // smallest distance a pending hub had when it was listed
+  uint32_t hub_min_next;   // the same over the hubs a hub step keeps
+  uint32_t hub_limit;      // a hub step expands the pending hubs at or below this distance
   unsigned long long phase_items0;  // items admitted before the current phase
   unsigned round;
   unsigned phase;
@@ -117,13 +130,17 @@ struct Bufs {
   uint32_t* hubmark;  // generation of the pending hub list a hub is in
   uint32_t* nearq[2];
   uint32_t* farq[2];
-  uint32_t* hubq;
+  uint32_t* hubq[2];  // pending hub lists (hq); without deferral hubq[0] holds the step's hubs
+  uint32_t* hubx;     // hubs a hub step expands
   SsspState* S;
   uint32_t shift;
   unsigned cap;
   uint32_t* hub_off;
   uint32_t hub;  // frontier.hub_degree
   uint32_t defer;  // sssp.defer_hubs: pending hub list expanded in hub steps
+  // sssp.hub_refine: width of the distance slice a hub step expands
+  // (pending hubs within hub_span of the smallest); 0 = all pending hubs.
+  uint32_t hub_span;
 };
 
 __global__ void sssp_init_kernel(uint32_t* __restrict__ dist, uint32_t* __restrict__ stamp,
@@ -159,6 +176,9 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
     S->max_width = max_width;
     S->hub_gen = 0;
     S->hub_expand = 0;
+    S->hq = 0;
+    S->hub_kept = 0;
+    S->hub_min = S->hub_min_next = S->hub_limit = 0xffffffffu;
     S->phase_items0 = 0;
     S->round = 0;
     S->phase = 0;
@@ -177,9 +197,12 @@ __device__ __forceinline__ void close_step(SsspState* S) {
     S->q ^= 1;
   } else if (S->mode == kHubs) {
     S->q ^= 1;
-    S->hub_count = 0;
+    S->hq ^= 1;
+    S->hub_count = S->hub_kept;  // kept hubs carry the new generation's mark
+    S->hub_kept = 0;
     S->hub_expand = 0;
     S->hub_gen += 1;
+    S->hub_min = S->hub_min_next;
   } else if (S->mode == kAdvance) {
     S->far_count[S->far_sel] = 0;
     S->far_sel ^= 1;
@@ -194,7 +217,7 @@ __device__ __forceinline__ void close_step(SsspState* S) {
 // minimum as the new first bucket and sets the phase's width from the items
 // the last phase admitted; with far items left but no live far bucket
 // (minimum unset) every far item is settled and the run is done.
-__global__ void step_kernel(SsspState* S, uint32_t defer) {
+__global__ void step_kernel(SsspState* S, uint32_t defer, uint32_t hub_span) {
   if (threadIdx.x != 0) return;
   close_step(S);
   if (S->done || S->overflow) return;
@@ -205,7 +228,10 @@ __global__ void step_kernel(SsspState* S, uint32_t defer) {
     S->mode = kRelax;
   } else if (defer && S->hub_count > 0) {
     S->mode = kHubs;
-    S->hub_expand = S->hub_count;
+    S->hub_expand = 0;  // counted by hub_select_kernel
+    S->hub_kept = 0;
+    S->hub_min_next = 0xffffffffu;
+    S->hub_limit = (hub_span == 0 || S->hub_min >= 0xffffffffu - hub_span) ? 0xffffffffu : S->hub_min + (hub_span - 1u);
   } else if (S->far_count[S->far_sel] > 0 && S->far_min != 0xffffffffu) {
     S->mode = kAdvance;
     const unsigned long long admitted = S->items - S->phase_items0;
@@ -248,6 +274,7 @@ struct RelaxOp {
   unsigned* far_count;
   uint32_t bucket_end, round, phase, hub_gen;  // bucket_end: last bucket of the phase
   uint32_t far_min = 0xffffffffu;  // this lane's smallest far bucket improved into
+  uint32_t hub_min = 0xffffffffu;  // this lane's smallest distance of a hub it listed
   unsigned long long edges = 0, items = 0;
   // Stale-item filter (the reference's test at SSSP.cpp:138): a near item is
   // expanded only if its distance changed since it was last admitted.  A
@@ -258,7 +285,12 @@ struct RelaxOp {
     if (B.done[u] == d) return false;
     B.done[u] = d;
     const uint32_t deg = static_cast<uint32_t>(B.row[u + 1] - B.row[u]);
-    if (deg > B.hub) return !B.defer || atomicExch(&B.hubmark[u], hub_gen) != hub_gen;
+    if (deg > B.hub) {
+      if (!B.defer) return true;
+      if (atomicExch(&B.hubmark[u], hub_gen) == hub_gen) return false;
+      hub_min = min(hub_min, d);
+      return true;
+    }
     edges += deg;
     items += 1;
     return true;
@@ -294,10 +326,11 @@ struct RelaxOp {
   // once per wavefront.
   __device__ __forceinline__ void finish() {
     flush(true);
-    const uint32_t fm = wave_min(far_min);
+    const uint32_t fm = wave_min(far_min), hm = wave_min(hub_min);
     const unsigned long long ed = wave_sum(edges), it = wave_sum(items);
     if (lane_id() == 0) {
       if (fm != 0xffffffffu) atomicMin(&B.S->far_min, fm);
+      if (hm != 0xffffffffu) atomicMin(&B.S->hub_min, hm);
       if (ed) atomicAdd(&B.S->edges, ed);  // hub steps count edges in every workgroup, items in one
       if (it) atomicAdd(&B.S->items, it);
     }
@@ -327,7 +360,7 @@ __device__ __forceinline__ void relax_step(const Bufs& B, LdsQueue<kQCap>& qn, L
   SsspState* S = B.S;
   RelaxOp op = make_relax(B, &qn, &qf);
   const unsigned q = S->q;
-  expand_queue<kBlock, RelaxOp, true>(B.row, B.nearq[q], S->near_count[q], B.hub, B.hubq, &S->hub_count, B.cap,
+  expand_queue<kBlock, RelaxOp, true>(B.row, B.nearq[q], S->near_count[q], B.hub, B.hubq[S->hq], &S->hub_count, B.cap,
                                       &S->overflow, op);
   op.finish();
 }
@@ -344,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
   lq_init(qf);
   RelaxOp op = make_relax(B, &qn, &qf);
   HubCount<RelaxOp> hc{op};
-  expand_hubs<kBlock>(B.row, B.hubq, hubs, B.cap, B.hub_off, hc);
+  expand_hubs<kBlock>(B.row, B.defer ? B.hubx : B.hubq[0], hubs, B.cap, B.hub_off, hc);
   if (blockIdx.x == 0 && threadIdx.x == 0) op.items += min(hubs, B.cap);
   op.finish();
 }
@@ -386,6 +419,40 @@ __device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& q
   if (lane_id() == 0 && kept_min != 0xffffffffu) atomicMin(&S->far_min, kept_min);
 }
 
+// Hub step, first part: the pending hubs at or below hub_limit go to the
+// expand list; the others move to the other pending list under the next
+// generation's mark (the generation advances when the step closes), so the
+// admission filter does not list them twice, and their smallest distance
+// becomes the next step's hub_min.
+__global__ void __launch_bounds__(kBlock) hub_select_kernel(Bufs B) {
+  SsspState* S = B.S;
+  if (S->mode != kHubs) return;
+  const uint32_t* __restrict__ dist = S->dist_out;
+  const unsigned cnt = min(S->hub_count, B.cap), hq = S->hq, gen = S->hub_gen + 1;
+  const uint32_t limit = S->hub_limit;
+  const uint32_t* pending = B.hubq[hq];
+  uint32_t kept_min = 0xffffffffu;
+  for (unsigned b0 = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1u)); b0 < cnt; b0 += gridDim.x * kBlock) {
+    const unsigned i = b0 + lane_id();
+    bool sel = false, keep = false;
+    uint32_t h = 0;
+    if (i < cnt) {
+      h = pending[i];
+      const uint32_t d = dist[h];
+      sel = d <= limit;
+      keep = !sel;
+      if (keep) {
+        B.hubmark[h] = gen;
+        kept_min = min(kept_min, d);
+      }
+    }
+    wave_append<uint32_t>(sel, h, B.hubx, &S->hub_expand, B.cap, &S->overflow);
+    wave_append<uint32_t>(keep, h, B.hubq[hq ^ 1], &S->hub_kept, B.cap, &S->overflow);
+  }
+  kept_min = wave_min(kept_min);
+  if (lane_id() == 0 && kept_min != 0xffffffffu) atomicMin(&S->hub_min_next, kept_min);
+}
+
 // The work of a step: a relax or an advance (the step's mode picks one; the
 // two never share a step, so one launch serves both).
 __global__ void __launch_bounds__(kBlock) work_kernel(Bufs B) {
@@ -403,10 +470,12 @@ __global__ void __launch_bounds__(kBlock) work_kernel(Bufs B) {
 // `hubs` false: no vertex exceeds the hub threshold, so the hub list stays
 // empty and its two launches are left out of the step.
 void enqueue_step(const Bufs& B, bool hubs, hipStream_t s) {
-  step_kernel<<<1, 64, 0, s>>>(B.S, B.defer);
+  step_kernel<<<1, 64, 0, s>>>(B.S, B.defer, B.hub_span);
   work_kernel<<<hubs ? kGrid : kGridNoHubs, kBlock, 0, s>>>(B);
   if (hubs) {
-    launch_hub_offsets<kBlock>(B.row, B.hubq, B.defer ? &B.S->hub_expand : &B.S->hub_count, B.cap, B.hub_off, s);
+    if (B.defer) hub_select_kernel<<<kHubSelectGrid, kBlock, 0, s>>>(B);
+    launch_hub_offsets<kBlock>(B.row, B.defer ? B.hubx : B.hubq[0], B.defer ? &B.S->hub_expand : &B.S->hub_count,
+                               B.cap, B.hub_off, s);
     hubs_kernel<<<kGrid, kBlock, 0, s>>>(B);
   }
   GX_LAUNCHED("sssp step");
@@ -433,7 +502,9 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.nearq[1] = g->scratch<uint32_t>(WS_Q1, cap);
   B.farq[0] = g->scratch<uint32_t>(WS_Q2, cap);
   B.farq[1] = g->scratch<uint32_t>(WS_Q3, cap);
-  B.hubq = g->scratch<uint32_t>(WS_V0, cap);
+  B.hubq[0] = g->scratch<uint32_t>(WS_V0, cap);
+  B.hubq[1] = g->scratch<uint32_t>(WS_BITMAP0, cap);
+  B.hubx = g->scratch<uint32_t>(WS_BITMAP1, cap);
   B.stamp = g->scratch<uint32_t>(WS_V1, cap);
   B.farmark = g->scratch<uint32_t>(WS_V2, cap);
   B.done = g->scratch<uint32_t>(WS_V3, cap);
@@ -443,7 +514,9 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.cap = cap;
   B.hub_off = g->scratch<uint32_t>(WS_HUBOFF, hub_off_words(cap));
   B.hub = hub_degree();
-  B.defer = static_cast<uint32_t>(tune_effective("sssp.defer_hubs", n, g->m));
+  const int64_t refine = tune_effective("sssp.hub_refine", n, g->m);
+  B.defer = static_cast<uint32_t>(tune_effective("sssp.defer_hubs", n, g->m)) | (refine > 0 ? 1u : 0u);
+  B.hub_span = refine > 0 ? 1u << (refine >= static_cast<int64_t>(shift) ? 0 : shift - static_cast<uint32_t>(refine)) : 0u;
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
   // Graphs with hubs run on the degree-ordered copy (frontier.relabel).
   const bool relabel = hubs && tune_effective("frontier.relabel", n, g->m) != 0;
@@ -469,10 +542,12 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   uint64_t key = 0;
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.w,
                         (const void*)B.nearq[0], (const void*)B.nearq[1], (const void*)B.farq[0],
-                        (const void*)B.farq[1], (const void*)B.hubq, (const void*)B.stamp, (const void*)B.farmark,
+                        (const void*)B.farq[1], (const void*)B.hubq[0], (const void*)B.hubq[1], (const void*)B.hubx,
+                        (const void*)B.stamp, (const void*)B.farmark,
                         (const void*)B.done, (const void*)B.hubmark, (const void*)B.S, (const void*)B.hub_off})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
-  key = mix_key(mix_key(mix_key(mix_key(mix_key(key, shift), cap), B.hub), hubs ? 1u : 0u), B.defer);
+  key = mix_key(mix_key(mix_key(mix_key(mix_key(mix_key(key, shift), cap), B.hub), hubs ? 1u : 0u), B.defer),
+                B.hub_span);
   uint32_t replays = 0;
   for (;;) {
     ktimer.begin();
@@ -493,7 +568,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   if (st) {
     std::memset(st, 0, sizeof(*st));
     st->iterations = host->steps;
-    st->launches = 2 + replays * (kStepsPerGraph * (hubs ? 4 : 2) + 1);
+    st->launches = 2 + replays * (kStepsPerGraph * (hubs ? (B.defer ? 5 : 4) : 2) + 1);
     st->edges_visited = host->edges;
     st->items = host->items;
     st->elapsed_ms = ms;

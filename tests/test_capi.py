@@ -228,6 +228,14 @@ def test_tuning_knobs():
     assert gx.tune_effective("bfs.visited_bitmap", (1 << 25) - 1, 0) == 0
     assert gx.tune_effective("bfs.visited_bitmap", 1 << 25, 0) == 1
     assert gx.tune_effective("frontier.hub_degree", 1, 1) == 128
+    for key in ("frontier.relabel", "sssp.hub_refine"):
+        assert gx.tune_effective(key, (1 << 24) - 1, 0) == 0, key
+        assert gx.tune_effective(key, 1 << 24, 0) != 0, key
+    gx.tune_set("sssp.hub_refine", 3)
+    assert gx.tune_effective("sssp.hub_refine", 1 << 26, 0) == 3
+    gx.tune_set("sssp.hub_refine", -1)
+    with pytest.raises(GxError):
+        gx.tune_set("sssp.hub_refine", 32)
     assert gx.tune_effective("pr.blocking", 1, 1) == 1
     with pytest.raises(GxError):
         gx.tune_effective("no.such.knob", 1, 1)

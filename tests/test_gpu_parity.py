@@ -434,17 +434,20 @@ def test_hub_degree_knob(gx, oracle, rmat16, hub):
     np.testing.assert_array_equal(u32(deg), oracle.kcore(row, col, 10, threads=8))
 
 
-@pytest.mark.parametrize("merge,defer", [(1, 0), (4096, 0), (256, 1)])
-def test_sssp_bucket_merge_knob(gx, oracle, rmat16, merge, defer):
+@pytest.mark.parametrize("merge,defer,refine", [(1, 0, 0), (4096, 0, 0), (256, 1, 0), (256, 0, 3), (1, 0, 6),
+                                                 (256, 0, 31)])
+def test_sssp_bucket_merge_knob(gx, oracle, rmat16, merge, defer, refine):
     """Distances do not depend on how many buckets a phase may merge
-    (sssp.max_merge; 1 = one bucket per phase) or on deferring hubs
-    (sssp.defer_hubs)."""
+    (sssp.max_merge; 1 = one bucket per phase), on deferring hubs
+    (sssp.defer_hubs) or on the distance slices hub steps expand
+    (sssp.hub_refine; 31 = one distance per hub step)."""
     G, row, col, w = rmat16
     Gg = gx.Graph.grid(256)
     grow, gcol, gw = Gg.to_host()
-    say(f"sssp max_merge={merge} defer_hubs={defer}")
+    say(f"sssp max_merge={merge} defer_hubs={defer} hub_refine={refine}")
     gx.tune_set("sssp.max_merge", merge)
     gx.tune_set("sssp.defer_hubs", defer)
+    gx.tune_set("sssp.hub_refine", refine)
     try:
         d, _ = gx.sssp(G, 0, 6)
         dg, stg = gx.sssp(Gg, 0, 6)
@@ -452,6 +455,7 @@ def test_sssp_bucket_merge_knob(gx, oracle, rmat16, merge, defer):
     finally:
         gx.tune_set("sssp.max_merge", -1)
         gx.tune_set("sssp.defer_hubs", -1)
+        gx.tune_set("sssp.hub_refine", -1)
     np.testing.assert_array_equal(u32(d), oracle.sssp(row, col, w, 0, 6, threads=8))
     np.testing.assert_array_equal(u32(dg), oracle.sssp(grow, gcol, gw, 0, 6, threads=8))
     np.testing.assert_array_equal(u32(dg2), oracle.sssp(grow, gcol, gw, 77, 0, threads=8))
