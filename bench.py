@@ -114,7 +114,7 @@ def cpu_baseline_pagerank(G, iters: int, reps: int = 5) -> dict:
     fn.restype = C.c_int
     fn.argtypes = [C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_double, C.c_double, C.c_uint32,
                    C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
-    row, _, _ = G.to_host()
+    row = G.row_host()
     trow, tcol, _ = G.transpose_host()
     outdeg = np.diff(row).astype(np.uint32)
     x = np.empty(G.n, np.float64)
@@ -531,7 +531,7 @@ def main() -> int:
 
     if args.frontier_pass:
         G = gx.Graph.rmat(args.scale, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
-        row, _, _ = G.to_host()
+        row = G.row_host()
         src = int(np.argmax(np.diff(row)))
         del row
         for name, fn in (("kcore", lambda: gx.kcore(G, 10)), ("sssp", lambda: gx.sssp(G, 0, 6)),
