@@ -40,11 +40,11 @@
 //    inside a bucket is expanded once per drain rather than once per
 //    improvement.  With sssp.hub_refine a hub step expands only the pending
 //    hubs within a distance slice above the smallest pending hub distance
-//    (one distance by default) and keeps the rest pending: the non-hub
+//    (two distances by default) and keeps the rest pending: the non-hub
 //    frontier drains between hub steps, so hubs go out nearly in distance
 //    order, almost all at their final distance, and a hub's hundreds of
 //    thousands of edges are relaxed about once instead of once per
-//    improvement (RMAT-26: 1.84 -> 1.12 relaxations per edge).  Hub steps
+//    improvement (RMAT-26: 1.84 -> 1.11 relaxations per edge).  Hub steps
 //    are extra dependent steps, which pays on large power-law graphs only,
 //    so the default enables it from 2^24 vertices on.
 //  - Locality: on graphs with hubs (knob frontier.relabel) the kernels run
@@ -138,8 +138,8 @@ struct Bufs {
   uint32_t* hub_off;
   uint32_t hub;  // frontier.hub_degree
   uint32_t defer;  // sssp.defer_hubs: pending hub list expanded in hub steps
-  // sssp.hub_refine: width of the distance slice a hub step expands
-  // (pending hubs within hub_span of the smallest); 0 = all pending hubs.
+  // sssp.hub_refine r: a hub step expands the pending hubs within
+  // hub_span = 2^(r - 1) of the smallest pending hub distance; 0 = all.
   uint32_t hub_span;
 };
 
@@ -516,7 +516,7 @@ extern "C" gx_status gx_sssp_delta(gx_graph* g, uint32_t src, uint32_t shift, ui
   B.hub = hub_degree();
   const int64_t refine = tune_effective("sssp.hub_refine", n, g->m);
   B.defer = static_cast<uint32_t>(tune_effective("sssp.defer_hubs", n, g->m)) | (refine > 0 ? 1u : 0u);
-  B.hub_span = refine > 0 ? 1u << (refine >= static_cast<int64_t>(shift) ? 0 : shift - static_cast<uint32_t>(refine)) : 0u;
+  B.hub_span = refine > 0 ? 1u << std::min<int64_t>(refine - 1, 31) : 0u;
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
   // Graphs with hubs run on the degree-ordered copy (frontier.relabel).
   const bool relabel = hubs && tune_effective("frontier.relabel", n, g->m) != 0;

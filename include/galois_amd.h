@@ -136,11 +136,11 @@ void gx_set_profiling(int on);
  *                          the same step
  *   "sssp.hub_refine"      gx_sssp_delta: r > 0 defers hubs as above and
  *                          expands, per hub step, only the pending hubs
- *                          within 2^(shift - r) (at least 1) of the smallest
- *                          pending hub distance, so that hubs go out nearly
- *                          in distance order and most are expanded once;
- *                          0 = off; default 31 (one distance per hub step)
- *                          from 2^24 vertices on, else 0; range [0, 31]
+ *                          within 2^(r - 1) of the smallest pending hub
+ *                          distance, so that hubs go out nearly in distance
+ *                          order and most are expanded once; 0 = off;
+ *                          default 2 (two distances per hub step) from 2^24
+ *                          vertices on, else 0; range [0, 32]
  *   "sssp.batch_width"     gx_sssp_batched: buckets per phase, shared by
  *                          the 64 sources of a wavefront; 0 = adaptive as
  *                          gx_sssp_delta's phases; default 64

@@ -235,7 +235,7 @@ def test_tuning_knobs():
     assert gx.tune_effective("sssp.hub_refine", 1 << 26, 0) == 3
     gx.tune_set("sssp.hub_refine", -1)
     with pytest.raises(GxError):
-        gx.tune_set("sssp.hub_refine", 32)
+        gx.tune_set("sssp.hub_refine", 33)
     assert gx.tune_effective("pr.blocking", 1, 1) == 1
     with pytest.raises(GxError):
         gx.tune_effective("no.such.knob", 1, 1)

@@ -434,13 +434,13 @@ def test_hub_degree_knob(gx, oracle, rmat16, hub):
     np.testing.assert_array_equal(u32(deg), oracle.kcore(row, col, 10, threads=8))
 
 
-@pytest.mark.parametrize("merge,defer,refine", [(1, 0, 0), (4096, 0, 0), (256, 1, 0), (256, 0, 3), (1, 0, 6),
-                                                 (256, 0, 31)])
+@pytest.mark.parametrize("merge,defer,refine", [(1, 0, 0), (4096, 0, 0), (256, 1, 0), (256, 0, 1), (1, 0, 2),
+                                                 (256, 0, 7), (256, 0, 32)])
 def test_sssp_bucket_merge_knob(gx, oracle, rmat16, merge, defer, refine):
     """Distances do not depend on how many buckets a phase may merge
     (sssp.max_merge; 1 = one bucket per phase), on deferring hubs
     (sssp.defer_hubs) or on the distance slices hub steps expand
-    (sssp.hub_refine; 31 = one distance per hub step)."""
+    (sssp.hub_refine; 1 = one distance per hub step, 32 = all pending)."""
     G, row, col, w = rmat16
     Gg = gx.Graph.grid(256)
     grow, gcol, gw = Gg.to_host()
