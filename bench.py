@@ -516,8 +516,8 @@ def main() -> int:
                          "device copy (the calibration reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(the per-kernel sharding design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: there is no multi-GPU path (the kernels shard by "
+              "vertex range, a design that is not built); use --gpus 1", file=sys.stderr)
         return 2
 
     import torch
