@@ -19,6 +19,9 @@ Knobs and candidates:
                        candidate: it changes FP64 results in the last bit)
   pr.hot_sources       sources of the first PageRank pull pass (0 = one pass)
   sssp.max_merge       most delta buckets one SSSP phase may cover
+  sssp.hub_refine      SSSP hub steps: 0 = hubs expanded in the step that
+                       admits them, r > 0 = deferred and expanded 2^(r-1)
+                       distances per hub step
   bc.chunk             batched BC: edges per work item, the degree-bin boundary
                        between whole-adjacency items and split ones
   bc.grid              batched BC: workgroups of each level sweep
@@ -37,6 +40,7 @@ CANDIDATES: dict[str, list[int]] = {
     "pr.blocking": [1, 0],
     "pr.hot_sources": [0, 1 << 17, 1 << 18, 1 << 19, 1 << 20, 1 << 21],
     "sssp.max_merge": [1, 16, 64, 256, 1024],
+    "sssp.hub_refine": [0, 1, 2, 3],
     "bc.chunk": [64, 128, 256, 512, 1024],
     "bc.grid": [2048, 4096, 8192, 16384],
 }
@@ -44,7 +48,7 @@ CANDIDATES: dict[str, list[int]] = {
 # Knobs each algorithm responds to.
 KNOBS: dict[str, list[str]] = {
     "bfs": ["frontier.hub_degree"],
-    "sssp": ["frontier.hub_degree", "sssp.max_merge"],
+    "sssp": ["frontier.hub_degree", "sssp.hub_refine", "sssp.max_merge"],
     "cc": ["frontier.hub_degree"],
     "kcore": ["frontier.hub_degree"],
     "bc": ["bc.chunk", "bc.grid"],

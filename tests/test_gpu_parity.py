@@ -816,8 +816,12 @@ def test_autotuner_picks_and_caches(gx, tmp_path):
         c4 = autotune.tune(G, "bc", cache=cache, sources=gx.pick_sources(row, 70))
         assert c4["bc.chunk"] in autotune.CANDIDATES["bc.chunk"] and c4["bc.grid"] in autotune.CANDIDATES["bc.grid"]
         assert gx.tune_get("bc.chunk") == c4["bc.chunk"]
+        Gw = gx.Graph.rmat(14, 16, symmetric=True, weights="uniform", wlo=1, whi=255)
+        c5 = autotune.tune(Gw, "sssp", cache=cache, source=0)
+        for knob in ("frontier.hub_degree", "sssp.hub_refine", "sssp.max_merge"):
+            assert c5[knob] in autotune.CANDIDATES[knob] and gx.tune_get(knob) == c5[knob], knob
         autotune.reset()
         assert autotune.tune(G, "bfs", cache=cache, source=0) == c1  # from the cache
-        assert len(__import__("json").loads(cache.read_text())) == 4  # FP32 and FP64 keyed apart
+        assert len(__import__("json").loads(cache.read_text())) == 5  # FP32 and FP64 keyed apart
     finally:
         autotune.reset()
