@@ -221,9 +221,6 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   RegionTimer region(s);
   KernelTimer ktimer(s);
   uint32_t launches = 2;
-  kcore_reset_kernel<<<1, 64, 0, s>>>(c);
-  kcore_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(g->row, degree, n, k, q[0], c, cap);
-  GX_LAUNCHED("kcore_init_kernel");
   // Rounds are driven on the device: rounds (begin, peel, hub layout, hubs,
   // end) captured in HIP graphs, kRoundsFirst for the first replay and
   // kRoundsPerGraph for the next ones, replayed until the state says done,
@@ -232,6 +229,20 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   // No vertex above the hub threshold: the hub list stays empty and its two
   // launches are left out of each round.
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
+  // Graphs with hubs peel on the degree-ordered copy of the CSR
+  // (frontier.relabel): the decrements mostly land on the hubs, whose
+  // degree words then share a few cached megabytes; the degrees are mapped
+  // back before the membership pass.
+  const bool relabel = hubs && tune_effective("frontier.relabel", n, g->m) != 0;
+  if (relabel) {
+    build_relabeled(g);
+    B.row = g->relab.row.as<int32_t>();
+    B.col = g->relab.col.as<int32_t>();
+    B.degree = g->scratch<uint32_t>(WS_V5, cap);
+  }
+  kcore_reset_kernel<<<1, 64, 0, s>>>(c);
+  kcore_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(B.row, B.degree, n, k, q[0], c, cap);
+  GX_LAUNCHED("kcore_init_kernel");
   uint64_t key = 0;
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.degree, (const void*)B.q0,
                         (const void*)B.q1, (const void*)B.hubq, (const void*)B.hub_off, (const void*)B.c})
@@ -268,6 +279,10 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   const uint32_t rounds = host->rounds;
   const uint64_t items = host->items;
   uint64_t edges = 0;
+  if (relabel) {
+    unpermute_u32(B.degree, g->relab.new_of_old.as<uint32_t>(), degree, n, s);
+    launches += 1;
+  }
   kcore_finish_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(degree, alive, n, k, c);
   GX_LAUNCHED("kcore_finish_kernel");
   launches += 1;

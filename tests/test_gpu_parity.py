@@ -461,6 +461,32 @@ def test_sssp_bucket_merge_knob(gx, oracle, rmat16, merge, defer, refine):
     np.testing.assert_array_equal(u32(dg2), oracle.sssp(grow, gcol, gw, 77, 0, threads=8))
 
 
+@pytest.mark.parametrize("refine", [0, 2])
+def test_degree_ordered_copy_matches_oracle(gx, oracle, rmat16, refine):
+    """BFS, SSSP and k-core on the degree-ordered copy of the CSR
+    (frontier.relabel = 1; on by default only from 2^24 vertices) give the
+    oracle's levels, distances and core degrees in the caller's vertex ids,
+    with and without the distance-ordered hub steps (sssp.hub_refine)."""
+    G, row, col, w = rmat16
+    srcs = [0, int(np.argmax(np.diff(row))), 12345]
+    say(f"rmat16 frontier.relabel=1 hub_refine={refine} sources={srcs}")
+    gx.tune_set("frontier.relabel", 1)
+    gx.tune_set("sssp.hub_refine", refine)
+    try:
+        got = [(u32(gx.bfs(G, s)[0]), u32(gx.sssp(G, s, 6)[0])) for s in srcs]
+        deg, alive, count, _ = gx.kcore(G, 10)
+    finally:
+        gx.tune_set("frontier.relabel", -1)
+        gx.tune_set("sssp.hub_refine", -1)
+    for s, (lv, d) in zip(srcs, got):
+        np.testing.assert_array_equal(lv, oracle.bfs(row, col, s, threads=8), err_msg=f"bfs from {s}")
+        np.testing.assert_array_equal(d, oracle.sssp(row, col, w, s, 6, threads=8), err_msg=f"sssp from {s}")
+    exp = oracle.kcore(row, col, 10, threads=8)
+    np.testing.assert_array_equal(u32(deg), exp)
+    np.testing.assert_array_equal(alive.cpu().numpy(), (exp >= 10).astype(np.uint8))
+    assert count == int((exp >= 10).sum())
+
+
 def _sssp_rows_match(gx, oracle, G, row, col, w, srcs, shift):
     got, st = gx.sssp_batched(G, srcs, shift)
     got = got.cpu().numpy().view(np.uint32)
