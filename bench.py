@@ -60,6 +60,18 @@ def sync():
     torch.cuda.synchronize()
 
 
+def frontier_traffic() -> dict:
+    """Measured HBM bytes per run of each frontier algorithm on symmetric
+    RMAT-26 from the committed rocprofv3 PMC summary (FETCH_SIZE and
+    WRITE_SIZE passes of `--frontier-pass --frontier-only <algorithm>`,
+    calibrated against a copy kernel), or {} when it is absent."""
+    f = ROOT / "profiles" / "frontier_rmat26_traffic.json"
+    try:
+        return {k: v["bytes_per_run"] for k, v in json.loads(f.read_text())["per_algorithm"].items()}
+    except (OSError, ValueError, KeyError, TypeError):
+        return {}
+
+
 def measured_traffic(scale: int, sweep: str):
     """HBM bytes per PageRank sweep from the committed rocprofv3 PMC summary
     of that sweep (FETCH_SIZE and WRITE_SIZE passes, calibrated against a copy
@@ -461,6 +473,7 @@ def rmat26_table(gx, O) -> dict:
     cpu = {"bfs": lambda: O.bfs(row, col, src, CPU_THREADS), "sssp": lambda: O.sssp(row, col, wts, 0, 6, CPU_THREADS),
            "cc": lambda: O.cc(row, col, CPU_THREADS), "kcore10": lambda: O.kcore(row, col, 10, CPU_THREADS)}
     ref = {}
+    traffic = frontier_traffic()
     runs = {"bfs": lambda: gx.bfs(G, src), "bfs_do": lambda: gx.bfs_do(G, src), "sssp": lambda: gx.sssp(G, 0, 6),
             "cc": lambda: gx.connected_components(G), "kcore10": lambda: gx.kcore(G, 10)}
     for kname, fn in runs.items():
@@ -485,6 +498,11 @@ def rmat26_table(gx, O) -> dict:
         if base not in ref:
             ref[base] = cpu_time(cpu[base], 3)
         cms, exp = ref[base]
+        tkey = "kcore" if kname == "kcore10" else kname
+        if tkey in traffic:  # measured bytes of the same kernels (profiles/frontier_rmat26_traffic.json)
+            extra.update({"pct_hbm_basis": "algorithmic bytes (work-equivalent for DO-BFS and CC)",
+                          "measured_hbm_GB": round(traffic[tkey] / 1e9, 2),
+                          "pct_hbm_measured": pct(traffic[tkey], st.elapsed_ms)})
         out[f"{kname}_rmat26"] = {"ms": round(st.elapsed_ms, 4), "GTEPS": round(m / st.elapsed_ms / 1e6, 3),
                                   "pct_hbm": pct(byt, st.elapsed_ms), "rounds": st.iterations, "n": n, "m": m,
                                   **extra, **cpu_row(cms, m, 3, np.array_equal(res[0].cpu().numpy().view(np.uint32),
@@ -511,6 +529,9 @@ def main() -> int:
                          "(k=10), SSSP (delta 64), top-down and direction-optimising BFS and CC on symmetric "
                          "weighted RMAT at --scale, each twice, then a 1 GiB device copy (the calibration "
                          "reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
+    ap.add_argument("--frontier-only", default="",
+                    help="comma-separated subset of the frontier pass (kcore,sssp,bfs,bfs_do,cc), so that counters "
+                         "of kernels shared by several algorithms can be attributed to one")
     ap.add_argument("--pmc-pass", action="store_true",
                     help="workload for rocprofv3 --pmc runs: 3 pull sweeps on the headline graph, then a 1 GiB "
                          "device copy (the calibration reference for FETCH_SIZE/WRITE_SIZE); prints no JSON")
@@ -534,9 +555,12 @@ def main() -> int:
         row = G.row_host()
         src = int(np.argmax(np.diff(row)))
         del row
+        only = set(args.frontier_only.split(",")) if args.frontier_only else None
         for name, fn in (("kcore", lambda: gx.kcore(G, 10)), ("sssp", lambda: gx.sssp(G, 0, 6)),
                          ("bfs", lambda: gx.bfs(G, src)), ("bfs_do", lambda: gx.bfs_do(G, src)),
                          ("cc", lambda: gx.connected_components(G))):
+            if only is not None and name not in only:
+                continue
             fn()
             st = fn()[-1]
             log(f"frontier pass: {name} {st.elapsed_ms:.2f} ms, {st.iterations} rounds, "
