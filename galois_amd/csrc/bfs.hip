@@ -429,6 +429,18 @@ extern "C" gx_status gx_bfs(gx_graph* g, uint32_t src, uint32_t* level, gx_stats
   GX_API_END
 }
 
+// Parents back to the caller's vertex ids: out[v] = old_of_new[p] for the
+// parent p of v's degree-ordered id (the source's own id, or none).
+__global__ void unpermute_parent_kernel(const uint32_t* __restrict__ pin, const uint32_t* __restrict__ new_of_old,
+                                        const uint32_t* __restrict__ old_of_new, uint32_t* __restrict__ out,
+                                        int64_t n) {
+  for (int64_t v = blockIdx.x * static_cast<int64_t>(blockDim.x) + threadIdx.x; v < n;
+       v += static_cast<int64_t>(gridDim.x) * blockDim.x) {
+    const uint32_t p = pin[new_of_old[v]];
+    out[v] = p == kNoParent ? p : old_of_new[p];
+  }
+}
+
 extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32_t beta, uint32_t* level,
                                uint32_t* parent, gx_stats* st) {
   GX_API_BEGIN
@@ -453,6 +465,25 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   const int64_t words = (n + 31) / 32;
   uint32_t* bm[2] = {g->scratch<uint32_t>(WS_BITMAP0, words), g->scratch<uint32_t>(WS_BITMAP1, words)};
   const bool hubs = max_degree(g) > static_cast<int64_t>(B.hub);
+  // Symmetric graphs with hubs run on the degree-ordered copy of the CSR
+  // (frontier.relabel): a bottom-up step scans each unvisited vertex's
+  // neighbours hubs first, and the hubs are the first vertices reached, so
+  // most scans stop at their first neighbour; levels and parents are mapped
+  // back at the end.
+  const bool relabel = (g->flags & GX_GRAPH_SYMMETRIC) && hubs && tune_effective("frontier.relabel", n, g->m) != 0;
+  const int32_t* row = g->row;
+  uint32_t run_src = src;
+  if (relabel) {
+    build_relabeled(g);
+    row = g->relab.row.as<int32_t>();
+    trow = row;
+    tcol = g->relab.col.as<int32_t>();
+    B.row = row;
+    B.col = tcol;
+    B.level = g->scratch<uint32_t>(WS_V5, cap);
+    B.parent = parent ? g->scratch<uint32_t>(WS_V4, cap) : nullptr;
+    run_src = read_scalar(g, g->relab.new_of_old.as<uint32_t>() + src);
+  }
   BfsState* host = static_cast<BfsState*>(g->host_pinned(sizeof(BfsState)));
   auto read_state = [&]() {
     GX_HIP(hipMemcpyAsync(host, B.S, sizeof(BfsState), hipMemcpyDeviceToHost, s));
@@ -462,11 +493,11 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
   RegionTimer region(s);
   KernelTimer ktimer(s);
   uint32_t launches = 2;
-  bfs_init_kernel<<<sweep_grid(n), kBlock, 0, s>>>(level, parent, n, src);
-  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], src, level, parent, nullptr);
+  bfs_init_kernel<<<sweep_grid(n), kBlock, 0, s>>>(B.level, B.parent, n, run_src);
+  bfs_reset_kernel<<<1, 64, 0, s>>>(B.S, B.q[0], run_src, B.level, B.parent, nullptr);
   GX_LAUNCHED("bfs init");
   int32_t rsrc[2];
-  read_bytes(g, rsrc, g->row + src, sizeof(rsrc));
+  read_bytes(g, rsrc, row + run_src, sizeof(rsrc));
   int64_t scout = rsrc[1] - rsrc[0];
   int64_t edges_to_check = g->m;
   uint64_t edges_visited = static_cast<uint64_t>(scout), items = 1;
@@ -489,8 +520,8 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
         fill_kernel<uint32_t><<<sweep_grid(words), kBlock, 0, s>>>(bm[b ^ 1], 0u, words);
         do_pull_begin_kernel<<<1, 64, 0, s>>>(B.S);
         ktimer.begin();
-        bu_step_kernel<<<sweep_grid(n), kBlock, 0, s>>>(trow, tcol, g->row, level, parent, bm[b], bm[b ^ 1], B.S, n,
-                                                        lvl);
+        bu_step_kernel<<<sweep_grid(n), kBlock, 0, s>>>(trow, tcol, row, B.level, B.parent, bm[b], bm[b ^ 1], B.S,
+                                                        n, lvl);
         ktimer.end();
         GX_LAUNCHED("bu_step_kernel");
         launches += 3;
@@ -525,6 +556,15 @@ extern "C" gx_status gx_bfs_do(gx_graph* g, uint32_t src, uint32_t alpha, uint32
     items += frontier;
     scout = static_cast<int64_t>(host->scout);
     edges_visited += host->scout;
+  }
+  if (relabel) {
+    unpermute_u32(B.level, g->relab.new_of_old.as<uint32_t>(), level, n, s);
+    if (parent) {
+      unpermute_parent_kernel<<<grid_for(n, kBlock, 8192), kBlock, 0, s>>>(
+          B.parent, g->relab.new_of_old.as<uint32_t>(), g->relab.old_of_new.as<uint32_t>(), parent, n);
+      GX_LAUNCHED("unpermute_parent_kernel");
+    }
+    launches += parent ? 2 : 1;
   }
   double ms = region.stop();
   if (st) {

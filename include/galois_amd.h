@@ -144,7 +144,8 @@ void gx_set_profiling(int on);
  *   "sssp.batch_width"     gx_sssp_batched: buckets per phase, shared by
  *                          the 64 sources of a wavefront; 0 = adaptive as
  *                          gx_sssp_delta's phases; default 64
- *   "frontier.relabel"     gx_sssp_delta, gx_bfs and gx_kcore on graphs with hubs:
+ *   "frontier.relabel"     gx_sssp_delta, gx_bfs, gx_bfs_do (symmetric graphs)
+ *                          and gx_kcore on graphs with hubs:
  *                          1 = run on a degree-ordered copy of the CSR
  *                          built once per graph (hot vertices' words
  *                          contiguous), 0 = on the CSR as given; default 1

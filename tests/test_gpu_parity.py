@@ -463,10 +463,11 @@ def test_sssp_bucket_merge_knob(gx, oracle, rmat16, merge, defer, refine):
 
 @pytest.mark.parametrize("refine", [0, 2])
 def test_degree_ordered_copy_matches_oracle(gx, oracle, rmat16, refine):
-    """BFS, SSSP and k-core on the degree-ordered copy of the CSR
-    (frontier.relabel = 1; on by default only from 2^24 vertices) give the
-    oracle's levels, distances and core degrees in the caller's vertex ids,
-    with and without the distance-ordered hub steps (sssp.hub_refine)."""
+    """BFS, direction-optimising BFS, SSSP and k-core on the degree-ordered
+    copy of the CSR (frontier.relabel = 1; on by default only from 2^24
+    vertices) give the oracle's levels, distances and core degrees in the
+    caller's vertex ids, and valid parents, with and without the
+    distance-ordered hub steps (sssp.hub_refine)."""
     G, row, col, w = rmat16
     srcs = [0, int(np.argmax(np.diff(row))), 12345]
     say(f"rmat16 frontier.relabel=1 hub_refine={refine} sources={srcs}")
@@ -474,13 +475,28 @@ def test_degree_ordered_copy_matches_oracle(gx, oracle, rmat16, refine):
     gx.tune_set("sssp.hub_refine", refine)
     try:
         got = [(u32(gx.bfs(G, s)[0]), u32(gx.sssp(G, s, 6)[0])) for s in srcs]
+        dos = [tuple(u32(t) for t in gx.bfs_do(G, s, parents=True)[:2]) for s in srcs]
         deg, alive, count, _ = gx.kcore(G, 10)
     finally:
         gx.tune_set("frontier.relabel", -1)
         gx.tune_set("sssp.hub_refine", -1)
-    for s, (lv, d) in zip(srcs, got):
-        np.testing.assert_array_equal(lv, oracle.bfs(row, col, s, threads=8), err_msg=f"bfs from {s}")
+    n = len(row) - 1
+    src_of = np.repeat(np.arange(n, dtype=np.int64), np.diff(row))
+    edge_keys = np.sort(src_of * n + col.astype(np.int64))
+    for s, (lv, d), (lv2, par) in zip(srcs, got, dos):
+        exp = oracle.bfs(row, col, s, threads=8)
+        np.testing.assert_array_equal(lv, exp, err_msg=f"bfs from {s}")
+        np.testing.assert_array_equal(lv2, exp, err_msg=f"direction-optimising bfs from {s}")
         np.testing.assert_array_equal(d, oracle.sssp(row, col, w, s, 6, threads=8), err_msg=f"sssp from {s}")
+        # parents in the caller's ids: one level up and joined by an edge
+        reached = exp != INF
+        assert par[s] == s and np.all(par[~reached] == 0xFFFFFFFF)
+        v = np.nonzero(reached & (np.arange(n) != s))[0]
+        p = par[v].astype(np.int64)
+        assert np.all(exp[p] == exp[v] - 1)
+        keys = p * n + v
+        at = np.searchsorted(edge_keys, keys)
+        assert np.all(at < len(edge_keys)) and np.all(edge_keys[np.minimum(at, len(edge_keys) - 1)] == keys)
     exp = oracle.kcore(row, col, 10, threads=8)
     np.testing.assert_array_equal(u32(deg), exp)
     np.testing.assert_array_equal(alive.cpu().numpy(), (exp >= 10).astype(np.uint8))
