@@ -50,11 +50,11 @@ int64_t default_off(int64_t, int64_t) { return 0; }
 // 49.5 ms, BFS 17.2 -> 15.5 ms); on RMAT-18 BFS it cost 0.41 -> 0.44 ms and
 // on RMAT-22 SSSP nothing changed.
 int64_t default_relabel(int64_t n, int64_t) { return n >= (int64_t(1) << 24) ? 1 : 0; }
-// Hub steps in distance order, two distances per hub step (r = 2): RMAT-26
-// SSSP 49.7 -> 40.5 ms (1.84 -> 1.11 relaxations per edge; one distance per
-// step 42.6 ms, four 43.2 ms); on RMAT-22 the extra steps cost more than the
-// relaxations saved (5.5 -> 7.9..10.4 ms).
-int64_t default_hub_refine(int64_t n, int64_t) { return n >= (int64_t(1) << 24) ? 2 : 0; }
+// Hubs expanded in distance order, one distance at a time (r = 1), in the
+// relax steps whenever no near item is closer: RMAT-26 SSSP 49.7 -> 39.0 ms
+// (1.84 -> 1.10 relaxations per edge; two distances 39.8 ms); on RMAT-22 the
+// extra steps cost more than the relaxations saved (5.7 -> 6.7 ms).
+int64_t default_hub_refine(int64_t n, int64_t) { return n >= (int64_t(1) << 24) ? 1 : 0; }
 int64_t default_batch_width(int64_t, int64_t) { return 64; }
 int64_t default_bc_grid(int64_t, int64_t) { return 256 * 32; }
 struct Knob {

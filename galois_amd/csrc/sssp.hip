@@ -38,15 +38,19 @@
 //    word per vertex) that is expanded in a step of its own when the near
 //    frontier has drained, so a hub whose distance improves several times
 //    inside a bucket is expanded once per drain rather than once per
-//    improvement.  With sssp.hub_refine a hub step expands only the pending
+//    improvement.  With sssp.hub_refine a step expands only the pending
 //    hubs within a distance slice above the smallest pending hub distance
-//    (two distances by default) and keeps the rest pending: the non-hub
-//    frontier drains between hub steps, so hubs go out nearly in distance
-//    order, almost all at their final distance, and a hub's hundreds of
-//    thousands of edges are relaxed about once instead of once per
-//    improvement (RMAT-26: 1.84 -> 1.11 relaxations per edge).  Hub steps
-//    are extra dependent steps, which pays on large power-law graphs only,
-//    so the default enables it from 2^24 vertices on.
+//    (one distance by default) and keeps the rest pending, so hubs go out
+//    nearly in distance order, almost all at their final distance, and a
+//    hub's hundreds of thousands of edges are relaxed about once instead
+//    of once per improvement (RMAT-26: 1.84 -> 1.10 relaxations per edge).
+//    The hub selection joins a relax step whenever the smallest pending hub
+//    distance is at or below the smallest distance in that step's near
+//    queue (hubs above it wait; positive weights mean nothing left to
+//    expand can lower the hubs taken), and otherwise runs in a step of its
+//    own once the near frontier drains.  The extra steps pay on large
+//    power-law graphs only, so the default enables it from 2^24 vertices
+//    on.
 //  - Locality: on graphs with hubs (knob frontier.relabel) the kernels run
 //    on the degree-ordered copy of the CSR (gx_graph::relab): the hubs and
 //    the vertices most often relaxed get the smallest ids, so their
@@ -109,6 +113,9 @@ struct SsspState {
   uint32_t hub_min;        // smallest distance a pending hub had when it was listed
   uint32_t hub_min_next;   // the same over the hubs a hub step keeps
   uint32_t hub_limit;      // a hub step expands the pending hubs at or below this distance
+  unsigned hub_step;       // this step selects and expands pending hubs
+  uint32_t near_min;       // smallest distance in the near queue the step reads
+  uint32_t near_min_next;  // the same for the near queue the step fills
   unsigned long long phase_items0;  // items admitted before the current phase
   unsigned round;
   unsigned phase;
@@ -179,6 +186,9 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
     S->hq = 0;
     S->hub_kept = 0;
     S->hub_min = S->hub_min_next = S->hub_limit = 0xffffffffu;
+    S->hub_step = 0;
+    S->near_min = 0;  // the source
+    S->near_min_next = 0xffffffffu;
     S->phase_items0 = 0;
     S->round = 0;
     S->phase = 0;
@@ -190,25 +200,40 @@ __global__ void sssp_reset_kernel(SsspState* S, uint32_t* nearq0, uint32_t src, 
 }
 
 // Closes a step: the queue it filled becomes the input; after a split the
-// far piles swap; after a hub step the pending list starts empty under a new
-// generation.
+// far piles swap; after a step that expanded hubs the pending list holds
+// the hubs it kept, under a new generation.
 __device__ __forceinline__ void close_step(SsspState* S) {
-  if (S->mode == kRelax) {
-    S->q ^= 1;
-  } else if (S->mode == kHubs) {
-    S->q ^= 1;
+  if (S->hub_step) {
     S->hq ^= 1;
     S->hub_count = S->hub_kept;  // kept hubs carry the new generation's mark
     S->hub_kept = 0;
     S->hub_expand = 0;
     S->hub_gen += 1;
     S->hub_min = S->hub_min_next;
+    S->hub_step = 0;
+  }
+  if (S->mode == kRelax || S->mode == kHubs) {
+    S->q ^= 1;
   } else if (S->mode == kAdvance) {
     S->far_count[S->far_sel] = 0;
     S->far_sel ^= 1;
     S->q ^= 1;
   }
+  if (S->mode != kIdle) {
+    S->near_min = S->near_min_next;
+    S->near_min_next = 0xffffffffu;
+  }
   S->mode = kIdle;
+}
+
+// Starts the hub part of a step: pending hubs at or below `limit` are
+// selected (hub_select_kernel) and expanded (hubs_kernel).
+__device__ __forceinline__ void begin_hub_step(SsspState* S, uint32_t limit) {
+  S->hub_step = 1;
+  S->hub_expand = 0;  // counted by hub_select_kernel
+  S->hub_kept = 0;
+  S->hub_min_next = 0xffffffffu;
+  S->hub_limit = limit;
 }
 
 // Closes the previous step, then chooses this step's mode and clears the
@@ -224,14 +249,18 @@ __global__ void step_kernel(SsspState* S, uint32_t defer, uint32_t hub_span) {
   S->round += 1;
   S->near_count[S->q ^ 1] = 0;
   if (!defer) S->hub_count = 0;  // hubs of a relax step are expanded in that step
+  const uint32_t span_limit =
+      (hub_span == 0 || S->hub_min >= 0xffffffffu - hub_span) ? 0xffffffffu : S->hub_min + (hub_span - 1u);
   if (S->near_count[S->q] > 0) {
     S->mode = kRelax;
+    // With hub slices, the pending hubs at or below every near distance are
+    // expanded in the same step as the near frontier: nothing left to
+    // expand can still lower them (weights are positive).
+    if (defer && hub_span != 0 && S->hub_count > 0 && S->hub_min <= S->near_min)
+      begin_hub_step(S, min(span_limit, S->near_min));
   } else if (defer && S->hub_count > 0) {
     S->mode = kHubs;
-    S->hub_expand = 0;  // counted by hub_select_kernel
-    S->hub_kept = 0;
-    S->hub_min_next = 0xffffffffu;
-    S->hub_limit = (hub_span == 0 || S->hub_min >= 0xffffffffu - hub_span) ? 0xffffffffu : S->hub_min + (hub_span - 1u);
+    begin_hub_step(S, span_limit);
   } else if (S->far_count[S->far_sel] > 0 && S->far_min != 0xffffffffu) {
     S->mode = kAdvance;
     const unsigned long long admitted = S->items - S->phase_items0;
@@ -275,6 +304,7 @@ struct RelaxOp {
   uint32_t bucket_end, round, phase, hub_gen;  // bucket_end: last bucket of the phase
   uint32_t far_min = 0xffffffffu;  // this lane's smallest far bucket improved into
   uint32_t hub_min = 0xffffffffu;  // this lane's smallest distance of a hub it listed
+  uint32_t near_min = 0xffffffffu;  // this lane's smallest distance improved into the near range
   unsigned long long edges = 0, items = 0;
   // Stale-item filter (the reference's test at SSSP.cpp:138): a near item is
   // expanded only if its distance changed since it was last admitted.  A
@@ -306,6 +336,7 @@ struct RelaxOp {
         if (nd < old) {
           const uint32_t nb = nd >> B.shift;
           if (nb <= bucket_end) {
+            near_min = min(near_min, nd);
             to_near = atomicExch(&B.stamp[v], round) != round;
           } else {
             to_far = atomicExch(&B.farmark[v], phase) != phase;
@@ -326,11 +357,12 @@ struct RelaxOp {
   // once per wavefront.
   __device__ __forceinline__ void finish() {
     flush(true);
-    const uint32_t fm = wave_min(far_min), hm = wave_min(hub_min);
+    const uint32_t fm = wave_min(far_min), hm = wave_min(hub_min), nm = wave_min(near_min);
     const unsigned long long ed = wave_sum(edges), it = wave_sum(items);
     if (lane_id() == 0) {
       if (fm != 0xffffffffu) atomicMin(&B.S->far_min, fm);
       if (hm != 0xffffffffu) atomicMin(&B.S->hub_min, hm);
+      if (nm != 0xffffffffu && B.hub_span) atomicMin(&B.S->near_min_next, nm);  // read only with hub slices
       if (ed) atomicAdd(&B.S->edges, ed);  // hub steps count edges in every workgroup, items in one
       if (it) atomicAdd(&B.S->items, it);
     }
@@ -370,7 +402,7 @@ __device__ __forceinline__ void relax_step(const Bufs& B, LdsQueue<kQCap>& qn, L
 // hub_offsets ran over the same count.
 __global__ void __launch_bounds__(kBlock) hubs_kernel(Bufs B) {
   SsspState* S = B.S;
-  const unsigned hubs = B.defer ? (S->mode == kHubs ? S->hub_expand : 0u) : (S->mode == kRelax ? S->hub_count : 0u);
+  const unsigned hubs = B.defer ? (S->hub_step ? S->hub_expand : 0u) : (S->mode == kRelax ? S->hub_count : 0u);
   if (hubs == 0) return;
   __shared__ LdsQueue<kQCap> qn, qf;
   lq_init(qn);
@@ -393,15 +425,16 @@ __device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& q
   const unsigned cnt = S->far_count[fs];
   const uint32_t* farq = B.farq[fs];
   const uint32_t round = S->round, phase = S->phase;
-  uint32_t kept_min = 0xffffffffu;
+  uint32_t kept_min = 0xffffffffu, near_min = 0xffffffffu;
   for (unsigned b0 = blockIdx.x * kBlock; b0 < cnt; b0 += gridDim.x * kBlock) {
     const unsigned i = b0 + threadIdx.x;
     bool to_near = false, keep = false;
     uint32_t v = 0;
     if (i < cnt) {
       v = farq[i];
-      const uint32_t b = dist[v] >> B.shift;
+      const uint32_t dv = dist[v], b = dv >> B.shift;
       if (b >= next && b <= last) {
+        near_min = min(near_min, dv);
         to_near = atomicExch(&B.stamp[v], round) != round;
       } else if (b > last) {
         keep = atomicExch(&B.farmark[v], phase) != phase;
@@ -416,7 +449,9 @@ __device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& q
   lq_flush(qn, B.nearq[q ^ 1], &S->near_count[q ^ 1], B.cap, &S->overflow, true);
   lq_flush(qk, B.farq[fs ^ 1], &S->far_count[fs ^ 1], B.cap, &S->overflow, true);
   kept_min = wave_min(kept_min);
+  near_min = wave_min(near_min);
   if (lane_id() == 0 && kept_min != 0xffffffffu) atomicMin(&S->far_min, kept_min);
+  if (lane_id() == 0 && near_min != 0xffffffffu && B.hub_span) atomicMin(&S->near_min_next, near_min);
 }
 
 // Hub step, first part: the pending hubs at or below hub_limit go to the
@@ -426,7 +461,7 @@ __device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& q
 // becomes the next step's hub_min.
 __global__ void __launch_bounds__(kBlock) hub_select_kernel(Bufs B) {
   SsspState* S = B.S;
-  if (S->mode != kHubs) return;
+  if (!S->hub_step) return;
   const uint32_t* __restrict__ dist = S->dist_out;
   const unsigned cnt = min(S->hub_count, B.cap), hq = S->hq, gen = S->hub_gen + 1;
   const uint32_t limit = S->hub_limit;

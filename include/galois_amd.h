@@ -139,8 +139,10 @@ void gx_set_profiling(int on);
  *                          within 2^(r - 1) of the smallest pending hub
  *                          distance, so that hubs go out nearly in distance
  *                          order and most are expanded once; 0 = off;
- *                          default 2 (two distances per hub step) from 2^24
- *                          vertices on, else 0; range [0, 32]
+ *                          hub selection joins a relax step whenever no near
+ *                          item is closer than the hubs it takes; default 1
+ *                          (one distance at a time) from 2^24 vertices on,
+ *                          else 0; range [0, 32]
  *   "sssp.batch_width"     gx_sssp_batched: buckets per phase, shared by
  *                          the 64 sources of a wavefront; 0 = adaptive as
  *                          gx_sssp_delta's phases; default 64
