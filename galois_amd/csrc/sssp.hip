@@ -459,16 +459,22 @@ __device__ __forceinline__ void far_split_step(const Bufs& B, LdsQueue<kQCap>& q
 // generation's mark (the generation advances when the step closes), so the
 // admission filter does not list them twice, and their smallest distance
 // becomes the next step's hub_min.
+// The two lists are staged in LDS and moved out with one global atomic per
+// flush, not one per wavefront: the pending list can hold millions of hubs,
+// and every selecting step passes over it.
 __global__ void __launch_bounds__(kBlock) hub_select_kernel(Bufs B) {
   SsspState* S = B.S;
   if (!S->hub_step) return;
+  __shared__ LdsQueue<kQCap> qs, qk;
+  lq_init(qs);
+  lq_init(qk);
   const uint32_t* __restrict__ dist = S->dist_out;
   const unsigned cnt = min(S->hub_count, B.cap), hq = S->hq, gen = S->hub_gen + 1;
   const uint32_t limit = S->hub_limit;
   const uint32_t* pending = B.hubq[hq];
   uint32_t kept_min = 0xffffffffu;
-  for (unsigned b0 = blockIdx.x * kBlock + (threadIdx.x & ~(kWave - 1u)); b0 < cnt; b0 += gridDim.x * kBlock) {
-    const unsigned i = b0 + lane_id();
+  for (unsigned b0 = blockIdx.x * kBlock; b0 < cnt; b0 += gridDim.x * kBlock) {  // workgroup-uniform
+    const unsigned i = b0 + threadIdx.x;
     bool sel = false, keep = false;
     uint32_t h = 0;
     if (i < cnt) {
@@ -481,9 +487,13 @@ __global__ void __launch_bounds__(kBlock) hub_select_kernel(Bufs B) {
         kept_min = min(kept_min, d);
       }
     }
-    wave_append<uint32_t>(sel, h, B.hubx, &S->hub_expand, B.cap, &S->overflow);
-    wave_append<uint32_t>(keep, h, B.hubq[hq ^ 1], &S->hub_kept, B.cap, &S->overflow);
+    lq_push(qs, sel, h);
+    lq_push(qk, keep, h);
+    lq_flush(qs, B.hubx, &S->hub_expand, B.cap, &S->overflow, false);
+    lq_flush(qk, B.hubq[hq ^ 1], &S->hub_kept, B.cap, &S->overflow, false);
   }
+  lq_flush(qs, B.hubx, &S->hub_expand, B.cap, &S->overflow, true);
+  lq_flush(qk, B.hubq[hq ^ 1], &S->hub_kept, B.cap, &S->overflow, true);
   kept_min = wave_min(kept_min);
   if (lane_id() == 0 && kept_min != 0xffffffffu) atomicMin(&S->hub_min_next, kept_min);
 }
