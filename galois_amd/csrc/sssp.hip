@@ -355,16 +355,40 @@ struct RelaxOp {
   __device__ __forceinline__ void after_step() { flush(false); }
   // End of a launch: queues flushed, far minimum and filter counts reduced
   // once per wavefront.
+  // Workgroup-uniform.  The minima and counts are reduced over the
+  // workgroup first, so the state words (one cache line) take one atomic
+  // per workgroup rather than one per wavefront.
   __device__ __forceinline__ void finish() {
     flush(true);
+    constexpr int kW = kBlock / kWave;
+    __shared__ uint32_t s_min[3][kW];
+    __shared__ unsigned long long s_sum[2][kW];
     const uint32_t fm = wave_min(far_min), hm = wave_min(hub_min), nm = wave_min(near_min);
     const unsigned long long ed = wave_sum(edges), it = wave_sum(items);
+    const unsigned w = threadIdx.x / kWave;
     if (lane_id() == 0) {
-      if (fm != 0xffffffffu) atomicMin(&B.S->far_min, fm);
-      if (hm != 0xffffffffu) atomicMin(&B.S->hub_min, hm);
-      if (nm != 0xffffffffu && B.hub_span) atomicMin(&B.S->near_min_next, nm);  // read only with hub slices
-      if (ed) atomicAdd(&B.S->edges, ed);  // hub steps count edges in every workgroup, items in one
-      if (it) atomicAdd(&B.S->items, it);
+      s_min[0][w] = fm;
+      s_min[1][w] = hm;
+      s_min[2][w] = nm;
+      s_sum[0][w] = ed;
+      s_sum[1][w] = it;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      uint32_t f = 0xffffffffu, h = 0xffffffffu, nn = 0xffffffffu;
+      unsigned long long e = 0, t = 0;
+      for (int i = 0; i < kW; ++i) {
+        f = min(f, s_min[0][i]);
+        h = min(h, s_min[1][i]);
+        nn = min(nn, s_min[2][i]);
+        e += s_sum[0][i];
+        t += s_sum[1][i];
+      }
+      if (f != 0xffffffffu) atomicMin(&B.S->far_min, f);
+      if (h != 0xffffffffu) atomicMin(&B.S->hub_min, h);
+      if (nn != 0xffffffffu && B.hub_span) atomicMin(&B.S->near_min_next, nn);  // read only with hub slices
+      if (e) atomicAdd(&B.S->edges, e);  // hub steps count edges in every workgroup, items in one
+      if (t) atomicAdd(&B.S->items, t);
     }
   }
 };
