@@ -196,10 +196,19 @@ __device__ __forceinline__ TdOp make_td(const Bufs& B, LdsQueue<kQCap>* sq) {
   return TdOp{B.row, B.col, S->level_out, S->parent_out, B.q[nq], &S->count[nq], B.cap, S, sq, S->level, 0ull, B.vis};
 }
 
+// Workgroup-uniform: the scout count is summed over the workgroup, so the
+// state word takes one atomic per workgroup rather than one per wavefront.
 __device__ __forceinline__ void td_finish(TdOp& op) {
   lq_flush(*op.sq, op.qout, op.qcount, op.cap, &op.S->overflow, true);
-  unsigned long long sc = wave_sum(op.scout);
-  if (lane_id() == 0 && sc) atomicAdd(&op.S->scout, sc);
+  __shared__ unsigned long long s_sc[kBlock / kWave];
+  const unsigned long long sc = wave_sum(op.scout);
+  if (lane_id() == 0) s_sc[threadIdx.x / kWave] = sc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / kWave; ++i) t += s_sc[i];
+    if (t) atomicAdd(&op.S->scout, t);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) td_expand_kernel(Bufs B) {

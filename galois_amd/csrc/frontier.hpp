@@ -32,7 +32,8 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
   __shared__ int32_t s_start[BLOCK];
   __shared__ uint32_t s_off[BLOCK];
   __shared__ uint32_t s_src[BLOCK];
-  __shared__ uint32_t s_scan[BLOCK / kWave + 1];
+  __shared__ uint64_t s_scan[BLOCK / kWave + 1];
+  __shared__ unsigned s_hub_base;
   const unsigned tid = threadIdx.x;
   for (unsigned base = blockIdx.x * BLOCK; base < qcount; base += gridDim.x * BLOCK) {
     const unsigned i = base + tid;
@@ -52,13 +53,27 @@ __device__ __forceinline__ void expand_queue(const int32_t* __restrict__ row, co
         }
       }
     }
-    if (hubq) wave_append<uint32_t>(hub, v, hubq, hub_count, cap, overflow);
-    uint32_t total;
-    const uint32_t excl = block_exclusive_scan<BLOCK>(deg, s_scan, &total);
+    // One scan gives both the edge offsets (low 40 bits) and the hub slots
+    // (high bits), so the hub list takes one global atomic per workgroup
+    // batch instead of one per wavefront.
+    uint64_t total64;
+    const uint64_t ex64 = block_exclusive_scan<BLOCK>(static_cast<uint64_t>(deg) | (static_cast<uint64_t>(hub) << 40),
+                                                      s_scan, &total64);
+    constexpr uint64_t kLow = (uint64_t(1) << 40) - 1;
+    const uint32_t excl = static_cast<uint32_t>(ex64 & kLow), total = static_cast<uint32_t>(total64 & kLow);
+    const uint32_t hub_total = static_cast<uint32_t>(total64 >> 40);  // workgroup-uniform
+    if (hubq && hub_total && tid == 0) s_hub_base = atomicAdd(hub_count, hub_total);
     s_start[tid] = start;
     s_off[tid] = excl;
     s_src[tid] = v;
     __syncthreads();
+    if (hubq && hub) {
+      const unsigned slot = s_hub_base + static_cast<uint32_t>(ex64 >> 40);
+      if (slot < cap)
+        hubq[slot] = v;
+      else
+        atomicOr(overflow, 1u);
+    }
     for (uint32_t e0 = 0; e0 < total; e0 += BLOCK) {
       const uint32_t e = e0 + tid;
       const bool valid = e < total;

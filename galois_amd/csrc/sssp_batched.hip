@@ -220,15 +220,36 @@ __global__ void __launch_bounds__(kBlock) sb_relax_kernel(BBufs B) {
       }
     }
   }
+  // Reduced over the workgroup first: one atomic per state word per
+  // workgroup rather than per wavefront.
+  constexpr int kW = kBlock / kWave;
+  __shared__ unsigned s_min[kW];
+  __shared__ unsigned long long s_sum[3][kW];
   far_min = wave_min(far_min);
   near = wave_sum(near);
   pushed = wave_sum(pushed);
   edges = wave_sum(edges);
+  const unsigned w = threadIdx.x / kWave;
   if (lane == 0) {
-    if (far_min != 0xffffffffu) atomicMin(&S->far_min, far_min);
-    if (near) atomicAdd(&S->near, near);
-    if (pushed) atomicAdd(&S->pushed, pushed);
-    if (edges) atomicAdd(&S->edges, edges);
+    s_min[w] = far_min;
+    s_sum[0][w] = near;
+    s_sum[1][w] = pushed;
+    s_sum[2][w] = edges;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned fm = 0xffffffffu;
+    unsigned long long nr = 0, pu = 0, ed = 0;
+    for (int i = 0; i < kW; ++i) {
+      fm = min(fm, s_min[i]);
+      nr += s_sum[0][i];
+      pu += s_sum[1][i];
+      ed += s_sum[2][i];
+    }
+    if (fm != 0xffffffffu) atomicMin(&S->far_min, fm);
+    if (nr) atomicAdd(&S->near, nr);
+    if (pu) atomicAdd(&S->pushed, pu);
+    if (ed) atomicAdd(&S->edges, ed);
   }
 }
 
