@@ -394,6 +394,21 @@ struct FwdState {
   unsigned done;             // set when a level finds nothing new; deepest = level
 };
 
+// Workgroup-uniform: adds the per-wavefront counts `v` (held by lane 0 of
+// each wavefront) into *out with one atomic per workgroup; the level
+// counters are single words that every wavefront of the grid would
+// otherwise hit once per level.
+__device__ __forceinline__ void block_add(unsigned long long v, unsigned long long* out) {
+  __shared__ unsigned long long s_v[kBlock / kWave];
+  if (lane_id() == 0) s_v[threadIdx.x / kWave] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long t = 0;
+    for (int i = 0; i < kBlock / kWave; ++i) t += s_v[i];
+    if (t) atomicAdd(out, t);
+  }
+}
+
 // Start of a level: clears the counters and the next level's masks.
 __global__ void bcb_level_begin_kernel(FwdState* S, uint64_t* const* __restrict__ F, int64_t total) {
   if (S->done) return;
@@ -475,7 +490,7 @@ __global__ void __launch_bounds__(kBlock) bcb_forward_kernel(
     }
     if (lane == 0) found += __popcll(acc_mask);
   }
-  if (lane == 0 && found) atomicAdd(newcount, found);
+  block_add(found, newcount);
 }
 
 // visited |= new frontier.  Counting of new (vertex, source) pairs is done
@@ -495,8 +510,7 @@ __global__ void __launch_bounds__(kBlock) bcb_visit_kernel(uint64_t* const* __re
       cnt += __popcll(f);
     }
   }
-  cnt = wave_sum(cnt);
-  if (lane_id() == 0 && cnt) atomicAdd(exact, cnt);
+  block_add(wave_sum(cnt), exact);
 }
 
 // Backward pull for level L (successors at L+1).  Whole-adjacency chunks
