@@ -14,8 +14,9 @@
 // Design (gfx950): the dead frontier is expanded with the load-balanced
 // workgroup expansion of frontier.hpp (hubs spread over the grid); each
 // decrement is one atomicSub on the degree word, and the `old == k` winner
-// appends the neighbour to the next frontier with a wavefront-aggregated
-// append.
+// appends the neighbour to the next frontier through the workgroup's LDS
+// queue.  On the degree-ordered copy the decrements of the hub ids are
+// counted in LDS first (kHot below).
 #include <algorithm>
 
 #include "frontier.hpp"
@@ -57,6 +58,14 @@ __global__ void kcore_reset_kernel(KcoreCounters* c) {
 }
 
 constexpr int kQCap = 2048;
+// On the degree-ordered copy the hubs are the smallest ids, and in the first
+// round (every vertex below k dies at once) most decrements land on them: a
+// hub's degree word then takes millions of returning atomics one after
+// another.  Decrements of ids below kHot are counted per workgroup in LDS
+// instead and applied with one atomicSub per counter at the end of the
+// launch; the atomic that takes a word from k or above to below k finds
+// the death, so every vertex still dies exactly once.
+constexpr uint32_t kHot = 4096;
 constexpr int kRoundsPerGraph = 4;
 constexpr int kRoundsFirst = 2;  // first replay: many graphs need one round (plus the empty one)
 
@@ -94,13 +103,18 @@ struct PeelOp {
   uint32_t k;
   unsigned out;
   unsigned cap;
-  uint32_t hub;  // frontier.hub_degree
+  uint32_t hub;   // frontier.hub_degree
+  uint32_t* hot;  // LDS counters of the ids below nhot (nhot = 0: none)
+  uint32_t nhot;
   __device__ __forceinline__ void operator()(bool valid, uint32_t, int32_t e) {
     bool dies = false;
     uint32_t v = 0;
     if (valid) {
       v = static_cast<uint32_t>(col[e]);
-      dies = atomicSub(&degree[v], 1u) == k;
+      if (v < nhot)
+        atomicAdd(&hot[v], 1u);
+      else
+        dies = atomicSub(&degree[v], 1u) == k;
     }
     lq_push(*sq, dies, v);
   }
@@ -124,30 +138,60 @@ struct KcoreBufs {
   uint32_t hub;
 };
 
-__device__ __forceinline__ PeelOp make_peel(const KcoreBufs& B, LdsQueue<kQCap>* sq) {
+__device__ __forceinline__ PeelOp make_peel(const KcoreBufs& B, LdsQueue<kQCap>* sq, uint32_t* hot, uint32_t nhot) {
   const unsigned out = B.c->in ^ 1u;
-  return PeelOp{B.col, B.degree, out ? B.q1 : B.q0, B.c, sq, B.k, out, B.cap, B.hub};
+  return PeelOp{B.col, B.degree, out ? B.q1 : B.q0, B.c, sq, B.k, out, B.cap, B.hub, hot, nhot};
 }
 
+// Workgroup-uniform.  Clears the LDS counters of the hot ids.
+__device__ __forceinline__ void hot_clear(uint32_t* hot) {
+  for (uint32_t i = threadIdx.x; i < kHot; i += kBlock) hot[i] = 0;
+  __syncthreads();
+}
+
+// Workgroup-uniform.  Applies the workgroup's counted decrements of the hot
+// ids (ids at or above n are never counted).
+__device__ __forceinline__ void hot_apply(PeelOp& op) {
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < kHot; i += kBlock) {
+    const uint32_t cnt = op.hot[i];
+    bool dies = false;
+    if (cnt) {
+      const uint32_t old = atomicSub(&op.degree[i], cnt);
+      dies = old >= op.k && old - cnt < op.k;
+    }
+    lq_push(*op.sq, dies, i);
+    op.flush(false);
+  }
+}
+
+template <bool kAgg>
 __global__ void __launch_bounds__(kBlock) kcore_peel_kernel(KcoreBufs B) {
   KcoreCounters* c = B.c;
   if (!c->active) return;
   __shared__ LdsQueue<kQCap> sq;
+  __shared__ uint32_t hot[kAgg ? kHot : 1];
   lq_init(sq);
-  PeelOp op = make_peel(B, &sq);
+  if constexpr (kAgg) hot_clear(hot);
+  PeelOp op = make_peel(B, &sq, hot, kAgg ? kHot : 0u);
   const unsigned in = c->in;
   expand_queue<kBlock>(B.row, in ? B.q1 : B.q0, c->count[in], B.hub, B.hubq, &c->hub_count, B.cap, &c->overflow,
                        op);
+  if constexpr (kAgg) hot_apply(op);
   op.flush(true);
 }
 
+template <bool kAgg>
 __global__ void __launch_bounds__(kBlock) kcore_hubs_kernel(KcoreBufs B) {
   KcoreCounters* c = B.c;
   if (!c->active || c->hub_count == 0) return;
   __shared__ LdsQueue<kQCap> sq;
+  __shared__ uint32_t hot[kAgg ? kHot : 1];
   lq_init(sq);
-  PeelOp op = make_peel(B, &sq);
+  if constexpr (kAgg) hot_clear(hot);
+  PeelOp op = make_peel(B, &sq, hot, kAgg ? kHot : 0u);
   expand_hubs<kBlock>(B.row, B.hubq, c->hub_count, B.cap, B.hub_off, op);
+  if constexpr (kAgg) hot_apply(op);
   op.flush(true);
 }
 
@@ -247,7 +291,7 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.degree, (const void*)B.q0,
                         (const void*)B.q1, (const void*)B.hubq, (const void*)B.hub_off, (const void*)B.c})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
-  key = mix_key(mix_key(mix_key(mix_key(key, k), cap), B.hub), hubs ? 1u : 0u);
+  key = mix_key(mix_key(mix_key(mix_key(mix_key(key, k), cap), B.hub), hubs ? 1u : 0u), relabel ? 1u : 0u);
   uint32_t replays = 0, slots = 0;
   for (;;) {
     const int nr = replays == 0 ? kRoundsFirst : kRoundsPerGraph;
@@ -255,10 +299,18 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
     g->replay(replays == 0 ? g->kcore_head : g->kcore_batch, key, [&](hipStream_t cs) {
       kcore_round_begin_kernel<<<1, 64, 0, cs>>>(c);
       for (int i = 0; i < nr; ++i) {
-        kcore_peel_kernel<<<256 * 8, kBlock, 0, cs>>>(B);
+        // Hot-id counting on the degree-ordered copy, whose smallest ids are
+        // the hubs.
+        if (relabel)
+          kcore_peel_kernel<true><<<256 * 8, kBlock, 0, cs>>>(B);
+        else
+          kcore_peel_kernel<false><<<256 * 8, kBlock, 0, cs>>>(B);
         if (hubs) {
           launch_hub_offsets<kBlock>(B.row, B.hubq, &c->hub_count, cap, B.hub_off, cs);
-          kcore_hubs_kernel<<<256 * 8, kBlock, 0, cs>>>(B);
+          if (relabel)
+            kcore_hubs_kernel<true><<<256 * 8, kBlock, 0, cs>>>(B);
+          else
+            kcore_hubs_kernel<false><<<256 * 8, kBlock, 0, cs>>>(B);
         }
         if (i + 1 < nr)
           kcore_round_turn_kernel<<<1, 64, 0, cs>>>(c);
