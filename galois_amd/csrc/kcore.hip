@@ -58,13 +58,15 @@ __global__ void kcore_reset_kernel(KcoreCounters* c) {
 }
 
 constexpr int kQCap = 2048;
-// On the degree-ordered copy the hubs are the smallest ids, and in the first
-// round (every vertex below k dies at once) most decrements land on them: a
-// hub's degree word then takes millions of returning atomics one after
-// another.  Decrements of ids below kHot are counted per workgroup in LDS
-// instead and applied with one atomicSub per counter at the end of the
-// launch; the atomic that takes a word from k or above to below k finds
-// the death, so every vertex still dies exactly once.
+// On the degree-ordered copy the hubs are the smallest ids (and on graphs
+// whose generator gives small ids the high degrees, such as RMAT without a
+// relabelling, too), and in the first round (every vertex below k dies at
+// once) most decrements land on them: a hub's degree word then takes
+// millions of returning atomics one after another.  Decrements of ids below
+// kHot are counted per workgroup in LDS instead and applied with one
+// atomicSub per counter at the end of the launch; the atomic that takes a
+// word from k or above to below k finds the death, so every vertex still
+// dies exactly once.
 constexpr uint32_t kHot = 4096;
 constexpr int kRoundsPerGraph = 4;
 constexpr int kRoundsFirst = 2;  // first replay: many graphs need one round (plus the empty one)
@@ -284,6 +286,16 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
     B.col = g->relab.col.as<int32_t>();
     B.degree = g->scratch<uint32_t>(WS_V5, cap);
   }
+  // LDS counting of the ids below kHot: on the degree-ordered copy, or when
+  // those ids hold at least 8 times their share of the adjacency entries.
+  bool agg = relabel;
+  if (!agg && hubs && n > static_cast<int64_t>(kHot)) {
+    int32_t low = 0;
+    GX_HIP(hipMemcpyAsync(&low, B.row + kHot, sizeof(low), hipMemcpyDeviceToHost, s));
+    GX_HIP(hipStreamSynchronize(s));
+    agg = static_cast<double>(low) * static_cast<double>(n) >=
+          8.0 * static_cast<double>(kHot) * static_cast<double>(g->m);
+  }
   kcore_reset_kernel<<<1, 64, 0, s>>>(c);
   kcore_init_kernel<<<grid_for(n, kBlock, 4096), kBlock, 0, s>>>(B.row, B.degree, n, k, q[0], c, cap);
   GX_LAUNCHED("kcore_init_kernel");
@@ -291,7 +303,7 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
   for (const void* p : {(const void*)B.row, (const void*)B.col, (const void*)B.degree, (const void*)B.q0,
                         (const void*)B.q1, (const void*)B.hubq, (const void*)B.hub_off, (const void*)B.c})
     key = mix_key(key, reinterpret_cast<uintptr_t>(p));
-  key = mix_key(mix_key(mix_key(mix_key(mix_key(key, k), cap), B.hub), hubs ? 1u : 0u), relabel ? 1u : 0u);
+  key = mix_key(mix_key(mix_key(mix_key(mix_key(key, k), cap), B.hub), hubs ? 1u : 0u), agg ? 1u : 0u);
   uint32_t replays = 0, slots = 0;
   for (;;) {
     const int nr = replays == 0 ? kRoundsFirst : kRoundsPerGraph;
@@ -299,15 +311,13 @@ extern "C" gx_status gx_kcore(gx_graph* g, uint32_t k, uint32_t* degree, uint8_t
     g->replay(replays == 0 ? g->kcore_head : g->kcore_batch, key, [&](hipStream_t cs) {
       kcore_round_begin_kernel<<<1, 64, 0, cs>>>(c);
       for (int i = 0; i < nr; ++i) {
-        // Hot-id counting on the degree-ordered copy, whose smallest ids are
-        // the hubs.
-        if (relabel)
+        if (agg)
           kcore_peel_kernel<true><<<256 * 8, kBlock, 0, cs>>>(B);
         else
           kcore_peel_kernel<false><<<256 * 8, kBlock, 0, cs>>>(B);
         if (hubs) {
           launch_hub_offsets<kBlock>(B.row, B.hubq, &c->hub_count, cap, B.hub_off, cs);
-          if (relabel)
+          if (agg)
             kcore_hubs_kernel<true><<<256 * 8, kBlock, 0, cs>>>(B);
           else
             kcore_hubs_kernel<false><<<256 * 8, kBlock, 0, cs>>>(B);
