@@ -318,6 +318,13 @@ __global__ void pr_delta_kernel(uint32_t* __restrict__ value, const uint32_t* __
 // PageRank (pagerank_push.cpp:290-307): a node with a positive delta (counted
 // active) clears it and adds it to every out-neighbour's residual (float
 // atomics, bitset residual).  The delta is parked in `push` for the edges.
+// Pushes to the local ids below kPrHot (on generator graphs and degree-ordered
+// ones, the hubs, whose residual words otherwise take one atomic per
+// in-neighbour one after another) are summed per workgroup in LDS first and
+// added with one atomic per id at the end of the launch; every push is
+// positive, so an id was pushed to exactly when its LDS sum is nonzero.
+constexpr uint32_t kPrHot = 4096;
+
 struct PrPushOp {
   const int32_t* __restrict__ col;
   uint32_t* delta;
@@ -326,6 +333,7 @@ struct PrPushOp {
   unsigned long long* bits;
   PartCounters* c;
   unsigned active = 0;
+  float* hot = nullptr;  // LDS sums of the pushes to ids below kPrHot
   __device__ __forceinline__ bool admit(uint32_t v) {
     const float d = as_f(delta[v]);
     if (!(d > 0.0f)) return false;
@@ -337,11 +345,36 @@ struct PrPushOp {
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     if (!valid) return;
     const uint32_t v = static_cast<uint32_t>(col[e]);
+    if (v < kPrHot) {
+      atomicAdd(&hot[v], push[u]);
+      return;
+    }
     atomicAdd(&res[v], push[u]);
     set_bit(bits, v);
   }
   __device__ __forceinline__ void after_step() {}
   __device__ __forceinline__ void finish() { flush_counts(c, active, 0ull); }
+  // Workgroup-uniform.  Ids at or above n are never pushed to.
+  __device__ __forceinline__ void flush_hot() {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kPrHot; i += blockDim.x) {
+      const float sum = hot[i];
+      if (sum != 0.0f) {
+        atomicAdd(&res[i], sum);
+        set_bit(bits, i);
+      }
+    }
+  }
+};
+
+// Operators whose pushes to the hot ids are summed in LDS (flush_hot).
+template <class Op>
+struct HotSums {
+  static constexpr bool value = false;
+};
+template <>
+struct HotSums<PrPushOp> {
+  static constexpr bool value = true;
 };
 
 // PageRankSanity (pagerank_push.cpp:415-429): value and residual extremes and
@@ -597,7 +630,16 @@ template <class Op>
 __global__ void __launch_bounds__(kBlock) range_expand_kernel(const int32_t* __restrict__ row, uint32_t begin,
                                                               uint32_t count, uint32_t hub, uint32_t* hubq,
                                                               PartCounters* c, unsigned cap, Op op) {
-  expand_queue<kBlock, Op, true>(row, nullptr, count, hub, hubq, &c->hub_count, cap, &c->overflow, op, begin);
+  if constexpr (HotSums<Op>::value) {
+    __shared__ float hot[kPrHot];
+    for (uint32_t i = threadIdx.x; i < kPrHot; i += kBlock) hot[i] = 0.0f;
+    __syncthreads();
+    op.hot = hot;
+    expand_queue<kBlock, Op, true>(row, nullptr, count, hub, hubq, &c->hub_count, cap, &c->overflow, op, begin);
+    op.flush_hot();
+  } else {
+    expand_queue<kBlock, Op, true>(row, nullptr, count, hub, hubq, &c->hub_count, cap, &c->overflow, op, begin);
+  }
   op.finish();
 }
 
@@ -605,7 +647,16 @@ template <class Op>
 __global__ void __launch_bounds__(kBlock) range_hubs_kernel(const int32_t* __restrict__ row,
                                                             const uint32_t* __restrict__ hubq, PartCounters* c,
                                                             unsigned cap, const uint32_t* __restrict__ off, Op op) {
-  expand_hubs<kBlock>(row, hubq, c->hub_count, cap, off, op);
+  if constexpr (HotSums<Op>::value) {
+    __shared__ float hot[kPrHot];
+    for (uint32_t i = threadIdx.x; i < kPrHot; i += kBlock) hot[i] = 0.0f;
+    __syncthreads();
+    op.hot = hot;
+    expand_hubs<kBlock>(row, hubq, c->hub_count, cap, off, op);
+    op.flush_hot();
+  } else {
+    expand_hubs<kBlock>(row, hubq, c->hub_count, cap, off, op);
+  }
   op.finish();
 }
 
