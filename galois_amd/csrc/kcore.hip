@@ -72,7 +72,9 @@ constexpr int kRoundsPerGraph = 4;
 constexpr int kRoundsFirst = 2;  // first replay: many graphs need one round (plus the empty one)
 
 // degree[v] = out-degree; vertices below k form the first dead frontier
-// (staged in LDS, one global append per workgroup flush).
+// (staged in LDS, one global append per workgroup flush).  Vertices without
+// edges are dead from the start but decrement no one, so they are left out
+// of it (on RMAT graphs they are a large share of the vertices).
 __global__ void __launch_bounds__(kBlock) kcore_init_kernel(const int32_t* __restrict__ row,
                                                             uint32_t* __restrict__ degree, int64_t n, uint32_t k,
                                                             uint32_t* __restrict__ q, KcoreCounters* c,
@@ -86,7 +88,7 @@ __global__ void __launch_bounds__(kBlock) kcore_init_kernel(const int32_t* __res
     if (v < n) {
       const uint32_t d = static_cast<uint32_t>(row[v + 1] - row[v]);
       degree[v] = d;
-      dead = d < k;
+      dead = d < k && d > 0;
     }
     lq_push(sq, dead, static_cast<uint32_t>(v));
     lq_flush(sq, q, &c->count[0], cap, &c->overflow, false);
