@@ -318,12 +318,12 @@ __global__ void pr_delta_kernel(uint32_t* __restrict__ value, const uint32_t* __
 // PageRank (pagerank_push.cpp:290-307): a node with a positive delta (counted
 // active) clears it and adds it to every out-neighbour's residual (float
 // atomics, bitset residual).  The delta is parked in `push` for the edges.
-// Pushes to the local ids below kPrHot (on generator graphs and degree-ordered
+// Pushes to the local ids below kHotIds (on generator graphs and degree-ordered
 // ones, the hubs, whose residual words otherwise take one atomic per
 // in-neighbour one after another) are summed per workgroup in LDS first and
 // added with one atomic per id at the end of the launch; every push is
 // positive, so an id was pushed to exactly when its LDS sum is nonzero.
-constexpr uint32_t kPrHot = 4096;
+constexpr uint32_t kHotIds = 4096;
 
 struct PrPushOp {
   const int32_t* __restrict__ col;
@@ -333,7 +333,7 @@ struct PrPushOp {
   unsigned long long* bits;
   PartCounters* c;
   unsigned active = 0;
-  float* hot = nullptr;  // LDS sums of the pushes to ids below kPrHot
+  float* hot = nullptr;  // LDS sums of the pushes to ids below kHotIds
   __device__ __forceinline__ bool admit(uint32_t v) {
     const float d = as_f(delta[v]);
     if (!(d > 0.0f)) return false;
@@ -345,7 +345,7 @@ struct PrPushOp {
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
     if (!valid) return;
     const uint32_t v = static_cast<uint32_t>(col[e]);
-    if (v < kPrHot) {
+    if (v < kHotIds) {
       atomicAdd(&hot[v], push[u]);
       return;
     }
@@ -357,7 +357,7 @@ struct PrPushOp {
   // Workgroup-uniform.  Ids at or above n are never pushed to.
   __device__ __forceinline__ void flush_hot() {
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < kPrHot; i += blockDim.x) {
+    for (uint32_t i = threadIdx.x; i < kHotIds; i += blockDim.x) {
       const float sum = hot[i];
       if (sum != 0.0f) {
         atomicAdd(&res[i], sum);
@@ -367,15 +367,7 @@ struct PrPushOp {
   }
 };
 
-// Operators whose pushes to the hot ids are summed in LDS (flush_hot).
-template <class Op>
-struct HotSums {
-  static constexpr bool value = false;
-};
-template <>
-struct HotSums<PrPushOp> {
-  static constexpr bool value = true;
-};
+
 
 // PageRankSanity (pagerank_push.cpp:415-429): value and residual extremes and
 // sums, residuals above the tolerance.
@@ -585,6 +577,7 @@ struct KcoreOp {
   uint32_t k;
   bool step;  // KCoreStep1
   unsigned active = 0;
+  uint32_t* hot = nullptr;  // LDS counts of the pushes to ids below kHotIds
   __device__ __forceinline__ bool admit(uint32_t v) {
     if (!step) return true;
     if (flag[v] && deg[v] < k) {
@@ -597,11 +590,44 @@ struct KcoreOp {
   __device__ __forceinline__ void operator()(bool valid, uint32_t, int32_t e) {
     if (!valid) return;
     const uint32_t v = static_cast<uint32_t>(col[e]);
+    if (v < kHotIds) {  // counted in LDS, as the PageRank pushes
+      atomicAdd(&hot[v], 1u);
+      return;
+    }
     atomicAdd(&target[v], 1u);
     set_bit(bits, v);
   }
   __device__ __forceinline__ void after_step() {}
   __device__ __forceinline__ void finish() { flush_counts(c, active, 0ull); }
+  // Workgroup-uniform.  Ids at or above n are never counted.
+  __device__ __forceinline__ void flush_hot() {
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kHotIds; i += blockDim.x) {
+      const uint32_t cnt = hot[i];
+      if (cnt) {
+        atomicAdd(&target[i], cnt);
+        set_bit(bits, i);
+      }
+    }
+  }
+};
+
+// Operators whose pushes to the hot ids are summed in LDS (flush_hot), with
+// the type of their LDS sums.
+template <class Op>
+struct HotSums {
+  static constexpr bool value = false;
+  using T = uint32_t;
+};
+template <>
+struct HotSums<PrPushOp> {
+  static constexpr bool value = true;
+  using T = float;
+};
+template <>
+struct HotSums<KcoreOp> {
+  static constexpr bool value = true;
+  using T = uint32_t;
 };
 
 // KCoreStep2 (kcore_push.cpp:184-224): a live vertex loses
@@ -631,8 +657,8 @@ __global__ void __launch_bounds__(kBlock) range_expand_kernel(const int32_t* __r
                                                               uint32_t count, uint32_t hub, uint32_t* hubq,
                                                               PartCounters* c, unsigned cap, Op op) {
   if constexpr (HotSums<Op>::value) {
-    __shared__ float hot[kPrHot];
-    for (uint32_t i = threadIdx.x; i < kPrHot; i += kBlock) hot[i] = 0.0f;
+    __shared__ typename HotSums<Op>::T hot[kHotIds];
+    for (uint32_t i = threadIdx.x; i < kHotIds; i += kBlock) hot[i] = 0;
     __syncthreads();
     op.hot = hot;
     expand_queue<kBlock, Op, true>(row, nullptr, count, hub, hubq, &c->hub_count, cap, &c->overflow, op, begin);
@@ -648,8 +674,8 @@ __global__ void __launch_bounds__(kBlock) range_hubs_kernel(const int32_t* __res
                                                             const uint32_t* __restrict__ hubq, PartCounters* c,
                                                             unsigned cap, const uint32_t* __restrict__ off, Op op) {
   if constexpr (HotSums<Op>::value) {
-    __shared__ float hot[kPrHot];
-    for (uint32_t i = threadIdx.x; i < kPrHot; i += kBlock) hot[i] = 0.0f;
+    __shared__ typename HotSums<Op>::T hot[kHotIds];
+    for (uint32_t i = threadIdx.x; i < kHotIds; i += kBlock) hot[i] = 0;
     __syncthreads();
     op.hot = hot;
     expand_hubs<kBlock>(row, hubq, c->hub_count, cap, off, op);
