@@ -204,6 +204,18 @@ def kernel_table(gx, quick: bool, tuned: dict) -> dict:
             out[f"{kname}_{gname}"] = {"ms": round(st.elapsed_ms, 4), "GTEPS": round(m / st.elapsed_ms / 1e6, 3),
                                        "pct_hbm": pct(byt, st.elapsed_ms), "rounds": st.iterations, "n": n,
                                        "m": m, **extra, **cpu_row(cms, m, 5, np.array_equal(u32(res[0]), exp))}
+        if gname.startswith("rmat"):
+            # Residual push PageRank (pagerank_push.cpp, tolerance 1e-6 as the
+            # D-IrGL row) against the oracle's residual pull, the same
+            # unnormalised fixed point.
+            gx.pagerank_push(G, tol=1e-6)
+            pp = min((gx.pagerank_push(G, tol=1e-6) for _ in range(3)), key=lambda r: r[-1].elapsed_ms)
+            ref, _ = O.pagerank_residual(row, col, tol=1e-6, threads=CPU_THREADS)
+            got = pp[0].cpu().numpy()
+            out[f"pagerank_push_{gname}"] = {
+                "ms": round(pp[-1].elapsed_ms, 3), "rounds": pp[-1].iterations,
+                "max_rel_diff_vs_oracle_residual_pull": float(np.max(np.abs(got - ref) / np.maximum(np.abs(ref), 1.0)))}
+            del pp, got, ref
         # Config 4a: BC on the symmetric RMAT-22, 256 seeded sources.
         if gname.startswith("rmat"):
             nsrc = 64 if quick else 256

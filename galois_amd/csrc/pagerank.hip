@@ -755,6 +755,14 @@ __global__ void __launch_bounds__(kBlock) push_take_kernel(const uint32_t* __res
 }
 
 constexpr int kQCap = 2048;
+// Pushes to the ids below kPushHot (the hubs on generator and degree-ordered
+// graphs, whose residual words otherwise take one returning atomic per
+// in-neighbour one after another) are summed per workgroup in LDS and added
+// with one returning atomic per id at the end of the launch.  Residuals only
+// grow, so exactly one add takes a word from at most tol to at least tol,
+// whether it adds one delta or a workgroup's sum: the activation rule and
+// the set of active vertices stay those of the per-edge adds.
+constexpr uint32_t kPushHot = 4096;
 
 struct PushOp {
   const int32_t* col;
@@ -768,6 +776,7 @@ struct PushOp {
   uint32_t round;
   int out;
   unsigned cap;
+  float* hot;  // LDS sums of the pushes to ids below kPushHot
   __device__ __forceinline__ void operator()(bool valid, uint32_t u, int32_t e) {
 #pragma clang fp contract(off)
     bool act = false;
@@ -775,10 +784,35 @@ struct PushOp {
     if (valid) {
       d = static_cast<uint32_t>(col[e]);
       const float dl = delta[u];
-      const float old = atomicAdd(&residual[d], dl);
-      if (old <= tol && old + dl >= tol) act = atomicExch(&stamp[d], round) != round;
+      if (d < kPushHot) {
+        atomicAdd(&hot[d], dl);
+      } else {
+        const float old = atomicAdd(&residual[d], dl);
+        if (old <= tol && old + dl >= tol) act = atomicExch(&stamp[d], round) != round;
+      }
     }
     lq_push(*sq, act, d);
+  }
+  // Workgroup-uniform.  Clears the LDS sums.
+  __device__ __forceinline__ void hot_clear() {
+    for (uint32_t i = threadIdx.x; i < kPushHot; i += blockDim.x) hot[i] = 0.0f;
+    __syncthreads();
+  }
+  // Workgroup-uniform.  Adds the LDS sums (ids at or above n are never pushed
+  // to; every delta is positive, so a pushed id has a positive sum).
+  __device__ __forceinline__ void hot_apply() {
+#pragma clang fp contract(off)
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < kPushHot; i += blockDim.x) {
+      const float sum = hot[i];
+      bool act = false;
+      if (sum > 0.0f) {
+        const float old = atomicAdd(&residual[i], sum);
+        if (old <= tol && old + sum >= tol) act = atomicExch(&stamp[i], round) != round;
+      }
+      lq_push(*sq, act, i);
+      flush(false);
+    }
   }
   __device__ __forceinline__ void flush(bool force) {
     lq_flush(*sq, next, &c->count[out], cap, &c->overflow, force);
@@ -790,9 +824,13 @@ __global__ void __launch_bounds__(kBlock) push_spread_kernel(const int32_t* __re
                                                              const uint32_t* __restrict__ pushers, int in,
                                                              uint32_t* __restrict__ hubq, uint32_t hub) {
   __shared__ LdsQueue<kQCap> sq;
+  __shared__ float hot[kPushHot];
   lq_init(sq);
   op.sq = &sq;
+  op.hot = hot;
+  op.hot_clear();
   expand_queue<kBlock>(row, pushers, op.c->count[in], hub, hubq, &op.c->hub_count, op.cap, &op.c->overflow, op);
+  op.hot_apply();
   op.flush(true);
 }
 
@@ -801,9 +839,13 @@ __global__ void __launch_bounds__(kBlock) push_hubs_kernel(const int32_t* __rest
                                                            const uint32_t* __restrict__ hub_off) {
   if (op.c->hub_count == 0) return;
   __shared__ LdsQueue<kQCap> sq;
+  __shared__ float hot[kPushHot];
   lq_init(sq);
   op.sq = &sq;
+  op.hot = hot;
+  op.hot_clear();
   expand_hubs<kBlock>(row, hubq, op.c->hub_count, op.cap, hub_off, op);
+  op.hot_apply();
   op.flush(true);
 }
 
@@ -867,7 +909,7 @@ extern "C" gx_status gx_pagerank_push(gx_graph* g, float alpha, float tol, uint3
     push_take_kernel<<<std::min<unsigned>(kSweepGrid, grid_for(count, kBlock)), kBlock, 0, s>>>(
         active, c, 0, pr, residual, delta, g->row, alpha, tol, pushers, 1, cap);
     push_set_kernel<<<1, 64, 0, s>>>(c, 0, 0u);
-    PushOp op{g->col, delta, residual, stamp, active, c, nullptr, tol, it, 0, cap};
+    PushOp op{g->col, delta, residual, stamp, active, c, nullptr, tol, it, 0, cap, nullptr};
     push_spread_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, pushers, 1, hubq, hub_degree());
     launch_hub_offsets<kBlock>(g->row, hubq, &c->hub_count, cap, hub_off, s);
     push_hubs_kernel<<<kSweepGrid, kBlock, 0, s>>>(g->row, op, hubq, hub_off);
