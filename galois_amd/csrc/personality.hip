@@ -212,8 +212,13 @@ struct MinPushOp {
     const uint32_t v = static_cast<uint32_t>(col[e]);
     const uint32_t base = cur[u];
     const uint32_t nd = kind == kPushBfs ? base + 1u : (kind == kPushSssp ? base + w[e] : base);
-    const uint32_t prev = atomicMin(&cur[v], nd);
-    if (prev > nd) set_bit(bits, v);
+    // Labels only decrease, so a plain read at or below nd (a stale one is
+    // larger, never smaller) means the atomic could not lower the label:
+    // hubs' labels are read by many pushers and settle early.
+    if (nd < cur[v]) {
+      const uint32_t prev = atomicMin(&cur[v], nd);
+      if (prev > nd) set_bit(bits, v);
+    }
     if (!first) {
       if (kind == kPushCc)
         ++active;
