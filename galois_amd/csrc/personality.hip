@@ -452,6 +452,7 @@ struct BcForwardOp {
     if (!valid) return;
     const uint32_t v = static_cast<uint32_t>(col[e]);
     const uint32_t nd = r + 1;
+    if (len[v] < nd) return;  // reached at an earlier level (a stale read is larger, never smaller)
     const uint32_t old = atomicMin(&len[v], nd);
     if (old >= nd) {
       if (old > nd) set_bit(len_bits, v);
